@@ -1,0 +1,200 @@
+"""Benchmark: s-mu DD+DR+RR pair counting on the jackknife batch.
+
+Workload: 2M Neyman-Scott galaxies (40k parents x Poisson(50), 5 Mpc/h)
++ 10M randoms over ra 150-250, dec 0-60, z 0.2-0.6 (omega_m 0.25, h 1),
+100 jackknife regions (10x10 ra/dec grid), unit weights, s-mu 40 log bins
+(base 1.3, s < 100 Mpc/h) x 50 mu bins.  One step = DD + DR + RR counted
+with the batched FP32 + edge-recheck method in one persistent launch, inputs
+already resident in HBM.  Value = accepted (in-range, binned) pairs per
+second of step wall time.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--scale S]
+
+Prints one JSON line.  Also measures the plain FP64 kernel on the same
+workload (the north-star ratio), the FP32-VALU roofline of the counting
+kernel (26 FP32 flop per accepted s-mu pair, HIP-event timing on the
+library's stream) and the reference's OpenMP path on a bounded sample.
+"""
+import argparse
+import json
+import os
+import subprocess
+import sys
+import tempfile
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+FLOP_PER_PAIR = 26            # FP32 flop per accepted s-mu pair
+PEAK_FP32_TFLOPS = 157.3      # MI355X FP32 vector peak
+AXES = [(0.0, 100.0, 40, 1.3), (0.0, 1.0, 50, 0.0)]
+
+
+def log(*a):
+    print("[bench]", *a, file=sys.stderr, flush=True)
+
+
+def make_catalogs(scale):
+    from two_pcf_amd import lib as L
+    from two_pcf_amd import synth
+    t0 = time.time()
+    c = synth.jackknife_batch(scale=scale)
+    out = {}
+    for tag in ("data", "randoms"):
+        a = c[tag]
+        d = L.comoving_distance(c["om"], c["h"], c["z_min"], c["z_max"], a["z"])
+        x, y, z, nx, ny, nz = L.eq_to_cart(a["ra"], a["dec"], d)
+        out[tag] = dict(x=x, y=y, z=z, nx=nx, ny=ny, nz=nz, w=a["w"], jk=a["jk"],
+                        ra=a["ra"], dec=a["dec"], zz=a["z"])
+    log(f"catalogues: {out['data']['x'].size} data, {out['randoms']['x'].size} randoms "
+        f"({time.time() - t0:.1f} s)")
+    return out, c["n_jk"]
+
+
+def cpu_baseline(cats, n_jk, frac, seed=5):
+    """The reference's OpenMP s-mu path (oracle/_ref/ref_counts, compiled
+    from the reference sources) on a seeded random subsample of both
+    catalogues; rate = accepted pairs / counting seconds."""
+    ref = os.path.join(ROOT, "oracle", "_ref", "ref_counts")
+    if not os.path.exists(ref):
+        return None
+    rng = np.random.Generator(np.random.PCG64(seed))
+    nthreads = min(os.cpu_count() or 1, 16)
+    with tempfile.TemporaryDirectory() as td:
+        files = {}
+        for tag in ("data", "randoms"):
+            a = cats[tag]
+            keep = rng.random(a["x"].size) < frac
+            arr = np.stack([a["ra"][keep], a["dec"][keep], a["zz"][keep]], axis=1)
+            fn = os.path.join(td, tag + ".txt")
+            with open(fn, "w") as f:
+                for (r, d, z), j in zip(arr.tolist(), a["jk"][keep].tolist()):
+                    f.write("%.17g %.17g %.17g %d\n" % (r, d, z, j))
+            files[tag] = (fn, int(keep.sum()))
+        params = dict(data_filename=files["data"][0], data_file_type="ascii",
+                      random_filename=files["randoms"][0], random_file_type="ascii",
+                      coord_system="equatorial", ra_x_dataset_name="x", dec_y_dataset_name="y",
+                      z_z_dataset_name="z", weight_dataset_name="w", jk_dataset_name="jk",
+                      use_weights=0, n_threads=nthreads, n_jk_regions=n_jk, omega_m=0.25, h=1.0,
+                      z_min=0.0, z_max=1.0, plot_monopole=0, plot_sigma_pi=0, plot_s_mu=1,
+                      s_mu_filename="smu.txt", s_mu_output_type="ascii", s_log_base=1.3, s_min=0.0,
+                      s_max=100.0, s_n_bins=40, mu_n_bins=50)
+        pf = os.path.join(td, "params.txt")
+        with open(pf, "w") as f:
+            f.write("".join(f"{k} = {v}\n" for k, v in params.items()))
+        js = os.path.join(td, "out.json")
+        subprocess.run([ref, pf, js, "0", "0"], check=True, stdout=subprocess.DEVNULL, cwd=td)
+        with open(js) as f:
+            r = json.load(f)["s_mu"]
+    pairs = sum(r["DD"]["full"]) / 2 + sum(r["DR"]["full"]) + sum(r["RR"]["full"]) / 2
+    secs = sum(r["seconds"])
+    return dict(value=pairs / secs, unit="pairs/s", cores=nthreads, kind="reference",
+                sample=f"seeded {frac:.0%} subsample of data and randoms "
+                       f"({files['data'][1]} + {files['randoms'][1]} objects, {pairs:.3g} pairs, "
+                       f"{secs:.1f} s of counting)")
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--scale", type=float, default=1.0)
+    ap.add_argument("--plain-steps", type=int, default=1)
+    ap.add_argument("--cpu-frac", type=float, default=0.12)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--subdiv", type=int, default=0)
+    ap.add_argument("--slots", type=int, default=0)
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py measures one GPU only: this build has no multi-GPU path "
+              "(the sharding design is described in DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+
+    from two_pcf_amd import lib as L
+    cats, n_jk = make_catalogs(args.scale)
+    ctx = L.Context(0)
+    info = ctx.info()
+    log("device", info)
+    dc = {t: ctx.upload(a["x"], a["y"], a["z"], a["nx"], a["ny"], a["nz"], a["w"], a["jk"])
+          for t, a in cats.items()}
+    jobs = [dict(kind=L.S_MU, a=dc["data"], b=None, axes=AXES, n_jk=n_jk),
+            dict(kind=L.S_MU, a=dc["data"], b=dc["randoms"], axes=AXES, n_jk=n_jk),
+            dict(kind=L.S_MU, a=dc["randoms"], b=None, axes=AXES, n_jk=n_jk)]
+    tuning = {}
+    if args.subdiv:
+        tuning["cell_subdiv"] = args.subdiv
+    if args.slots:
+        tuning["max_slots"] = args.slots
+
+    ref = None
+    for i in range(args.warmup):
+        ref, t = ctx.count(jobs, method=L.FP32_RECHECK, tuning=tuning or None)
+        log(f"warmup {i}: kernel {t['kernel_ms']:.1f} ms, total {t['total_ms']:.1f} ms, "
+            f"prep {t['prep_ms']:.1f} ms, items {t['n_work_items']}, rechecked {t['n_rechecked']}")
+    kms = []
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        res, t = ctx.count(jobs, method=L.FP32_RECHECK, tuning=tuning or None)
+        kms.append(t["kernel_ms"])
+        log(f"step {i}: kernel {t['kernel_ms']:.1f} ms, total {t['total_ms']:.1f} ms")
+    wall = time.perf_counter() - t0
+    pairs = t["n_pairs"]
+    rechecked = t["n_rechecked"]
+    if ref is not None:
+        for a, b in zip(ref, res):
+            assert np.array_equal(a["full"], b["full"]) and np.array_equal(a["jk"], b["jk"]), "steps differ"
+    ms_step = 1000.0 * wall / args.steps
+    value = pairs / (wall / args.steps)
+    kernel_s = float(np.mean(kms)) / 1000.0
+
+    plain = None
+    if args.plain_steps > 0:
+        pk = []
+        for i in range(args.plain_steps):
+            pres, pt = ctx.count(jobs, method=L.PLAIN_FP64)
+            pk.append(pt["kernel_ms"])
+            log(f"plain fp64 {i}: kernel {pt['kernel_ms']:.1f} ms, total {pt['total_ms']:.1f} ms")
+        same = all(np.array_equal(a["full"], b["full"]) and np.array_equal(a["jk"], b["jk"])
+                   for a, b in zip(pres, res))
+        plain = dict(kernel_ms=float(np.mean(pk)), pairs_per_s=pairs / (float(np.mean(pk)) / 1000.0),
+                     counts_identical=bool(same))
+
+    achieved = pairs * FLOP_PER_PAIR / kernel_s / 1e12
+    line = {
+        "metric": "pairs/s s-mu DD+DR+RR: FP32+edge-recheck batched vs FP64 kernel, exact counts",
+        "value": value, "unit": "pairs/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": ms_step, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32+f64-recheck",
+        "data": "synthetic (seeded Neyman-Scott + uniform randoms)",
+        "config": {"workload": "jackknife batch: 2M clustered data + 10M randoms, 100 jk regions, "
+                               "s-mu 40 log bins (base 1.3, s<100) x 50 mu, DD+DR+RR in one launch",
+                   "n_data": int(cats["data"]["x"].size), "n_randoms": int(cats["randoms"]["x"].size),
+                   "scale": args.scale, "n_jk": n_jk, "parallelism": "single GPU",
+                   "tuning": tuning or "default"},
+        "pairs_per_step": pairs,
+        "kernel_ms": float(np.mean(kms)),
+        "kernel_pairs_per_s": pairs / kernel_s,
+        "rechecked_pairs": rechecked,
+        "plain_fp64": plain,
+        "speedup_vs_plain_fp64": (plain["kernel_ms"] / float(np.mean(kms))) if plain else None,
+        "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
+                     "frac": achieved / PEAK_FP32_TFLOPS, "traffic": None,
+                     "note": "FP32 vector ALU; 26 algorithmic flop per accepted pair"},
+    }
+    if not args.no_cpu:
+        try:
+            line["cpu_baseline"] = cpu_baseline(cats, n_jk, args.cpu_frac)
+        except Exception as e:  # reported, not fatal
+            line["cpu_baseline"] = {"error": str(e)}
+    else:
+        line["cpu_baseline"] = None
+    ctx.close()
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

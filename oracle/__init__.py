@@ -94,6 +94,11 @@ def lib():
         L.or_comov_dist.restype = C.c_double
         L.or_eq_to_cart.argtypes = [C.c_long] + [C.c_void_p] * 9
         L.or_cart_norm.argtypes = [C.c_long] + [C.c_void_p] * 7
+        L.or_scan_s2_table.argtypes = [C.c_int, C.POINTER(Axis), C.POINTER(Axis), C.c_void_p,
+                                       C.c_float, C.c_float, C.c_int]
+        L.or_scan_s2_table.restype = C.c_long
+        L.or_scan_value_table.argtypes = [C.POINTER(Axis), C.c_void_p, C.c_float, C.c_float, C.c_int]
+        L.or_scan_value_table.restype = C.c_long
         _lib = L
     return _lib
 

@@ -597,3 +597,69 @@ void or_cart_norm(long n, const double *x, const double *y, const double *z,
 		nz[i] = z[i] / d[i];
 	}
 }
+
+/* ------------------------------------------------ threshold-table check -- */
+
+/* Scan every float s2 in [lo, hi] and compare the reference's accept/bin
+ * decision for the s axis (range test, sqrtf, Axis::bin_num, truncation;
+ * correlators.cpp:315-317, 338; hist.cpp:46-53, 486-490) with the table
+ * rule "bin k iff t[k] <= s2 < t[k+1], rejected outside [t[0], t[n])".
+ * Returns the number of disagreeing floats. */
+long or_scan_s2_table(int kind, const or_axis *ax1, const or_axis *ax2, const float *t,
+                      float lo, float hi, int nthreads) {
+	float mn, mx;
+	or_range_sq(kind, ax1, ax2, &mn, &mx);
+	uint32_t klo, khi;
+	memcpy(&klo, &lo, 4);
+	memcpy(&khi, &hi, 4);
+	const int n = ax1->n_bins;
+	long bad = 0;
+#ifdef _OPENMP
+	if (nthreads > 0) omp_set_num_threads(nthreads);
+#endif
+#pragma omp parallel for schedule(static, 1 << 16) reduction(+:bad)
+	for (long k = (long)klo; k <= (long)khi; ++k) {
+		uint32_t b = (uint32_t)k;
+		float s2;
+		memcpy(&s2, &b, 4);
+		int ref_ok = 0, ref_bin = -1, bb;
+		if (s2 < mx && s2 > mn && accept_idx(ax1, or_axis_bin_num(ax1, sqrtf(s2)), &bb)) { ref_ok = 1; ref_bin = bb; }
+		int tab_ok = (s2 >= t[0] && s2 < t[n]);
+		int tab_bin = -1;
+		if (tab_ok) {
+			int lo_ = 0, hi_ = n - 1;
+			while (lo_ < hi_) { int mid = (lo_ + hi_ + 1) / 2; if (t[mid] <= s2) lo_ = mid; else hi_ = mid - 1; }
+			tab_bin = lo_;
+		}
+		if (ref_ok != tab_ok || (ref_ok && ref_bin != tab_bin)) bad++;
+	}
+	return bad;
+}
+
+/* Same for a value-domain table of one axis over all floats in [lo, hi]. */
+long or_scan_value_table(const or_axis *a, const float *t, float lo, float hi, int nthreads) {
+	uint32_t klo, khi;
+	memcpy(&klo, &lo, 4);
+	memcpy(&khi, &hi, 4);
+	const int n = a->n_bins;
+	long bad = 0;
+#ifdef _OPENMP
+	if (nthreads > 0) omp_set_num_threads(nthreads);
+#endif
+#pragma omp parallel for schedule(static, 1 << 16) reduction(+:bad)
+	for (long k = (long)klo; k <= (long)khi; ++k) {
+		uint32_t b = (uint32_t)k;
+		float v;
+		memcpy(&v, &b, 4);
+		int bb, ref_ok = accept_idx(a, or_axis_bin_num(a, v), &bb);
+		int tab_ok = (v >= t[0] && v < t[n]);
+		int tab_bin = -1;
+		if (tab_ok) {
+			int lo_ = 0, hi_ = n - 1;
+			while (lo_ < hi_) { int mid = (lo_ + hi_ + 1) / 2; if (t[mid] <= v) lo_ = mid; else hi_ = mid - 1; }
+			tab_bin = lo_;
+		}
+		if (ref_ok != tab_ok || (ref_ok && bb != tab_bin)) bad++;
+	}
+	return bad;
+}

@@ -104,6 +104,12 @@ void or_eq_to_cart(long n, const double *ra, const double *dec, const double *d,
 void or_cart_norm(long n, const double *x, const double *y, const double *z,
                   double *d, double *nx, double *ny, double *nz);
 
+/* Exhaustive checks of threshold tables against the float pipeline
+ * (non-negative float ranges [lo, hi]); return the number of mismatches. */
+long or_scan_s2_table(int kind, const or_axis *ax1, const or_axis *ax2, const float *t,
+                      float lo, float hi, int nthreads);
+long or_scan_value_table(const or_axis *a, const float *t, float lo, float hi, int nthreads);
+
 #ifdef __cplusplus
 }
 #endif

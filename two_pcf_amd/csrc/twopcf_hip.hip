@@ -1,0 +1,1291 @@
+// twopcf_hip.hip — libtwopcf_hip: gfx950 pair counting behind the two_pcf
+// correlator entry points.  C ABI in include/twopcf_hip.h.
+//
+// Layout on the device (per catalogue and cell geometry):
+//   sorted SoA   double x,y,z,nx,ny,nz; float w, iw; int jk   (cell-major,
+//                region-sorted within a cell: hipCUB radix sort on the key
+//                cell * n_region + region)
+//   cell_start   int[n_cell + 1]
+// Work: a work item is (job, up to 64 first-side objects of one cell, a set
+// of neighbour-cell rows, up to S region slots).  A 256-thread block owns one
+// item at a time: each lane holds one first-side object, the second-side
+// objects of each row are staged through LDS in tiles of 256 and the four
+// waves share every tile.  Per-item histograms live in LDS (u32 counts or
+// f64 sums) and are flushed once per item with global atomics.
+#include <hip/hip_runtime.h>
+#include <hipcub/hipcub.hpp>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <string>
+#include <tuple>
+#include <vector>
+
+#include "../../include/twopcf_hip.h"
+#include "host_math.hpp"
+#include "pairs.hpp"
+
+using namespace twopcf;
+
+namespace {
+
+constexpr int WAVE = 64;
+constexpr int BLOCK = 256;
+constexpr int NWAVES = BLOCK / WAVE;
+constexpr int ICHUNK = 64;
+constexpr int TILE = 256;
+constexpr int MAX_SLOTS = 16;
+constexpr int MAX_ROWS = 64;
+constexpr int MAX_AXIS_BINS = 4096;
+
+// ------------------------------------------------------------ device data --
+
+struct CatDev {
+	const double *x, *y, *z, *nx, *ny, *nz;
+	const float *w, *iw;
+	const int *jk;
+	const uint64_t *masks;
+	const int *start;   // cell starts [n_cell + 1]
+};
+
+struct JobDev {
+	CatDev A, B;        // A: first side (lanes), B: second side (staged)
+	int dim[3];
+	double lo[3];
+	double h;
+	int kind, is_auto, n1, n2, nb, n_jk, weighted;
+	float min_r_sq, max_r_sq;
+	AxisDev ax1, ax2;
+	int t_s2_off, t1_off, t2_off, tA_off, tB_off;   // offsets into the table pool (floats)
+	float s_min, s_ilb_ln2, s_ils, s_inv_step;       // fast s-index estimate
+	int s_log;
+	float mu_c1, mu_c2;                              // fast mu margin: delta = c1 + c2 * rsq(s2)
+	int row_off, n_rows;                             // stencil rows in the row pool
+	unsigned long long *full_u, *jk_u;
+	double *full_d, *jk_d;
+	unsigned long long *counters;                    // [0] accepted, [1] rechecked
+};
+
+struct WorkItem {
+	int job;
+	int i_begin, i_count, i_cell;
+	unsigned long long row_mask;
+	int slot_base, n_slots;
+};
+
+// -------------------------------------------------------------- kernels --
+
+__global__ void k_cell_keys(const double *__restrict__ x, const double *__restrict__ y,
+                            const double *__restrict__ z, const int *__restrict__ jk, int64_t n,
+                            double lo0, double lo1, double lo2, double h, int d0, int d1, int d2,
+                            int n_region, unsigned long long *__restrict__ keys, int *__restrict__ vals) {
+	int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+	if (i >= n) return;
+	int c0 = (int)floor((x[i] - lo0) / h), c1 = (int)floor((y[i] - lo1) / h), c2 = (int)floor((z[i] - lo2) / h);
+	c0 = min(max(c0, 0), d0 - 1);
+	c1 = min(max(c1, 0), d1 - 1);
+	c2 = min(max(c2, 0), d2 - 1);
+	unsigned long long cell = ((unsigned long long)c0 * d1 + c1) * d2 + c2;
+	keys[i] = cell * (unsigned long long)n_region + (unsigned long long)(n_region > 1 ? jk[i] : 0);
+	vals[i] = (int)i;
+}
+
+__global__ void k_gather(const int *__restrict__ perm, int64_t n, const double *x, const double *y, const double *z,
+                         const double *nx, const double *ny, const double *nz, const float *w, const float *iw,
+                         const int *jk, const uint64_t *masks, int n_words, double *ox, double *oy, double *oz,
+                         double *onx, double *ony, double *onz, float *ow, float *oiw, int *ojk, uint64_t *omasks) {
+	int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+	if (i >= n) return;
+	int s = perm[i];
+	ox[i] = x[s];
+	oy[i] = y[s];
+	oz[i] = z[s];
+	onx[i] = nx[s];
+	ony[i] = ny[s];
+	onz[i] = nz[s];
+	ow[i] = w[s];
+	oiw[i] = iw[s];
+	ojk[i] = jk[s];
+	for (int k = 0; k < n_words; ++k) omasks[i * n_words + k] = masks[(int64_t)s * n_words + k];
+}
+
+// start[c] = first sorted index whose key >= c * n_region.
+__global__ void k_cell_start(const unsigned long long *__restrict__ keys, int64_t n, int n_cell, int n_region,
+                             int *__restrict__ start) {
+	int c = blockIdx.x * blockDim.x + threadIdx.x;
+	if (c > n_cell) return;
+	unsigned long long target = (unsigned long long)c * n_region;
+	int64_t lo = 0, hi = n;
+	while (lo < hi) {
+		int64_t mid = (lo + hi) >> 1;
+		if (keys[mid] < target) lo = mid + 1;
+		else hi = mid;
+	}
+	start[c] = (int)lo;
+}
+
+__global__ void k_fill_u64(unsigned long long *p, int64_t n, unsigned long long v) {
+	int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+	if (i < n) p[i] = v;
+}
+
+// LDS layout of the counting kernel (bytes), see lds_layout().
+struct Lds {
+	int tile, tables, slots, hist, total;
+};
+
+template <bool WEIGHTED>
+struct HistT;
+template <>
+struct HistT<false> {
+	using T = unsigned int;
+};
+template <>
+struct HistT<true> {
+	using T = double;
+};
+
+template <int KIND, bool FAST, bool WEIGHTED>
+__device__ void process_item(const JobDev &J, const WorkItem &W, const int4 *__restrict__ rows,
+                             const int *__restrict__ slot_regions, char *smem, const Lds &L) {
+	using HT = typename HistT<WEIGHTED>::T;
+	const int tid = threadIdx.x, lane = tid & (WAVE - 1), wave = tid / WAVE;
+	const int nb = J.nb;
+	const int n_slots = W.n_slots;
+	const bool jk = J.n_jk > 1;
+
+	// LDS carve-up
+	double *tx = (double *)(smem + L.tile);         // exact path: 6 doubles per entry
+	float4 *tfa = (float4 *)(smem + L.tile);        // fast path: (o0,o1,o2,w), (n0,n1,n2,gj)
+	float4 *tfb = tfa + TILE;
+	float *tw = (float *)(tx + 6 * TILE);
+	int *tgj = (int *)(tw + TILE);
+	int *tslot = tgj + TILE;
+	float *tab = (float *)(smem + L.tables);
+	int *sreg = (int *)(smem + L.slots);
+	HT *F = (HT *)(smem + L.hist);
+	HT *H = F + nb;
+
+	// tables and slot regions
+	const int n_tab = L.slots - L.tables;
+	(void)n_tab;
+	for (int k = tid; k < n_slots; k += BLOCK) sreg[k] = slot_regions[W.slot_base + k];
+
+	// first-side object of this lane
+	const bool ivalid = lane < W.i_count;
+	const int gi = W.i_begin + min(lane, W.i_count - 1);
+	const int d1 = J.dim[1], d2 = J.dim[2];
+	const int cx = W.i_cell / (d1 * d2), cy = (W.i_cell / d2) % d1, cz = W.i_cell % d2;
+	const double c0 = J.lo[0] + (cx + 0.5) * J.h, c1 = J.lo[1] + (cy + 0.5) * J.h, c2 = J.lo[2] + (cz + 0.5) * J.h;
+	double xi[3], ni[3];
+	xi[0] = J.A.x[gi];
+	xi[1] = J.A.y[gi];
+	xi[2] = J.A.z[gi];
+	ni[0] = J.A.nx[gi];
+	ni[1] = J.A.ny[gi];
+	ni[2] = J.A.nz[gi];
+	const float wi = J.A.w[gi];
+	const int jki = J.A.jk[gi];
+	const float oi0 = (float)(xi[0] - c0), oi1 = (float)(xi[1] - c1), oi2 = (float)(xi[2] - c2);
+	const float fni0 = (float)ni[0], fni1 = (float)ni[1], fni2 = (float)ni[2];
+	__syncthreads();
+	int slot_i = 0;
+	if (jk)
+		for (int s = 0; s < n_slots; ++s)
+			if (sreg[s] == jki) slot_i = s;
+
+	const float *t_s2 = tab + J.t_s2_off;
+	const float *t1 = tab + J.t1_off;
+	const float *t2 = tab + J.t2_off;
+	const float *tA = tab + J.tA_off;
+	const float *tB = tab + J.tB_off;
+	const float s2_lo = FAST ? tB[0] : 0.0f;
+	const float s2_hi = FAST ? tA[J.n1] : 0.0f;
+
+	unsigned int n_acc = 0, n_rc = 0;
+
+	auto add = [&](int bin, float w, int slot_j) {
+		if (WEIGHTED) {
+			atomicAdd((double *)&F[bin], (double)w);
+			if (jk) {
+				atomicAdd((double *)&H[slot_i * nb + bin], (double)w);
+				if (slot_j != slot_i) atomicAdd((double *)&H[slot_j * nb + bin], (double)w);
+			}
+		} else {
+			atomicAdd((unsigned int *)&F[bin], 1u);
+			if (jk) {
+				atomicAdd((unsigned int *)&H[slot_i * nb + bin], 1u);
+				if (slot_j != slot_i) atomicAdd((unsigned int *)&H[slot_j * nb + bin], 1u);
+			}
+		}
+	};
+
+	for (int r = 0; r < J.n_rows; ++r) {
+		if (!((W.row_mask >> r) & 1ull)) continue;
+		const int4 row = rows[J.row_off + r];
+		const int qx = cx + row.x, qy = cy + row.y;
+		if (qx < 0 || qy < 0 || qx >= J.dim[0] || qy >= d1) continue;
+		if (J.is_auto && (row.x < 0 || (row.x == 0 && row.y < 0))) continue;
+		int z0 = max(cz + row.z, 0), z1 = min(cz + row.w, d2 - 1);
+		if (J.is_auto && row.x == 0 && row.y == 0) z0 = max(z0, cz);
+		if (z0 > z1) continue;
+		const int base = (qx * d1 + qy) * d2;
+		const int jb = J.B.start[base + z0], je = J.B.start[base + z1 + 1];
+		for (int t0 = jb; t0 < je; t0 += TILE) {
+			const int nt = min(TILE, je - t0);
+			if (tid < nt) {
+				const int gj = t0 + tid;
+				const double bx = J.B.x[gj], by = J.B.y[gj], bz = J.B.z[gj];
+				const double bnx = J.B.nx[gj], bny = J.B.ny[gj], bnz = J.B.nz[gj];
+				const float bw = J.B.w[gj];
+				int sl = 0;
+				if (jk) {
+					const int rj = J.B.jk[gj];
+					for (int s = 0; s < n_slots; ++s)
+						if (sreg[s] == rj) sl = s;
+				}
+				if (FAST) {
+					tfa[tid] = make_float4((float)(bx - c0), (float)(by - c1), (float)(bz - c2), bw);
+					tfb[tid] = make_float4((float)bnx, (float)bny, (float)bnz, __int_as_float(gj));
+				} else {
+					tx[6 * tid + 0] = bx;
+					tx[6 * tid + 1] = by;
+					tx[6 * tid + 2] = bz;
+					tx[6 * tid + 3] = bnx;
+					tx[6 * tid + 4] = bny;
+					tx[6 * tid + 5] = bnz;
+					tw[tid] = bw;
+					tgj[tid] = gj;
+				}
+				tslot[tid] = sl;
+			}
+			__syncthreads();
+			if (ivalid) {
+				for (int q = wave; q < nt; q += NWAVES) {
+					int bin;
+					if (FAST) {
+						const float4 a = tfa[q];
+						const float4 b = tfb[q];
+						const int gj = __float_as_int(b.w);
+						const float dx = oi0 - a.x, dy = oi1 - a.y, dz = oi2 - a.z;
+						const float s2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
+						if (!(s2 >= s2_lo && s2 < s2_hi)) continue;
+						if (J.is_auto && gj <= gi) continue;
+						// s bin: closed-form estimate, one-step correction, proof by margins
+						const float sv = __builtin_amdgcn_sqrtf(s2) - J.s_min;
+						int k = J.s_log ? (int)(J.s_ilb_ln2 * __builtin_amdgcn_logf(fmaf(J.s_ils, sv, 1.0f)))
+						                : (int)(sv * J.s_inv_step);
+						k = min(max(k, 0), J.n1 - 1);
+						if (s2 < t_s2[k]) k = max(k - 1, 0);
+						else if (s2 >= t_s2[k + 1]) k = min(k + 1, J.n1 - 1);
+						bool sure = (s2 >= tA[k]) && (s2 < tB[k + 1]);
+						if (KIND == K_S_MU) {
+							const float p = 0.5f * fmaf(dz, fni2 + b.z, fmaf(dy, fni1 + b.y, dx * (fni0 + b.x)));
+							const float rs = __builtin_amdgcn_rsqf(s2);
+							const float nmu = (float)J.n2;
+							const float xm = fabsf(p) * rs * nmu;
+							const float del = fmaf(J.mu_c2, rs, J.mu_c1);
+							const int km = (int)xm;
+							const float fr = xm - (float)km;
+							sure = sure && (xm < nmu - del) && (fr <= 1.0f - del) && (km == 0 || fr >= del);
+							bin = k * J.n2 + km;
+						} else {
+							bin = k;
+						}
+						if (!sure) {
+							++n_rc;
+							double xj[3] = {J.B.x[gj], J.B.y[gj], J.B.z[gj]};
+							double nj[3] = {J.B.nx[gj], J.B.ny[gj], J.B.nz[gj]};
+							if (!exact_bin<KIND>(xi, ni, xj, nj, J.min_r_sq, J.max_r_sq, t_s2, t1, t2, J.ax1, J.ax2, bin))
+								continue;
+						}
+						++n_acc;
+						add(bin, wi * a.w, tslot[q]);
+					} else {
+						const int gj = tgj[q];
+						if (J.is_auto && gj <= gi) continue;
+						const double *e = tx + 6 * q;
+						if (!exact_bin<KIND>(xi, ni, e, e + 3, J.min_r_sq, J.max_r_sq, t_s2, t1, t2, J.ax1, J.ax2, bin))
+							continue;
+						++n_acc;
+						add(bin, wi * tw[q], tslot[q]);
+					}
+				}
+			}
+			__syncthreads();
+		}
+	}
+
+	// flush and clear the LDS histograms
+	for (int b = tid; b < nb; b += BLOCK) {
+		HT v = F[b];
+		if (v != (HT)0) {
+			if (WEIGHTED) atomicAdd(&J.full_d[b], (double)v);
+			else atomicAdd(&J.full_u[b], (unsigned long long)v);
+			F[b] = (HT)0;
+		}
+	}
+	if (jk) {
+		for (int e = tid; e < n_slots * nb; e += BLOCK) {
+			HT v = H[e];
+			if (v != (HT)0) {
+				const int s = e / nb, b = e - s * nb;
+				const size_t o = (size_t)sreg[s] * nb + b;
+				if (WEIGHTED) atomicAdd(&J.jk_d[o], (double)v);
+				else atomicAdd(&J.jk_u[o], (unsigned long long)v);
+				H[e] = (HT)0;
+			}
+		}
+	}
+	// pair statistics
+	for (int off = 32; off; off >>= 1) {
+		n_acc += __shfl_xor(n_acc, off);
+		n_rc += __shfl_xor(n_rc, off);
+	}
+	if (lane == 0) {
+		if (n_acc) atomicAdd(&J.counters[0], (unsigned long long)n_acc);
+		if (n_rc) atomicAdd(&J.counters[1], (unsigned long long)n_rc);
+	}
+	__syncthreads();
+}
+
+// One launch: item = blockIdx.x (plain) or pulled from a global queue
+// (persistent, queue != nullptr).  Every block leaves when the queue is
+// drained, so the grid always finishes.
+template <int KIND, bool FAST, bool WEIGHTED>
+__global__ void __launch_bounds__(BLOCK) k_count(const JobDev *__restrict__ jobs, const WorkItem *__restrict__ items,
+                                                 int n_items, unsigned int *queue, const int4 *__restrict__ rows,
+                                                 const int *__restrict__ slot_regions, const float *__restrict__ tables,
+                                                 int n_tables, Lds L) {
+	extern __shared__ __align__(16) char smem[];
+	__shared__ int s_item;
+	using HT = typename HistT<WEIGHTED>::T;
+	// tables into LDS once per block, histograms cleared once
+	float *tab = (float *)(smem + L.tables);
+	for (int k = threadIdx.x; k < n_tables; k += BLOCK) tab[k] = tables[k];
+	HT *hist = (HT *)(smem + L.hist);
+	const int n_hist = (L.total - L.hist) / (int)sizeof(HT);
+	for (int k = threadIdx.x; k < n_hist; k += BLOCK) hist[k] = (HT)0;
+	int it = blockIdx.x;
+	for (;;) {
+		if (queue) {
+			if (threadIdx.x == 0) s_item = (int)atomicAdd(queue, 1u);
+			__syncthreads();
+			it = s_item;
+			__syncthreads();
+		}
+		if (it >= n_items) break;
+		const WorkItem W = items[it];
+		process_item<KIND, FAST, WEIGHTED>(jobs[W.job], W, rows, slot_regions, smem, L);
+		if (!queue) break;
+	}
+}
+
+// ------------------------------------------------------------ host side --
+
+double now_ms() {
+	return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+}  // namespace
+
+struct twopcf_ctx {
+	int device = 0;
+	hipStream_t stream = nullptr;
+	hipEvent_t ev0 = nullptr, ev1 = nullptr;
+	std::string err;
+	int n_cu = 0;
+	size_t lds_max = 65536;
+	std::string name;
+	std::vector<twopcf_cat *> cats;
+};
+
+struct SortedCat {
+	double *x = nullptr, *y = nullptr, *z = nullptr, *nx = nullptr, *ny = nullptr, *nz = nullptr;
+	float *w = nullptr, *iw = nullptr;
+	int *jk = nullptr;
+	uint64_t *masks = nullptr;
+	int *start = nullptr;
+	std::vector<int> start_h;
+	// regions present in each cell (ascending), CSR over cells (n_region > 1 only)
+	std::vector<int> reg_off, reg_ids;
+	void release() {
+		for (void *p : {(void *)x, (void *)y, (void *)z, (void *)nx, (void *)ny, (void *)nz, (void *)w, (void *)iw,
+		                (void *)jk, (void *)masks, (void *)start})
+			if (p) (void)hipFree(p);
+		*this = SortedCat();
+	}
+};
+
+struct GridKey {
+	double lo[3];
+	double h;
+	int dim[3];
+	int n_region;
+	bool operator<(const GridKey &o) const {
+		return std::tie(lo[0], lo[1], lo[2], h, dim[0], dim[1], dim[2], n_region) <
+		       std::tie(o.lo[0], o.lo[1], o.lo[2], o.h, o.dim[0], o.dim[1], o.dim[2], o.n_region);
+	}
+};
+
+struct twopcf_cat {
+	twopcf_ctx *ctx = nullptr;
+	int64_t n = 0;
+	int n_words = 0;
+	double *x = nullptr, *y = nullptr, *z = nullptr, *nx = nullptr, *ny = nullptr, *nz = nullptr;
+	float *w = nullptr, *iw = nullptr;
+	int *jk = nullptr;
+	uint64_t *masks = nullptr;
+	double lo[3], hi[3], nlo[3], nhi[3];
+	bool unit_w = true, has_iw = false;
+	int jk_min = 0, jk_max = 0;
+	std::map<GridKey, SortedCat> grids;
+};
+
+namespace {
+
+int fail(twopcf_ctx *ctx, int code, const std::string &msg) {
+	if (ctx) ctx->err = msg;
+	return code;
+}
+
+#define HIPCK(ctx, call)                                                                           \
+	do {                                                                                           \
+		hipError_t e_ = (call);                                                                    \
+		if (e_ != hipSuccess)                                                                      \
+			return fail((ctx), TWOPCF_E_HIP, std::string(#call) + ": " + hipGetErrorString(e_));   \
+	} while (0)
+
+template <class T>
+int dalloc(twopcf_ctx *ctx, T **p, size_t n) {
+	HIPCK(ctx, hipMalloc((void **)p, std::max<size_t>(n, 1) * sizeof(T)));
+	return 0;
+}
+
+// Sort one catalogue into the cell geometry g (cached per geometry).
+int grid_catalog(twopcf_ctx *ctx, twopcf_cat *c, const GridKey &g, SortedCat **out) {
+	auto it = c->grids.find(g);
+	if (it != c->grids.end()) {
+		*out = &it->second;
+		return 0;
+	}
+	SortedCat s;
+	const int64_t n = c->n;
+	const int n_cell = g.dim[0] * g.dim[1] * g.dim[2];
+	unsigned long long *keys = nullptr, *keys_s = nullptr;
+	int *vals = nullptr, *perm = nullptr;
+	void *tmp = nullptr;
+	size_t tmp_bytes = 0;
+	int rc = 0;
+	auto cleanup = [&]() {
+		for (void *p : {(void *)keys, (void *)keys_s, (void *)vals, (void *)perm, tmp})
+			if (p) (void)hipFree(p);
+	};
+	if ((rc = dalloc(ctx, &keys, n)) || (rc = dalloc(ctx, &keys_s, n)) || (rc = dalloc(ctx, &vals, n)) ||
+	    (rc = dalloc(ctx, &perm, n))) {
+		cleanup();
+		return rc;
+	}
+	const unsigned long long max_key = (unsigned long long)n_cell * g.n_region;
+	int end_bit = 1;
+	while (end_bit < 64 && (1ull << end_bit) <= max_key) ++end_bit;
+	const int nblk = (int)((n + 255) / 256);
+	if (n > 0) {
+		k_cell_keys<<<nblk, 256, 0, ctx->stream>>>(c->x, c->y, c->z, c->jk, n, g.lo[0], g.lo[1], g.lo[2], g.h,
+		                                           g.dim[0], g.dim[1], g.dim[2], g.n_region, keys, vals);
+		HIPCK(ctx, hipGetLastError());
+		HIPCK(ctx, hipcub::DeviceRadixSort::SortPairs(nullptr, tmp_bytes, keys, keys_s, vals, perm, (int)n, 0,
+		                                               end_bit, ctx->stream));
+		HIPCK(ctx, hipMalloc(&tmp, std::max<size_t>(tmp_bytes, 1)));
+		HIPCK(ctx, hipcub::DeviceRadixSort::SortPairs(tmp, tmp_bytes, keys, keys_s, vals, perm, (int)n, 0, end_bit,
+		                                               ctx->stream));
+	}
+	if ((rc = dalloc(ctx, &s.x, n)) || (rc = dalloc(ctx, &s.y, n)) || (rc = dalloc(ctx, &s.z, n)) ||
+	    (rc = dalloc(ctx, &s.nx, n)) || (rc = dalloc(ctx, &s.ny, n)) || (rc = dalloc(ctx, &s.nz, n)) ||
+	    (rc = dalloc(ctx, &s.w, n)) || (rc = dalloc(ctx, &s.iw, n)) || (rc = dalloc(ctx, &s.jk, n)) ||
+	    (rc = dalloc(ctx, &s.masks, (size_t)n * std::max(c->n_words, 0))) ||
+	    (rc = dalloc(ctx, &s.start, (size_t)n_cell + 1))) {
+		cleanup();
+		s.release();
+		return rc;
+	}
+	if (n > 0) {
+		k_gather<<<nblk, 256, 0, ctx->stream>>>(perm, n, c->x, c->y, c->z, c->nx, c->ny, c->nz, c->w, c->iw, c->jk,
+		                                        c->masks, c->n_words, s.x, s.y, s.z, s.nx, s.ny, s.nz, s.w, s.iw,
+		                                        s.jk, s.masks);
+		HIPCK(ctx, hipGetLastError());
+	}
+	k_cell_start<<<(n_cell + 1 + 255) / 256, 256, 0, ctx->stream>>>(keys_s, n, n_cell, g.n_region, s.start);
+	HIPCK(ctx, hipGetLastError());
+	s.start_h.resize((size_t)n_cell + 1);
+	HIPCK(ctx, hipMemcpyAsync(s.start_h.data(), s.start, ((size_t)n_cell + 1) * sizeof(int), hipMemcpyDeviceToHost,
+	                          ctx->stream));
+	std::vector<int> jk_h;
+	if (g.n_region > 1 && n > 0) {
+		jk_h.resize(n);
+		HIPCK(ctx, hipMemcpyAsync(jk_h.data(), s.jk, n * sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+	}
+	HIPCK(ctx, hipStreamSynchronize(ctx->stream));
+	cleanup();
+	if (g.n_region > 1) {
+		s.reg_off.assign((size_t)n_cell + 1, 0);
+		for (int cc = 0; cc < n_cell; ++cc) {
+			s.reg_off[cc] = (int)s.reg_ids.size();
+			for (int i = s.start_h[cc]; i < s.start_h[cc + 1]; ++i)
+				if (i == s.start_h[cc] || jk_h[i] != jk_h[i - 1]) s.reg_ids.push_back(jk_h[i]);
+		}
+		s.reg_off[n_cell] = (int)s.reg_ids.size();
+	}
+	auto res = c->grids.emplace(g, s);
+	*out = &res.first->second;
+	return 0;
+}
+
+// Stencil rows (dx, dy, dz_lo, dz_hi) of neighbour cells that can hold a
+// pair within rmax when cells have side rmax/k: cells whose closest points
+// are at least rmax apart are left out.
+std::vector<int4> stencil_rows(int k) {
+	std::vector<int4> rows;
+	auto gap = [](int d) { return std::max(0, std::abs(d) - 1); };
+	for (int dx = -k; dx <= k; ++dx)
+		for (int dy = -k; dy <= k; ++dy) {
+			int g = gap(dx) * gap(dx) + gap(dy) * gap(dy);
+			if (g >= k * k) continue;
+			int m = 0;
+			for (int dz = 0; dz <= k; ++dz)
+				if (g + gap(dz) * gap(dz) < k * k) m = dz;
+			rows.push_back(make_int4(dx, dy, -m, m));
+		}
+	return rows;
+}
+
+struct Prepared {
+	JobDev jd;
+	SortedCat *A, *B;
+	bool swapped;
+	int nb;
+	size_t full_n, jk_n;
+	size_t full_off, jk_off;   // offsets into the device output pools
+	bool weighted;
+};
+
+// Fast-path error model: see DESIGN.md "FP32 + edge recheck".  E(S) bounds
+// |s2_fp32 - s2_ref| for true squared separation S, with R the summed
+// per-coordinate distance of both objects from the first object's cell
+// centre.
+inline double e_s2(double S, double R) { return std::ldexp(1.0, -23) * (11.0 * S + 4.0 * R * std::sqrt(S)); }
+
+float round_up(double v) {
+	float f = (float)v;
+	if ((double)f < v) f = std::nextafter(f, INFINITY);
+	return f;
+}
+float round_dn(double v) {
+	float f = (float)v;
+	if ((double)f > v) f = std::nextafter(f, -INFINITY);
+	return f;
+}
+
+}  // namespace
+
+// ================================================================== ABI ==
+
+extern "C" {
+
+int twopcf_abi_version(void) { return TWOPCF_ABI_VERSION; }
+
+int twopcf_ctx_create(int device, twopcf_ctx **out) {
+	if (!out) return TWOPCF_E_ARG;
+	*out = nullptr;
+	int n = 0;
+	if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) return TWOPCF_E_NODEV;
+	if (device < 0 || device >= n) return TWOPCF_E_ARG;
+	auto *ctx = new twopcf_ctx();
+	ctx->device = device;
+	hipDeviceProp_t prop;
+	if (hipSetDevice(device) != hipSuccess || hipGetDeviceProperties(&prop, device) != hipSuccess ||
+	    hipStreamCreate(&ctx->stream) != hipSuccess || hipEventCreate(&ctx->ev0) != hipSuccess ||
+	    hipEventCreate(&ctx->ev1) != hipSuccess) {
+		delete ctx;
+		return TWOPCF_E_HIP;
+	}
+	ctx->n_cu = prop.multiProcessorCount;
+	ctx->lds_max = std::max(prop.sharedMemPerBlock, prop.sharedMemPerBlockOptin);
+	ctx->name = prop.gcnArchName;
+	*out = ctx;
+	return 0;
+}
+
+int twopcf_ctx_destroy(twopcf_ctx *ctx) {
+	if (!ctx) return TWOPCF_E_ARG;
+	(void)hipSetDevice(ctx->device);
+	while (!ctx->cats.empty()) twopcf_catalog_free(ctx->cats.back());
+	if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
+	if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
+	if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+	delete ctx;
+	return 0;
+}
+
+const char *twopcf_last_error(const twopcf_ctx *ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+int twopcf_device_info(twopcf_ctx *ctx, char *name, int name_len, int *n_cu, int *lds_per_block) {
+	if (!ctx) return TWOPCF_E_ARG;
+	if (name && name_len > 0) {
+		std::strncpy(name, ctx->name.c_str(), name_len - 1);
+		name[name_len - 1] = 0;
+	}
+	if (n_cu) *n_cu = ctx->n_cu;
+	if (lds_per_block) *lds_per_block = (int)ctx->lds_max;
+	return 0;
+}
+
+int twopcf_catalog_upload(twopcf_ctx *ctx, const twopcf_cat_desc *d, twopcf_cat **out) {
+	if (!ctx || !d || !out) return fail(ctx, TWOPCF_E_ARG, "null argument");
+	*out = nullptr;
+	if (d->n < 0 || d->n > INT32_MAX - 1) return fail(ctx, TWOPCF_E_LIMIT, "catalogue size out of range");
+	if (d->n > 0 && (!d->x || !d->y || !d->z || !d->nx || !d->ny || !d->nz))
+		return fail(ctx, TWOPCF_E_ARG, "positions and unit vectors are required");
+	if (d->n_words < 0 || d->n_words > 64 || (d->n_words > 0 && !d->masks))
+		return fail(ctx, TWOPCF_E_ARG, "bad mask description");
+	HIPCK(ctx, hipSetDevice(ctx->device));
+	auto *c = new twopcf_cat();
+	c->ctx = ctx;
+	c->n = d->n;
+	c->n_words = d->n_words;
+	const int64_t n = d->n;
+	std::vector<float> w(n, 1.0f), iw;
+	std::vector<int> jk(n, 0);
+	for (int k = 0; k < 3; ++k) {
+		c->lo[k] = c->nlo[k] = INFINITY;
+		c->hi[k] = c->nhi[k] = -INFINITY;
+	}
+	const double *P[3] = {d->x, d->y, d->z}, *N[3] = {d->nx, d->ny, d->nz};
+	for (int64_t i = 0; i < n; ++i)
+		for (int k = 0; k < 3; ++k) {
+			if (!std::isfinite(P[k][i])) {
+				delete c;
+				return fail(ctx, TWOPCF_E_ARG, "non-finite position");
+			}
+			c->lo[k] = std::min(c->lo[k], P[k][i]);
+			c->hi[k] = std::max(c->hi[k], P[k][i]);
+			c->nlo[k] = std::min(c->nlo[k], N[k][i]);
+			c->nhi[k] = std::max(c->nhi[k], N[k][i]);
+		}
+	if (d->w) std::copy(d->w, d->w + n, w.begin());
+	for (int64_t i = 0; i < n; ++i) c->unit_w &= (w[i] == 1.0f);
+	if (d->jk) std::copy(d->jk, d->jk + n, jk.begin());
+	if (n > 0) {
+		c->jk_min = *std::min_element(jk.begin(), jk.end());
+		c->jk_max = *std::max_element(jk.begin(), jk.end());
+	}
+	c->has_iw = d->iw != nullptr;
+	iw.assign(d->iw ? d->iw : w.data(), (d->iw ? d->iw : w.data()) + n);
+	int rc = 0;
+	if ((rc = dalloc(ctx, &c->x, n)) || (rc = dalloc(ctx, &c->y, n)) || (rc = dalloc(ctx, &c->z, n)) ||
+	    (rc = dalloc(ctx, &c->nx, n)) || (rc = dalloc(ctx, &c->ny, n)) || (rc = dalloc(ctx, &c->nz, n)) ||
+	    (rc = dalloc(ctx, &c->w, n)) || (rc = dalloc(ctx, &c->iw, n)) || (rc = dalloc(ctx, &c->jk, n)) ||
+	    (rc = dalloc(ctx, &c->masks, (size_t)n * std::max(d->n_words, 0)))) {
+		ctx->cats.push_back(c);
+		twopcf_catalog_free(c);
+		return rc;
+	}
+	ctx->cats.push_back(c);
+	auto up = [&](void *dst, const void *src, size_t bytes) -> int {
+		if (bytes) HIPCK(ctx, hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, ctx->stream));
+		return 0;
+	};
+	if ((rc = up(c->x, d->x, n * 8)) || (rc = up(c->y, d->y, n * 8)) || (rc = up(c->z, d->z, n * 8)) ||
+	    (rc = up(c->nx, d->nx, n * 8)) || (rc = up(c->ny, d->ny, n * 8)) || (rc = up(c->nz, d->nz, n * 8)) ||
+	    (rc = up(c->w, w.data(), n * 4)) || (rc = up(c->iw, iw.data(), n * 4)) || (rc = up(c->jk, jk.data(), n * 4)) ||
+	    (rc = up(c->masks, d->masks, (size_t)n * d->n_words * 8))) {
+		twopcf_catalog_free(c);
+		return rc;
+	}
+	HIPCK(ctx, hipStreamSynchronize(ctx->stream));
+	*out = c;
+	return 0;
+}
+
+int twopcf_catalog_free(twopcf_cat *c) {
+	if (!c) return TWOPCF_E_ARG;
+	twopcf_ctx *ctx = c->ctx;
+	for (auto &kv : c->grids) kv.second.release();
+	for (void *p : {(void *)c->x, (void *)c->y, (void *)c->z, (void *)c->nx, (void *)c->ny, (void *)c->nz,
+	                (void *)c->w, (void *)c->iw, (void *)c->jk, (void *)c->masks})
+		if (p) (void)hipFree(p);
+	if (ctx) ctx->cats.erase(std::remove(ctx->cats.begin(), ctx->cats.end(), c), ctx->cats.end());
+	delete c;
+	return 0;
+}
+
+int64_t twopcf_catalog_size(const twopcf_cat *c) { return c ? c->n : -1; }
+
+int twopcf_count(twopcf_ctx *ctx, const twopcf_job *jobs, int n_jobs, int method, const twopcf_tuning *tuning,
+                 twopcf_timing *timing) {
+	const double t_start = now_ms();
+	if (!ctx || (!jobs && n_jobs > 0) || n_jobs < 0) return fail(ctx, TWOPCF_E_ARG, "null argument");
+	if (method != TWOPCF_PLAIN_FP64 && method != TWOPCF_FP32_RECHECK) return fail(ctx, TWOPCF_E_ARG, "bad method");
+	HIPCK(ctx, hipSetDevice(ctx->device));
+	const bool fast = method == TWOPCF_FP32_RECHECK;
+	int subdiv = fast ? 2 : 1;
+	int max_slots = 8;
+	if (fast && tuning) {
+		if (tuning->cell_subdiv > 0) subdiv = std::min(tuning->cell_subdiv, 4);
+		if (tuning->max_slots > 0) max_slots = std::min(tuning->max_slots, MAX_SLOTS);
+	}
+
+	// ---- validate and build per-job host descriptions ----
+	struct HostJob {
+		const twopcf_job *j;
+		twopcf_cat *a, *b;   // after the optional swap (a = lanes)
+		bool swapped;
+		AxisF ax1, ax2;
+		double rmax;
+		std::vector<float> t_s2, t1, t2, tA, tB;
+		float min_r_sq, max_r_sq;
+		int nb;
+		bool weighted;
+		GridKey g;
+	};
+	std::vector<HostJob> hj(n_jobs);
+	for (int q = 0; q < n_jobs; ++q) {
+		const twopcf_job &J = jobs[q];
+		HostJob &H = hj[q];
+		H.j = &J;
+		if (!J.a || !J.full) return fail(ctx, TWOPCF_E_ARG, "job needs a catalogue and a full[] output");
+		if (J.kind < TWOPCF_MONOPOLE || J.kind > TWOPCF_S_MU)
+			return fail(ctx, TWOPCF_E_ARG, "correlator kind not supported by this build");
+		if (J.weight_mode != TWOPCF_W_PLAIN) return fail(ctx, TWOPCF_E_ARG, "weight mode not supported by this build");
+		if (J.n_jk < 1) return fail(ctx, TWOPCF_E_ARG, "n_jk must be >= 1");
+		twopcf_cat *a = const_cast<twopcf_cat *>(J.a), *b = const_cast<twopcf_cat *>(J.b);
+		for (twopcf_cat *c : {a, b}) {
+			if (!c) continue;
+			if (c->ctx != ctx) return fail(ctx, TWOPCF_E_ARG, "catalogue belongs to another context");
+			if (J.n_jk > 1 && c->n > 0 && (c->jk_min < 0 || c->jk_max >= J.n_jk))
+				return fail(ctx, TWOPCF_E_ARG, "jackknife region out of [0, n_jk)");
+		}
+		const bool two_axes = J.kind == TWOPCF_SIGMA_PI || J.kind == TWOPCF_S_MU;
+		for (int k = 0; k < (two_axes ? 2 : 1); ++k)
+			if (J.ax[k].n_bins < 1 || J.ax[k].n_bins > MAX_AXIS_BINS)
+				return fail(ctx, TWOPCF_E_LIMIT, "axis bins out of [1, 4096]");
+		H.ax1 = AxisF((float)J.ax[0].min, (float)J.ax[0].max, J.ax[0].n_bins, (float)J.ax[0].log_base);
+		if (J.kind == TWOPCF_S_MU) H.ax2 = AxisF(0.0f, 1.0f, J.ax[1].n_bins, 0.0f);
+		else if (J.kind == TWOPCF_SIGMA_PI)
+			H.ax2 = AxisF((float)J.ax[1].min, (float)J.ax[1].max, J.ax[1].n_bins, (float)J.ax[1].log_base);
+		else H.ax2 = AxisF(0.0f, 1.0f, 1, 0.0f);
+		H.nb = H.ax1.n * (two_axes ? H.ax2.n : 1);
+		range_sq(J.kind, H.ax1, H.ax2, &H.min_r_sq, &H.max_r_sq);
+		if (J.kind == TWOPCF_SIGMA_PI) {
+			H.rmax = std::sqrt(J.ax[0].max * J.ax[0].max + J.ax[1].max * J.ax[1].max);
+			if (!value_thresholds(H.ax1, H.t1) || !value_thresholds(H.ax2, H.t2))
+				return fail(ctx, TWOPCF_E_AXIS, "axis index not monotone");
+		} else {
+			H.rmax = J.ax[0].max;
+			if (!s2_thresholds(J.kind, H.ax1, H.ax2, H.t_s2)) return fail(ctx, TWOPCF_E_AXIS, "axis index not monotone");
+		}
+		if (!(H.rmax > 0) || !std::isfinite(H.rmax)) return fail(ctx, TWOPCF_E_ARG, "rmax must be positive");
+		// The range test is on float s^2; the grid must cover every float s^2
+		// below max_r_sq.
+		H.rmax = std::max(H.rmax, std::sqrt((double)H.max_r_sq)) * (1.0 + 1e-6);
+		H.weighted = !(a->unit_w && (!b || b->unit_w));
+		H.swapped = b && b->n > a->n;
+		H.a = H.swapped ? b : a;
+		H.b = H.swapped ? a : b;
+	}
+
+	// ---- shared cell geometry per rmax over all catalogues of the call ----
+	const double t_prep0 = now_ms();
+	std::vector<int4> row_pool;
+	std::map<double, std::pair<int, int>> rows_of;   // rmax -> (row_off, n_rows)
+	{
+		auto rows = stencil_rows(subdiv);
+		if ((int)rows.size() > MAX_ROWS) return fail(ctx, TWOPCF_E_LIMIT, "stencil too large");
+	}
+	for (auto &H : hj) {
+		double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+		for (auto &G : hj) {
+			if (G.rmax != H.rmax) continue;
+			for (twopcf_cat *c : {G.a, G.b}) {
+				if (!c || c->n == 0) continue;
+				for (int k = 0; k < 3; ++k) {
+					lo[k] = std::min(lo[k], c->lo[k]);
+					hi[k] = std::max(hi[k], c->hi[k]);
+				}
+			}
+		}
+		for (int k = 0; k < 3; ++k)
+			if (!(lo[k] <= hi[k])) lo[k] = hi[k] = 0.0;
+		H.g.h = H.rmax / subdiv;
+		long long ncell = 1;
+		for (int k = 0; k < 3; ++k) {
+			H.g.lo[k] = lo[k];
+			long long dk = (long long)((hi[k] - lo[k]) / H.g.h) + 1;
+			if (dk > (1 << 20)) return fail(ctx, TWOPCF_E_LIMIT, "grid too fine");
+			H.g.dim[k] = (int)dk;
+			ncell *= dk;
+		}
+		if (ncell > (1LL << 28)) return fail(ctx, TWOPCF_E_LIMIT, "too many cells");
+		H.g.n_region = H.j->n_jk > 1 ? H.j->n_jk : 1;
+		if (!rows_of.count(H.rmax)) {
+			auto rows = stencil_rows(subdiv);
+			rows_of[H.rmax] = {(int)row_pool.size(), (int)rows.size()};
+			row_pool.insert(row_pool.end(), rows.begin(), rows.end());
+		}
+	}
+
+	// ---- grids, tables, outputs, work items ----
+	std::vector<Prepared> prep(n_jobs);
+	std::vector<float> tables;
+	std::vector<int> slot_pool;
+	std::vector<WorkItem> items;
+	std::vector<double> item_cost;
+	size_t pool_u = 0, pool_d = 0;
+	int max_tab_n1 = 0;
+	for (int q = 0; q < n_jobs; ++q) {
+		HostJob &H = hj[q];
+		Prepared &P = prep[q];
+		SortedCat *A = nullptr, *B = nullptr;
+		int rc;
+		if ((rc = grid_catalog(ctx, H.a, H.g, &A))) return rc;
+		if (H.b) {
+			if ((rc = grid_catalog(ctx, H.b, H.g, &B))) return rc;
+		} else {
+			B = A;
+		}
+		P.A = A;
+		P.B = B;
+		P.swapped = H.swapped;
+		P.nb = H.nb;
+		P.weighted = H.weighted;
+		JobDev &D = P.jd;
+		std::memset(&D, 0, sizeof(D));
+		auto fill_cat = [](CatDev &cd, SortedCat *s) {
+			cd.x = s->x; cd.y = s->y; cd.z = s->z;
+			cd.nx = s->nx; cd.ny = s->ny; cd.nz = s->nz;
+			cd.w = s->w; cd.iw = s->iw; cd.jk = s->jk; cd.masks = s->masks; cd.start = s->start;
+		};
+		fill_cat(D.A, A);
+		fill_cat(D.B, B);
+		for (int k = 0; k < 3; ++k) {
+			D.dim[k] = H.g.dim[k];
+			D.lo[k] = H.g.lo[k];
+		}
+		D.h = H.g.h;
+		D.kind = H.j->kind;
+		D.is_auto = H.b == nullptr;
+		D.n1 = H.ax1.n;
+		D.n2 = H.ax2.n;
+		D.nb = H.nb;
+		D.n_jk = H.j->n_jk;
+		D.weighted = H.weighted;
+		D.min_r_sq = H.min_r_sq;
+		D.max_r_sq = H.max_r_sq;
+		D.ax1 = AxisDev{H.ax1.n, H.ax1.log ? 1 : 0, H.ax1.minus_min, H.ax1.inv_step};
+		D.ax2 = AxisDev{H.ax2.n, H.ax2.log ? 1 : 0, H.ax2.minus_min, H.ax2.inv_step};
+		auto push = [&](const std::vector<float> &v) {
+			int off = (int)tables.size();
+			tables.insert(tables.end(), v.begin(), v.end());
+			return off;
+		};
+		if (H.j->kind == TWOPCF_SIGMA_PI) {
+			D.t1_off = push(H.t1);
+			D.t2_off = push(H.t2);
+		} else {
+			D.t_s2_off = push(H.t_s2);
+			// fast-path margins around each s^2 threshold
+			const double R = 1.01 * (subdiv + 1) * H.g.h;
+			std::vector<float> A_(H.t_s2.size()), B_(H.t_s2.size());
+			for (size_t k = 0; k < H.t_s2.size(); ++k) {
+				double T = H.t_s2[k];
+				if (!std::isfinite(T)) {
+					A_[k] = B_[k] = (float)T;
+					continue;
+				}
+				double E = 1.01 * e_s2(std::max(T, 0.0) * 1.001, R) + 1e-30;
+				A_[k] = round_up(T + E);
+				B_[k] = round_dn(T - E);
+			}
+			D.tA_off = push(A_);
+			D.tB_off = push(B_);
+			D.s_min = H.ax1.min;
+			D.s_log = H.ax1.log ? 1 : 0;
+			D.s_ilb_ln2 = (float)((double)H.ax1.ilb * std::log(2.0));
+			D.s_ils = H.ax1.ils;
+			D.s_inv_step = H.ax1.inv_step;
+			const double n2 = H.ax2.n;
+			D.mu_c1 = (float)(n2 * std::ldexp(1.0, -23) * 41.0);
+			D.mu_c2 = (float)(n2 * std::ldexp(1.0, -23) * 4.75 * R * 1.01);
+			max_tab_n1 = std::max(max_tab_n1, H.ax1.n);
+		}
+		auto rr = rows_of[H.rmax];
+		D.row_off = rr.first;
+		D.n_rows = rr.second;
+		P.full_n = H.nb;
+		P.jk_n = H.j->n_jk > 1 ? (size_t)H.j->n_jk * H.nb : 0;
+		if (H.weighted) {
+			P.full_off = pool_d;
+			P.jk_off = pool_d + P.full_n;
+			pool_d += P.full_n + P.jk_n;
+		} else {
+			P.full_off = pool_u;
+			P.jk_off = pool_u + P.full_n;
+			pool_u += P.full_n + P.jk_n;
+		}
+
+		// work items: chunks of <= 64 first-side objects per cell, rows
+		// grouped so that each item sees at most max_slots regions
+		const int n_cell = H.g.dim[0] * H.g.dim[1] * H.g.dim[2];
+		const int d1 = H.g.dim[1], d2 = H.g.dim[2];
+		const bool jk = H.j->n_jk > 1;
+		std::vector<int> rws;
+		for (int r = 0; r < D.n_rows; ++r) rws.push_back(r);
+		for (int c = 0; c < n_cell; ++c) {
+			const int ib = A->start_h[c], ie = A->start_h[c + 1];
+			if (ib == ie) continue;
+			const int cx = c / (d1 * d2), cy = (c / d2) % d1, cz = c % d2;
+			// per row: j range, cost
+			struct RowInfo {
+				int r, jb, je;
+			};
+			std::vector<RowInfo> ri;
+			for (int r : rws) {
+				const int4 row = row_pool[D.row_off + r];
+				const int qx = cx + row.x, qy = cy + row.y;
+				if (qx < 0 || qy < 0 || qx >= H.g.dim[0] || qy >= d1) continue;
+				if (D.is_auto && (row.x < 0 || (row.x == 0 && row.y < 0))) continue;
+				int z0 = std::max(cz + row.z, 0), z1 = std::min(cz + row.w, d2 - 1);
+				if (D.is_auto && row.x == 0 && row.y == 0) z0 = std::max(z0, cz);
+				if (z0 > z1) continue;
+				const int base = (qx * d1 + qy) * d2;
+				const int jb = B->start_h[base + z0], je = B->start_h[base + z1 + 1];
+				if (jb == je) continue;
+				ri.push_back({r, jb, je});
+			}
+			if (ri.empty()) continue;
+			for (int i0 = ib; i0 < ie; i0 += ICHUNK) {
+				const int ni = std::min(ICHUNK, ie - i0);
+				// group rows by region budget
+				std::vector<int> regs_i;
+				if (jk) {
+					// regions of this chunk (sorted runs inside the cell)
+					const int ro = A->reg_off[c], re = A->reg_off[c + 1];
+					(void)re;
+					std::vector<int> tmp;
+					for (int k = ro; k < re; ++k) tmp.push_back(A->reg_ids[k]);
+					regs_i = tmp;   // superset of the chunk's regions
+				}
+				size_t g0 = 0;
+				while (g0 < ri.size()) {
+					std::vector<int> regs = regs_i;
+					unsigned long long mask = 0;
+					double cost = 0;
+					size_t g1 = g0;
+					for (; g1 < ri.size(); ++g1) {
+						if (jk) {
+							std::vector<int> add;
+							const int4 row = row_pool[D.row_off + ri[g1].r];
+							const int qx = cx + row.x, qy = cy + row.y;
+							const int base = (qx * d1 + qy) * d2;
+							int z0 = std::max(cz + row.z, 0), z1 = std::min(cz + row.w, d2 - 1);
+							if (D.is_auto && row.x == 0 && row.y == 0) z0 = std::max(z0, cz);
+							for (int z = z0; z <= z1; ++z)
+								for (int k = B->reg_off[base + z]; k < B->reg_off[base + z + 1]; ++k)
+									if (std::find(regs.begin(), regs.end(), B->reg_ids[k]) == regs.end() &&
+									    std::find(add.begin(), add.end(), B->reg_ids[k]) == add.end())
+										add.push_back(B->reg_ids[k]);
+							if ((int)(regs.size() + add.size()) > max_slots && g1 > g0) break;
+							regs.insert(regs.end(), add.begin(), add.end());
+							if ((int)regs.size() > MAX_SLOTS)
+								return fail(ctx, TWOPCF_E_LIMIT, "too many jackknife regions around one cell");
+						}
+						mask |= 1ull << ri[g1].r;
+						cost += (double)ni * (ri[g1].je - ri[g1].jb);
+					}
+					if (cost * 1.0 >= 4.0e9) return fail(ctx, TWOPCF_E_LIMIT, "work item too large for 32-bit counters");
+					WorkItem W;
+					W.job = q;
+					W.i_begin = i0;
+					W.i_count = ni;
+					W.i_cell = c;
+					W.row_mask = mask;
+					W.slot_base = (int)slot_pool.size();
+					W.n_slots = jk ? (int)regs.size() : 0;
+					if (jk) slot_pool.insert(slot_pool.end(), regs.begin(), regs.end());
+					items.push_back(W);
+					item_cost.push_back(cost);
+					g0 = g1;
+				}
+			}
+		}
+	}
+	if (slot_pool.empty()) slot_pool.push_back(0);
+	if (tables.empty()) tables.push_back(0.0f);
+	if (row_pool.empty()) row_pool.push_back(make_int4(0, 0, 0, 0));
+
+	// ---- device buffers for this call ----
+	JobDev *d_jobs = nullptr;
+	WorkItem *d_items = nullptr;
+	int4 *d_rows = nullptr;
+	int *d_slots = nullptr;
+	float *d_tab = nullptr;
+	unsigned long long *d_pool_u = nullptr, *d_cnt = nullptr;
+	double *d_pool_d = nullptr;
+	unsigned int *d_queue = nullptr;
+	std::vector<void *> owned;
+	auto release = [&]() {
+		for (void *p : owned) (void)hipFree(p);
+	};
+	auto alloc = [&](void **p, size_t bytes) -> int {
+		hipError_t e = hipMalloc(p, std::max<size_t>(bytes, 16));
+		if (e != hipSuccess) return fail(ctx, TWOPCF_E_HIP, std::string("hipMalloc: ") + hipGetErrorString(e));
+		owned.push_back(*p);
+		return 0;
+	};
+	int rc = 0;
+	if ((rc = alloc((void **)&d_jobs, sizeof(JobDev) * std::max(n_jobs, 1))) ||
+	    (rc = alloc((void **)&d_items, sizeof(WorkItem) * std::max<size_t>(items.size(), 1))) ||
+	    (rc = alloc((void **)&d_rows, sizeof(int4) * row_pool.size())) ||
+	    (rc = alloc((void **)&d_slots, sizeof(int) * slot_pool.size())) ||
+	    (rc = alloc((void **)&d_tab, sizeof(float) * tables.size())) ||
+	    (rc = alloc((void **)&d_pool_u, sizeof(unsigned long long) * std::max<size_t>(pool_u, 1))) ||
+	    (rc = alloc((void **)&d_pool_d, sizeof(double) * std::max<size_t>(pool_d, 1))) ||
+	    (rc = alloc((void **)&d_cnt, sizeof(unsigned long long) * 2 * std::max(n_jobs, 1))) ||
+	    (rc = alloc((void **)&d_queue, sizeof(unsigned int) * 64))) {
+		release();
+		return rc;
+	}
+	for (int q = 0; q < n_jobs; ++q) {
+		Prepared &P = prep[q];
+		if (P.weighted) {
+			P.jd.full_d = d_pool_d + P.full_off;
+			P.jd.jk_d = d_pool_d + P.jk_off;
+		} else {
+			P.jd.full_u = d_pool_u + P.full_off;
+			P.jd.jk_u = d_pool_u + P.jk_off;
+		}
+		P.jd.counters = d_cnt + 2 * q;
+	}
+	std::vector<JobDev> jd_h(n_jobs);
+	for (int q = 0; q < n_jobs; ++q) jd_h[q] = prep[q].jd;
+	// cost-sorted queue for the batched method
+	if (fast) {
+		std::vector<size_t> ord(items.size());
+		for (size_t k = 0; k < ord.size(); ++k) ord[k] = k;
+		std::stable_sort(ord.begin(), ord.end(), [&](size_t x, size_t y) { return item_cost[x] > item_cost[y]; });
+		std::vector<WorkItem> s(items.size());
+		for (size_t k = 0; k < ord.size(); ++k) s[k] = items[ord[k]];
+		items.swap(s);
+	}
+	auto h2d = [&](void *dst, const void *src, size_t bytes) -> int {
+		if (!bytes) return 0;
+		HIPCK(ctx, hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, ctx->stream));
+		return 0;
+	};
+	if ((rc = h2d(d_jobs, jd_h.data(), sizeof(JobDev) * n_jobs)) ||
+	    (rc = h2d(d_items, items.data(), sizeof(WorkItem) * items.size())) ||
+	    (rc = h2d(d_rows, row_pool.data(), sizeof(int4) * row_pool.size())) ||
+	    (rc = h2d(d_slots, slot_pool.data(), sizeof(int) * slot_pool.size())) ||
+	    (rc = h2d(d_tab, tables.data(), sizeof(float) * tables.size()))) {
+		release();
+		return rc;
+	}
+	// clear outputs with a fill kernel (graph-safe), not a memset
+	auto fill = [&](unsigned long long *p, size_t n) {
+		if (n) k_fill_u64<<<(unsigned)((n + 255) / 256), 256, 0, ctx->stream>>>(p, (int64_t)n, 0ull);
+	};
+	fill(d_pool_u, pool_u);
+	fill((unsigned long long *)d_pool_d, pool_d);   // +0.0 has all bits clear
+	fill(d_cnt, 2 * (size_t)n_jobs);
+	fill((unsigned long long *)d_queue, 32);
+	HIPCK(ctx, hipGetLastError());
+	const double t_prep1 = now_ms();
+
+	// ---- launches ----
+	// LDS: tile + tables + slot regions + histograms
+	int max_nb = 1, max_tab = (int)tables.size();
+	for (auto &H : hj) max_nb = std::max(max_nb, H.nb);
+	(void)max_tab_n1;
+	auto layout = [&](bool weighted, int nb, int slots) {
+		Lds L;
+		L.tile = 0;
+		int tile_bytes = std::max(TILE * (6 * 8 + 4 + 4 + 4), TILE * (2 * 16 + 4 + 4 + 4));
+		L.tables = (tile_bytes + 15) & ~15;
+		L.slots = L.tables + (((int)tables.size() * 4 + 15) & ~15);
+		L.hist = L.slots + ((MAX_SLOTS * 4 + 15) & ~15);
+		L.total = L.hist + nb * (1 + slots) * (weighted ? 8 : 4);
+		return L;
+	};
+	HIPCK(ctx, hipEventRecord(ctx->ev0, ctx->stream));
+	int n_launch = 0;
+	auto launch = [&](int kind, bool weighted, const WorkItem *it, int n_it, int nb, int slots, unsigned *queue,
+	                  int grid) -> int {
+		Lds L = layout(weighted, nb, slots);
+		if ((size_t)L.total > ctx->lds_max)
+			return fail(ctx, TWOPCF_E_LIMIT, "histogram does not fit in LDS (" + std::to_string(L.total) + " B)");
+		void (*fn)(const JobDev *, const WorkItem *, int, unsigned *, const int4 *, const int *, const float *, int,
+		           Lds) = nullptr;
+#define PICK(K, F, W) fn = k_count<K, F, W>
+		if (fast) {
+			if (kind == K_MONO) { if (weighted) PICK(K_MONO, true, true); else PICK(K_MONO, true, false); }
+			else if (kind == K_S_MU) { if (weighted) PICK(K_S_MU, true, true); else PICK(K_S_MU, true, false); }
+			else { if (weighted) PICK(K_SIGMA_PI, false, true); else PICK(K_SIGMA_PI, false, false); }
+		} else {
+			if (kind == K_MONO) { if (weighted) PICK(K_MONO, false, true); else PICK(K_MONO, false, false); }
+			else if (kind == K_S_MU) { if (weighted) PICK(K_S_MU, false, true); else PICK(K_S_MU, false, false); }
+			else { if (weighted) PICK(K_SIGMA_PI, false, true); else PICK(K_SIGMA_PI, false, false); }
+		}
+#undef PICK
+		HIPCK(ctx, hipFuncSetAttribute((const void *)fn, hipFuncAttributeMaxDynamicSharedMemorySize, L.total));
+		if (n_it <= 0) return 0;
+		hipLaunchKernelGGL(fn, dim3(grid), dim3(BLOCK), L.total, ctx->stream, d_jobs, it, n_it, queue, d_rows,
+		                   d_slots, d_tab, (int)tables.size(), L);
+		HIPCK(ctx, hipGetLastError());
+		++n_launch;
+		return 0;
+	};
+	if (!fast) {
+		// plain: one launch per job, one work item per block, in cell order
+		size_t k0 = 0;
+		for (int q = 0; q < n_jobs; ++q) {
+			size_t k1 = k0;
+			while (k1 < items.size() && items[k1].job == q) ++k1;
+			int slots = 0;
+			for (size_t k = k0; k < k1; ++k) slots = std::max(slots, items[k].n_slots);
+			if ((rc = launch(hj[q].j->kind, prep[q].weighted, d_items + k0, (int)(k1 - k0), prep[q].nb, slots, nullptr,
+			                 (int)(k1 - k0)))) {
+				release();
+				return rc;
+			}
+			k0 = k1;
+		}
+	} else {
+		// batched: one persistent launch per (kind, weighted) class over the
+		// cost-sorted queue of all jobs in that class
+		std::map<std::pair<int, int>, std::vector<WorkItem>> cls;
+		for (auto &W : items) cls[{hj[W.job].j->kind, prep[W.job].weighted ? 1 : 0}].push_back(W);
+		size_t off = 0;
+		std::vector<WorkItem> flat;
+		std::vector<std::tuple<int, int, size_t, size_t>> spans;
+		for (auto &kv : cls) {
+			spans.emplace_back(kv.first.first, kv.first.second, off, kv.second.size());
+			flat.insert(flat.end(), kv.second.begin(), kv.second.end());
+			off += kv.second.size();
+		}
+		if ((rc = h2d(d_items, flat.data(), sizeof(WorkItem) * flat.size()))) {
+			release();
+			return rc;
+		}
+		int qi = 0;
+		for (auto &sp : spans) {
+			const int kind = std::get<0>(sp);
+			const bool weighted = std::get<1>(sp) != 0;
+			int nb = 1;
+			for (auto &H : hj)
+				if (H.j->kind == kind) nb = std::max(nb, H.nb);
+			int slots = 0;
+			for (size_t k = std::get<2>(sp); k < std::get<2>(sp) + std::get<3>(sp); ++k)
+				slots = std::max(slots, flat[k].n_slots);
+			Lds L = layout(weighted, nb, slots);
+			const int per_cu = std::max(1, std::min(8, (int)(ctx->lds_max / std::max(L.total, 1))));
+			const int grid = std::min<long long>((long long)ctx->n_cu * per_cu, (long long)std::get<3>(sp));
+			if ((rc = launch(kind, weighted, d_items + std::get<2>(sp), (int)std::get<3>(sp), nb, slots, d_queue + qi,
+			                 std::max(grid, 1)))) {
+				release();
+				return rc;
+			}
+			++qi;
+		}
+	}
+	HIPCK(ctx, hipEventRecord(ctx->ev1, ctx->stream));
+
+	// ---- results ----
+	std::vector<unsigned long long> hu(pool_u), hcnt(2 * (size_t)n_jobs);
+	std::vector<double> hd(pool_d);
+	if (pool_u) HIPCK(ctx, hipMemcpyAsync(hu.data(), d_pool_u, pool_u * 8, hipMemcpyDeviceToHost, ctx->stream));
+	if (pool_d) HIPCK(ctx, hipMemcpyAsync(hd.data(), d_pool_d, pool_d * 8, hipMemcpyDeviceToHost, ctx->stream));
+	if (n_jobs) HIPCK(ctx, hipMemcpyAsync(hcnt.data(), d_cnt, hcnt.size() * 8, hipMemcpyDeviceToHost, ctx->stream));
+	HIPCK(ctx, hipStreamSynchronize(ctx->stream));
+	float kms = 0.0f;
+	HIPCK(ctx, hipEventElapsedTime(&kms, ctx->ev0, ctx->ev1));
+	release();
+	uint64_t tot_acc = 0, tot_rc = 0;
+	for (int q = 0; q < n_jobs; ++q) {
+		const twopcf_job &J = jobs[q];
+		Prepared &P = prep[q];
+		for (size_t b = 0; b < P.full_n; ++b)
+			J.full[b] = P.weighted ? hd[P.full_off + b] : (double)hu[P.full_off + b];
+		if (J.jk) {
+			if (J.n_jk > 1) {
+				for (size_t e = 0; e < P.jk_n; ++e) J.jk[e] = P.weighted ? hd[P.jk_off + e] : (double)hu[P.jk_off + e];
+			} else {
+				for (size_t b = 0; b < P.full_n; ++b) J.jk[b] = J.full[b];   // every pair touches region 0
+			}
+		}
+		if (J.n_pairs) *J.n_pairs = hcnt[2 * q];
+		if (J.n_rechecked) *J.n_rechecked = hcnt[2 * q + 1];
+		tot_acc += hcnt[2 * q];
+		tot_rc += hcnt[2 * q + 1];
+	}
+	if (timing) {
+		timing->kernel_ms = kms;
+		timing->prep_ms = t_prep1 - t_prep0;
+		timing->total_ms = now_ms() - t_start;
+		timing->n_launches = n_launch;
+		timing->n_work_items = (int)items.size();
+		timing->n_pairs = tot_acc;
+		timing->n_rechecked = tot_rc;
+	}
+	return 0;
+}
+
+// ------------------------------------------------------ host-only helpers --
+
+int twopcf_s2_thresholds(int kind, const twopcf_axis *ax1, const twopcf_axis *ax2, float *t) {
+	if (!ax1 || !t) return TWOPCF_E_ARG;
+	AxisF a1((float)ax1->min, (float)ax1->max, ax1->n_bins, (float)ax1->log_base);
+	AxisF a2 = ax2 ? AxisF((float)ax2->min, (float)ax2->max, ax2->n_bins, (float)ax2->log_base) : a1;
+	if (kind == TWOPCF_SIGMA_PI) {
+		float mn, mx;
+		range_sq(kind, a1, a2, &mn, &mx);
+		t[0] = std::nextafter(mn, INFINITY);
+		t[1] = mx;
+		return 0;
+	}
+	if (kind != TWOPCF_MONOPOLE && kind != TWOPCF_S_MU) return TWOPCF_E_ARG;
+	std::vector<float> v;
+	if (!s2_thresholds(kind, a1, a2, v)) return TWOPCF_E_AXIS;
+	std::copy(v.begin(), v.end(), t);
+	return 0;
+}
+
+int twopcf_axis_thresholds(const twopcf_axis *ax, float *t) {
+	if (!ax || !t) return TWOPCF_E_ARG;
+	AxisF a((float)ax->min, (float)ax->max, ax->n_bins, (float)ax->log_base);
+	std::vector<float> v;
+	if (!value_thresholds(a, v)) return TWOPCF_E_AXIS;
+	std::copy(v.begin(), v.end(), t);
+	return 0;
+}
+
+int twopcf_comoving_distance(double omega_m, double h, double z_min, double z_max, int64_t n, const double *z,
+                             double *d) {
+	if ((n > 0 && (!z || !d)) || n < 0) return TWOPCF_E_ARG;
+	Cosmology c(omega_m, h, z_min, z_max);
+	for (int64_t i = 0; i < n; ++i) d[i] = c.comov_dist(z[i]);
+	return 0;
+}
+
+int twopcf_eq_to_cart(int64_t n, const double *ra, const double *dec, const double *d, double *x, double *y,
+                      double *z, double *nx, double *ny, double *nz) {
+	if (n < 0 || (n > 0 && (!ra || !dec || !d || !x || !y || !z || !nx || !ny || !nz))) return TWOPCF_E_ARG;
+	eq_to_cart(n, ra, dec, d, x, y, z, nx, ny, nz);
+	return 0;
+}
+
+}  // extern "C"
