@@ -74,13 +74,14 @@ def _jk_batch(scale, seed):
     return out, c["n_jk"]
 
 
-@pytest.mark.parametrize("method", [0, 1])
-def test_jackknife_batch_vs_oracle(ctx, oracle_lib, method):
-    """100 jackknife regions, s-mu 40x50, 1% of the 2M+10M batch."""
+@pytest.mark.parametrize("method,slots", [(0, 0), (1, 0), (1, 1), (0, 2)])
+def test_jackknife_batch_vs_oracle(ctx, oracle_lib, method, slots):
+    """100 jackknife regions, s-mu 40x50, 1% of the 2M+10M batch.  slots=1/2
+    forces most regions off the LDS slots onto the global-atomic path."""
     from two_pcf_amd import lib as L
     o = oracle_lib
     cats, n_jk = _jk_batch(0.01, 7)
-    print(f"[gpu] jk batch method={method} nd={cats['data']['x'].size} nr={cats['randoms']['x'].size}", flush=True)
+    print(f"[gpu] jk batch method={method} slots={slots} nd={cats['data']['x'].size} nr={cats['randoms']['x'].size}", flush=True)
     axes = [(0.0, 100.0, 40, 1.3), (0.0, 1.0, 50, 0.0)]
     dc = {t: ctx.upload(a["x"], a["y"], a["z"], a["nx"], a["ny"], a["nz"], a["w"], a["jk"]) for t, a in cats.items()}
     oc = {t: _oracle_cat(o, L, a) for t, a in cats.items()}
@@ -88,7 +89,7 @@ def test_jackknife_batch_vs_oracle(ctx, oracle_lib, method):
     jobs = [dict(kind=2, a=dc["data"], b=None, axes=axes, n_jk=n_jk),
             dict(kind=2, a=dc["data"], b=dc["randoms"], axes=axes, n_jk=n_jk),
             dict(kind=2, a=dc["randoms"], b=None, axes=axes, n_jk=n_jk)]
-    res, tm = ctx.count(jobs, method=method)
+    res, tm = ctx.count(jobs, method=method, tuning={"max_slots": slots} if slots else None)
     for r, (a, b) in zip(res, (("data", None), ("data", "randoms"), ("randoms", None))):
         full, sub, acc = o.count(2, oc[a], oc[b] if b else None, ax1, ax2, n_jk)
         np.testing.assert_array_equal(r["full"], full)

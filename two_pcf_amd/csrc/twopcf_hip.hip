@@ -35,10 +35,16 @@ using namespace twopcf;
 namespace {
 
 constexpr int WAVE = 64;
-constexpr int BLOCK = 256;
-constexpr int NWAVES = BLOCK / WAVE;
-constexpr int ICHUNK = 64;
-constexpr int TILE = 256;
+constexpr int IG = 4;                       // wave groups over first-side objects
+constexpr int JG = 4;                       // wave groups over each staged tile
+constexpr int BLOCK = WAVE * IG * JG;       // 1024 threads, 16 waves
+constexpr int ICHUNK = WAVE * IG;           // first-side objects per work item
+constexpr int TILE_FAST = 1024;             // staged second-side objects (fp32 path)
+constexpr int TILE_EXACT = 512;             // staged second-side objects (fp64 path)
+__host__ __device__ constexpr int tile_of(bool fast) { return fast ? TILE_FAST : TILE_EXACT; }
+__host__ __device__ constexpr int tile_bytes(bool fast) {
+	return fast ? TILE_FAST * (2 * 16 + 4) : TILE_EXACT * (6 * 8 + 4 + 4 + 4);
+}
 constexpr int MAX_SLOTS = 16;
 constexpr int MAX_ROWS = 64;
 constexpr int MAX_AXIS_BINS = 4096;
@@ -154,18 +160,20 @@ template <int KIND, bool FAST, bool WEIGHTED>
 __device__ void process_item(const JobDev &J, const WorkItem &W, const int4 *__restrict__ rows,
                              const int *__restrict__ slot_regions, char *smem, const Lds &L) {
 	using HT = typename HistT<WEIGHTED>::T;
+	constexpr int TILE = tile_of(FAST);
 	const int tid = threadIdx.x, lane = tid & (WAVE - 1), wave = tid / WAVE;
+	const int ig = wave % IG, jg = wave / IG;
 	const int nb = J.nb;
 	const int n_slots = W.n_slots;
 	const bool jk = J.n_jk > 1;
 
 	// LDS carve-up
-	double *tx = (double *)(smem + L.tile);         // exact path: 6 doubles per entry
-	float4 *tfa = (float4 *)(smem + L.tile);        // fast path: (o0,o1,o2,w), (n0,n1,n2,gj)
-	float4 *tfb = tfa + TILE;
+	double *tx = (double *)(smem + L.tile);         // exact path: 6 doubles per entry, then w, gj, slot
 	float *tw = (float *)(tx + 6 * TILE);
 	int *tgj = (int *)(tw + TILE);
-	int *tslot = tgj + TILE;
+	float4 *tfa = (float4 *)(smem + L.tile);        // fast path: (o0,o1,o2,w), (n0,n1,n2,gj), slot
+	float4 *tfb = tfa + TILE;
+	int *tslot = FAST ? (int *)(tfb + TILE) : tgj + TILE;
 	float *tab = (float *)(smem + L.tables);
 	int *sreg = (int *)(smem + L.slots);
 	HT *F = (HT *)(smem + L.hist);
@@ -177,8 +185,9 @@ __device__ void process_item(const JobDev &J, const WorkItem &W, const int4 *__r
 	for (int k = tid; k < n_slots; k += BLOCK) sreg[k] = slot_regions[W.slot_base + k];
 
 	// first-side object of this lane
-	const bool ivalid = lane < W.i_count;
-	const int gi = W.i_begin + min(lane, W.i_count - 1);
+	const int li = ig * WAVE + lane;
+	const bool ivalid = li < W.i_count;
+	const int gi = W.i_begin + min(li, W.i_count - 1);
 	const int d1 = J.dim[1], d2 = J.dim[2];
 	const int cx = W.i_cell / (d1 * d2), cy = (W.i_cell / d2) % d1, cz = W.i_cell % d2;
 	const double c0 = J.lo[0] + (cx + 0.5) * J.h, c1 = J.lo[1] + (cy + 0.5) * J.h, c2 = J.lo[2] + (cz + 0.5) * J.h;
@@ -194,10 +203,14 @@ __device__ void process_item(const JobDev &J, const WorkItem &W, const int4 *__r
 	const float oi0 = (float)(xi[0] - c0), oi1 = (float)(xi[1] - c1), oi2 = (float)(xi[2] - c2);
 	const float fni0 = (float)ni[0], fni1 = (float)ni[1], fni2 = (float)ni[2];
 	__syncthreads();
+	// region code: an LDS slot s >= 0, or -(region + 1) for a region without a
+	// slot in this item, whose sums go straight to global memory
 	int slot_i = 0;
-	if (jk)
+	if (jk) {
+		slot_i = -(jki + 1);
 		for (int s = 0; s < n_slots; ++s)
 			if (sreg[s] == jki) slot_i = s;
+	}
 
 	const float *t_s2 = tab + J.t_s2_off;
 	const float *t1 = tab + J.t1_off;
@@ -209,19 +222,22 @@ __device__ void process_item(const JobDev &J, const WorkItem &W, const int4 *__r
 
 	unsigned int n_acc = 0, n_rc = 0;
 
-	auto add = [&](int bin, float w, int slot_j) {
-		if (WEIGHTED) {
-			atomicAdd((double *)&F[bin], (double)w);
-			if (jk) {
-				atomicAdd((double *)&H[slot_i * nb + bin], (double)w);
-				if (slot_j != slot_i) atomicAdd((double *)&H[slot_j * nb + bin], (double)w);
-			}
+	auto add_region = [&](int code, int bin, float w) {
+		if (code >= 0) {
+			if (WEIGHTED) atomicAdd((double *)&H[code * nb + bin], (double)w);
+			else atomicAdd((unsigned int *)&H[code * nb + bin], 1u);
 		} else {
-			atomicAdd((unsigned int *)&F[bin], 1u);
-			if (jk) {
-				atomicAdd((unsigned int *)&H[slot_i * nb + bin], 1u);
-				if (slot_j != slot_i) atomicAdd((unsigned int *)&H[slot_j * nb + bin], 1u);
-			}
+			const size_t o = (size_t)(-code - 1) * nb + bin;
+			if (WEIGHTED) atomicAdd(&J.jk_d[o], (double)w);
+			else atomicAdd(&J.jk_u[o], 1ull);
+		}
+	};
+	auto add = [&](int bin, float w, int slot_j) {
+		if (WEIGHTED) atomicAdd((double *)&F[bin], (double)w);
+		else atomicAdd((unsigned int *)&F[bin], 1u);
+		if (jk) {
+			add_region(slot_i, bin, w);
+			if (slot_j != slot_i) add_region(slot_j, bin, w);
 		}
 	};
 
@@ -246,6 +262,7 @@ __device__ void process_item(const JobDev &J, const WorkItem &W, const int4 *__r
 				int sl = 0;
 				if (jk) {
 					const int rj = J.B.jk[gj];
+					sl = -(rj + 1);
 					for (int s = 0; s < n_slots; ++s)
 						if (sreg[s] == rj) sl = s;
 				}
@@ -266,7 +283,7 @@ __device__ void process_item(const JobDev &J, const WorkItem &W, const int4 *__r
 			}
 			__syncthreads();
 			if (ivalid) {
-				for (int q = wave; q < nt; q += NWAVES) {
+				for (int q = jg; q < nt; q += JG) {
 					int bin;
 					if (FAST) {
 						const float4 a = tfa[q];
@@ -734,9 +751,9 @@ int twopcf_count(twopcf_ctx *ctx, const twopcf_job *jobs, int n_jobs, int method
 	HIPCK(ctx, hipSetDevice(ctx->device));
 	const bool fast = method == TWOPCF_FP32_RECHECK;
 	int subdiv = fast ? 2 : 1;
-	int max_slots = 8;
-	if (fast && tuning) {
-		if (tuning->cell_subdiv > 0) subdiv = std::min(tuning->cell_subdiv, 4);
+	int max_slots = 4;
+	if (tuning) {
+		if (fast && tuning->cell_subdiv > 0) subdiv = std::min(tuning->cell_subdiv, 4);
 		if (tuning->max_slots > 0) max_slots = std::min(tuning->max_slots, MAX_SLOTS);
 	}
 
@@ -847,6 +864,13 @@ int twopcf_count(twopcf_ctx *ctx, const twopcf_job *jobs, int n_jobs, int method
 	std::vector<double> item_cost;
 	size_t pool_u = 0, pool_d = 0;
 	int max_tab_n1 = 0;
+	// table sizes are known up front: every job pushes at most 3 * (n1 + 1)
+	// + (n2 + 1) floats
+	size_t tab_floats = 1;
+	for (auto &H : hj) tab_floats += 3 * (H.ax1.n + 1) + (H.ax2.n + 1) + 8;
+	const int lds_fixed = (int)(((std::max(tile_bytes(true), tile_bytes(false)) + 15) & ~15) + ((tab_floats * 4 + 15) & ~15) +
+	                            ((MAX_SLOTS * 4 + 15) & ~15));
+	const size_t lds_budget = ctx->lds_max;
 	for (int q = 0; q < n_jobs; ++q) {
 		HostJob &H = hj[q];
 		Prepared &P = prep[q];
@@ -939,7 +963,13 @@ int twopcf_count(twopcf_ctx *ctx, const twopcf_job *jobs, int n_jobs, int method
 		}
 
 		// work items: chunks of <= 64 first-side objects per cell, rows
-		// grouped so that each item sees at most max_slots regions
+		// grouped so that each item sees at most slot_cap regions; the slot
+		// cap also keeps (1 + slots) histograms inside the LDS budget
+		const int hist_bytes = H.nb * (H.weighted ? 8 : 4);
+		const int slot_cap =
+		    std::max(1, std::min(max_slots, (int)((lds_budget - lds_fixed) / std::max(hist_bytes, 1)) - 1));
+		if ((long long)lds_fixed + 2LL * hist_bytes > (long long)ctx->lds_max)
+			return fail(ctx, TWOPCF_E_LIMIT, "histogram does not fit in LDS");
 		const int n_cell = H.g.dim[0] * H.g.dim[1] * H.g.dim[2];
 		const int d1 = H.g.dim[1], d2 = H.g.dim[2];
 		const bool jk = H.j->n_jk > 1;
@@ -999,10 +1029,8 @@ int twopcf_count(twopcf_ctx *ctx, const twopcf_job *jobs, int n_jobs, int method
 									if (std::find(regs.begin(), regs.end(), B->reg_ids[k]) == regs.end() &&
 									    std::find(add.begin(), add.end(), B->reg_ids[k]) == add.end())
 										add.push_back(B->reg_ids[k]);
-							if ((int)(regs.size() + add.size()) > max_slots && g1 > g0) break;
+							if ((int)(regs.size() + add.size()) > slot_cap && g1 > g0) break;
 							regs.insert(regs.end(), add.begin(), add.end());
-							if ((int)regs.size() > MAX_SLOTS)
-								return fail(ctx, TWOPCF_E_LIMIT, "too many jackknife regions around one cell");
 						}
 						mask |= 1ull << ri[g1].r;
 						cost += (double)ni * (ri[g1].je - ri[g1].jb);
@@ -1015,6 +1043,7 @@ int twopcf_count(twopcf_ctx *ctx, const twopcf_job *jobs, int n_jobs, int method
 					W.i_cell = c;
 					W.row_mask = mask;
 					W.slot_base = (int)slot_pool.size();
+					if ((int)regs.size() > slot_cap) regs.resize(slot_cap);   // the rest: global atomics
 					W.n_slots = jk ? (int)regs.size() : 0;
 					if (jk) slot_pool.insert(slot_pool.end(), regs.begin(), regs.end());
 					items.push_back(W);
@@ -1111,11 +1140,10 @@ int twopcf_count(twopcf_ctx *ctx, const twopcf_job *jobs, int n_jobs, int method
 	int max_nb = 1, max_tab = (int)tables.size();
 	for (auto &H : hj) max_nb = std::max(max_nb, H.nb);
 	(void)max_tab_n1;
-	auto layout = [&](bool weighted, int nb, int slots) {
+	auto layout = [&](bool fast_kernel, bool weighted, int nb, int slots) {
 		Lds L;
 		L.tile = 0;
-		int tile_bytes = std::max(TILE * (6 * 8 + 4 + 4 + 4), TILE * (2 * 16 + 4 + 4 + 4));
-		L.tables = (tile_bytes + 15) & ~15;
+		L.tables = (tile_bytes(fast_kernel) + 15) & ~15;
 		L.slots = L.tables + (((int)tables.size() * 4 + 15) & ~15);
 		L.hist = L.slots + ((MAX_SLOTS * 4 + 15) & ~15);
 		L.total = L.hist + nb * (1 + slots) * (weighted ? 8 : 4);
@@ -1125,7 +1153,8 @@ int twopcf_count(twopcf_ctx *ctx, const twopcf_job *jobs, int n_jobs, int method
 	int n_launch = 0;
 	auto launch = [&](int kind, bool weighted, const WorkItem *it, int n_it, int nb, int slots, unsigned *queue,
 	                  int grid) -> int {
-		Lds L = layout(weighted, nb, slots);
+		const bool fast_kernel = fast && kind != K_SIGMA_PI;
+		Lds L = layout(fast_kernel, weighted, nb, slots);
 		if ((size_t)L.total > ctx->lds_max)
 			return fail(ctx, TWOPCF_E_LIMIT, "histogram does not fit in LDS (" + std::to_string(L.total) + " B)");
 		void (*fn)(const JobDev *, const WorkItem *, int, unsigned *, const int4 *, const int *, const float *, int,
@@ -1191,7 +1220,7 @@ int twopcf_count(twopcf_ctx *ctx, const twopcf_job *jobs, int n_jobs, int method
 			int slots = 0;
 			for (size_t k = std::get<2>(sp); k < std::get<2>(sp) + std::get<3>(sp); ++k)
 				slots = std::max(slots, flat[k].n_slots);
-			Lds L = layout(weighted, nb, slots);
+			Lds L = layout(kind != K_SIGMA_PI, weighted, nb, slots);
 			const int per_cu = std::max(1, std::min(8, (int)(ctx->lds_max / std::max(L.total, 1))));
 			const int grid = std::min<long long>((long long)ctx->n_cu * per_cu, (long long)std::get<3>(sp));
 			if ((rc = launch(kind, weighted, d_items + std::get<2>(sp), (int)std::get<3>(sp), nb, slots, d_queue + qi,
