@@ -97,6 +97,24 @@ def cpu_baseline(cats, n_jk, frac, seed=5):
                        f"{secs:.1f} s of counting)")
 
 
+def copy_calibration(mib):
+    """One device-to-device copy of `mib` MiB (HIP's blit kernel): the
+    reference dispatch whose FETCH_SIZE / WRITE_SIZE counters calibrate the
+    HBM-traffic figures of the counting kernel."""
+    import ctypes as C
+    hip = C.CDLL("libamdhip64.so")
+    n = mib << 20
+    a, b = C.c_void_p(), C.c_void_p()
+    assert hip.hipMalloc(C.byref(a), C.c_size_t(n)) == 0 and hip.hipMalloc(C.byref(b), C.c_size_t(n)) == 0
+    assert hip.hipMemset(a, 1, C.c_size_t(n)) == 0
+    assert hip.hipDeviceSynchronize() == 0
+    assert hip.hipMemcpy(b, a, C.c_size_t(n), 3) == 0   # hipMemcpyDeviceToDevice
+    assert hip.hipDeviceSynchronize() == 0
+    hip.hipFree(a)
+    hip.hipFree(b)
+    print(json.dumps({"copy_calibration_mib": mib}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -108,7 +126,13 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--subdiv", type=int, default=0)
     ap.add_argument("--slots", type=int, default=0)
+    ap.add_argument("--n-jk", type=int, default=0, help="override the number of jackknife regions (1: none)")
+    ap.add_argument("--calib-copy-mib", type=int, default=0,
+                    help="only run a device copy of this size (counter calibration)")
     args = ap.parse_args()
+    if args.calib_copy_mib:
+        copy_calibration(args.calib_copy_mib)
+        return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
               "(the sharding design is described in DESIGN.md)", file=sys.stderr)
@@ -116,6 +140,8 @@ def main():
 
     from two_pcf_amd import lib as L
     cats, n_jk = make_catalogs(args.scale)
+    if args.n_jk:
+        n_jk = args.n_jk
     ctx = L.Context(0)
     info = ctx.info()
     log("device", info)

@@ -146,3 +146,31 @@ def test_edge_cases(ctx):
     bad = ctx.upload([1.0, 2.0], [1.0, 2.0], [1.0, 2.0], [0.6] * 2, [0.6] * 2, [0.5] * 2, jk=[0, 5])
     with pytest.raises(L.TwopcfError):
         ctx.count([dict(kind=2, a=bad, b=None, axes=axes, n_jk=3)], method=1)
+
+
+def test_graph_replays_match_eager(ctx, oracle_lib):
+    """The batched method replays a captured hipGraph on repeated calls; every
+    replay, with other GPU work and synchronisations in between, must equal
+    the plain method's eager result and the oracle."""
+    from two_pcf_amd import lib as L
+    o = oracle_lib
+    cats, n_jk = _jk_batch(0.005, 11)
+    axes = [(0.0, 100.0, 40, 1.3), (0.0, 1.0, 50, 0.0)]
+    dc = {t: ctx.upload(a["x"], a["y"], a["z"], a["nx"], a["ny"], a["nz"], a["w"], a["jk"]) for t, a in cats.items()}
+    jobs = [dict(kind=2, a=dc["data"], b=None, axes=axes, n_jk=n_jk),
+            dict(kind=2, a=dc["data"], b=dc["randoms"], axes=axes, n_jk=n_jk),
+            dict(kind=2, a=dc["randoms"], b=None, axes=axes, n_jk=n_jk)]
+    eager, _ = ctx.count(jobs, method=0)
+    oc = {t: _oracle_cat(o, L, a) for t, a in cats.items()}
+    want = o.count(2, oc["data"], oc["randoms"], o.axis(*axes[0]), o.axis(*axes[1]), n_jk)
+    np.testing.assert_array_equal(eager[1]["full"], want[0])
+    for rep in range(4):
+        print(f"[gpu] graph replay {rep}", flush=True)
+        res, t = ctx.count(jobs, method=1)
+        for a, b in zip(res, eager):
+            np.testing.assert_array_equal(a["full"], b["full"])
+            np.testing.assert_array_equal(a["jk"], b["jk"])
+            assert a["n_pairs"] == b["n_pairs"]
+        ctx.count([dict(kind=0, a=dc["data"], b=None, axes=[(0.0, 50.0, 10, 1.0)])], method=0)  # eager work between
+    for c in dc.values():
+        c.free()

@@ -39,11 +39,14 @@ constexpr int IG = 4;                       // wave groups over first-side objec
 constexpr int JG = 4;                       // wave groups over each staged tile
 constexpr int BLOCK = WAVE * IG * JG;       // 1024 threads, 16 waves
 constexpr int ICHUNK = WAVE * IG;           // first-side objects per work item
-constexpr int TILE_FAST = 1024;             // staged second-side objects (fp32 path)
+constexpr int TILE_FAST = 512;              // staged second-side objects (fp32 path)
 constexpr int TILE_EXACT = 512;             // staged second-side objects (fp64 path)
+constexpr int RING = 256;                   // per-wave queue of in-range pairs (fp32 path)
 __host__ __device__ constexpr int tile_of(bool fast) { return fast ? TILE_FAST : TILE_EXACT; }
+// tile + (fp32 path) first-side chunk and per-wave pair queues
 __host__ __device__ constexpr int tile_bytes(bool fast) {
-	return fast ? TILE_FAST * (2 * 16 + 4) : TILE_EXACT * (6 * 8 + 4 + 4 + 4);
+	return fast ? TILE_FAST * (2 * 16 + 4) + ICHUNK * 32 + (BLOCK / WAVE) * RING * 4
+	            : TILE_EXACT * (6 * 8 + 4 + 4 + 4);
 }
 constexpr int MAX_SLOTS = 16;
 constexpr int MAX_ROWS = 64;
@@ -68,6 +71,7 @@ struct JobDev {
 	float min_r_sq, max_r_sq;
 	AxisDev ax1, ax2;
 	int t_s2_off, t1_off, t2_off, tA_off, tB_off;   // offsets into the table pool (floats)
+	int tq_off;                                      // (T, A, B, 0) per s^2 threshold, 16-byte aligned
 	float s_min, s_ilb_ln2, s_ils, s_inv_step;       // fast s-index estimate
 	int s_log;
 	float mu_c1, mu_c2;                              // fast mu margin: delta = c1 + c2 * rsq(s2)
@@ -371,6 +375,243 @@ __device__ void process_item(const JobDev &J, const WorkItem &W, const int4 *__r
 	__syncthreads();
 }
 
+// FP32 path with per-wave compaction.  Screening: every wave tests its
+// first-side objects (one per lane) against its share of the staged tile with
+// FP32 s^2 from cell-local offsets; pairs that may be in range are appended
+// to the wave's LDS ring.  Binning: whenever 64 pairs are queued, each lane
+// takes one, recomputes the same s^2, bins it and proves the bin with the
+// host-computed margins, or re-evaluates it with the exact pipeline.
+template <int KIND, bool WEIGHTED>
+__device__ void process_item_fast(const JobDev &J, const WorkItem &W, const int4 *__restrict__ rows,
+                                  const int *__restrict__ slot_regions, char *smem, const Lds &L) {
+	using HT = typename HistT<WEIGHTED>::T;
+	constexpr int TILE = TILE_FAST;
+	const int tid = threadIdx.x, lane = tid & (WAVE - 1), wave = tid / WAVE;
+	const int ig = wave % IG, jg = wave / IG;
+	// job constants in registers (the job table lives in global memory)
+	const int nb = J.nb, n1 = J.n1, n2 = J.n2;
+	const bool is_auto = J.is_auto != 0;
+	const bool jk = J.n_jk > 1;
+	const bool s_log = J.s_log != 0;
+	const float s_min = J.s_min, s_ilb_ln2 = J.s_ilb_ln2, s_ils = J.s_ils, s_inv_step = J.s_inv_step;
+	const float mu_c1 = J.mu_c1, mu_c2 = J.mu_c2;
+	const float min_r_sq = J.min_r_sq, max_r_sq = J.max_r_sq;
+	const AxisDev ax1 = J.ax1, ax2 = J.ax2;
+	const CatDev A = J.A, B = J.B;
+	unsigned long long *const full_u = J.full_u, *const jk_u = J.jk_u;
+	double *const full_d = J.full_d, *const jk_d = J.jk_d;
+	const int n_slots = W.n_slots;
+
+	float4 *tfa = (float4 *)(smem + L.tile);   // (o0, o1, o2, gj)
+	float4 *tfb = tfa + TILE;                  // (n0, n1, n2, w)
+	int *tslot = (int *)(tfb + TILE);
+	float4 *cia = (float4 *)(tslot + TILE);    // first side: (o0, o1, o2, w)
+	float4 *cib = cia + ICHUNK;                // (n0, n1, n2, region code)
+	unsigned *ring = (unsigned *)(cib + ICHUNK) + wave * RING;
+	float *tab = (float *)(smem + L.tables);
+	int *sreg = (int *)(smem + L.slots);
+	HT *F = (HT *)(smem + L.hist);
+	HT *H = F + nb;
+
+	for (int k = tid; k < n_slots; k += BLOCK) sreg[k] = slot_regions[W.slot_base + k];
+	const int d1 = J.dim[1], d2 = J.dim[2];
+	const int cx = W.i_cell / (d1 * d2), cy = (W.i_cell / d2) % d1, cz = W.i_cell % d2;
+	const double c0 = J.lo[0] + (cx + 0.5) * J.h, c1 = J.lo[1] + (cy + 0.5) * J.h, c2 = J.lo[2] + (cz + 0.5) * J.h;
+	__syncthreads();
+	if (tid < ICHUNK) {
+		const int g = W.i_begin + min(tid, W.i_count - 1);
+		const int r = A.jk[g];
+		int code = 0;
+		if (jk) {
+			code = -(r + 1);
+			for (int s = 0; s < n_slots; ++s)
+				if (sreg[s] == r) code = s;
+		}
+		cia[tid] = make_float4((float)(A.x[g] - c0), (float)(A.y[g] - c1), (float)(A.z[g] - c2), A.w[g]);
+		cib[tid] = make_float4((float)A.nx[g], (float)A.ny[g], (float)A.nz[g], __int_as_float(code));
+	}
+	__syncthreads();
+	const int li = ig * WAVE + lane;
+	const bool ivalid = li < W.i_count;
+	const bool wave_active = ig * WAVE < W.i_count;
+	const int gi = W.i_begin + min(li, W.i_count - 1);
+	const float4 me = cia[li];
+
+	const float *t_s2 = tab + J.t_s2_off;
+	const float4 *tq = (const float4 *)(tab + J.tq_off);
+	const float s2_lo = tab[J.tB_off];        // below: certainly out of range
+	const float s2_hi = tab[J.tA_off + n1];   // at or above: certainly out of range
+	const unsigned packed_li = (unsigned)li;
+
+	unsigned int n_acc = 0, n_rc = 0;
+
+	auto add_region = [&](int code, int bin, float w) {
+		if (code >= 0) {
+			if (WEIGHTED) atomicAdd((double *)&H[code * nb + bin], (double)w);
+			else atomicAdd((unsigned int *)&H[code * nb + bin], 1u);
+		} else {
+			const size_t o = (size_t)(-code - 1) * nb + bin;
+			if (WEIGHTED) atomicAdd(&jk_d[o], (double)w);
+			else atomicAdd(&jk_u[o], 1ull);
+		}
+	};
+
+	// bin 64 queued pairs (lanes >= n idle)
+	auto consume = [&](unsigned head, unsigned n) {
+		__builtin_amdgcn_wave_barrier();
+		if (lane < (int)n) {
+			const unsigned e = ring[(head + lane) & (RING - 1)];
+			const int l2 = e & 255, q = e >> 8;
+			const float4 ia = cia[l2], ib = cib[l2], a = tfa[q], b = tfb[q];
+			const float dx = ia.x - a.x, dy = ia.y - a.y, dz = ia.z - a.z;
+			const float s2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
+			// s bin: closed-form estimate, checked against the exact thresholds,
+			// proved by the error margins; anything else goes to the exact path
+			const float sv = __builtin_amdgcn_sqrtf(s2) - s_min;
+			int k = s_log ? (int)(s_ilb_ln2 * __builtin_amdgcn_logf(fmaf(s_ils, sv, 1.0f))) : (int)(sv * s_inv_step);
+			k = min(max(k, 0), n1 - 1);
+			const float4 q0 = tq[k], q1 = tq[k + 1];
+			bool sure = (s2 >= q0.y) && (s2 < q1.z);   // T_k + E <= s2 < T_k+1 - E
+			int bin;
+			if (KIND == K_S_MU) {
+				const float p = 0.5f * fmaf(dz, ib.z + b.z, fmaf(dy, ib.y + b.y, dx * (ib.x + b.x)));
+				const float rs = __builtin_amdgcn_rsqf(s2);
+				const float nmu = (float)n2;
+				const float xm = fabsf(p) * rs * nmu;
+				const float del = fmaf(mu_c2, rs, mu_c1);
+				const int km = (int)xm;
+				const float fr = xm - (float)km;
+				sure = sure && (xm < nmu - del) && (fr <= 1.0f - del) && (km == 0 || fr >= del);
+				bin = k * n2 + km;
+			} else {
+				bin = k;
+			}
+			bool ok = true;
+			if (!sure) {
+				++n_rc;
+				const int g1 = W.i_begin + l2, g2 = __float_as_int(a.w);
+				const double xi[3] = {A.x[g1], A.y[g1], A.z[g1]};
+				const double ni[3] = {A.nx[g1], A.ny[g1], A.nz[g1]};
+				const double xj[3] = {B.x[g2], B.y[g2], B.z[g2]};
+				const double nj[3] = {B.nx[g2], B.ny[g2], B.nz[g2]};
+				ok = exact_bin<KIND>(xi, ni, xj, nj, min_r_sq, max_r_sq, t_s2, nullptr, nullptr, ax1, ax2, bin);
+			}
+			if (ok) {
+				++n_acc;
+				const float w = ia.w * b.w;
+				if (WEIGHTED) atomicAdd((double *)&F[bin], (double)w);
+				else atomicAdd((unsigned int *)&F[bin], 1u);
+				if (jk) {
+					const int si = __float_as_int(ib.w), sj = tslot[q];
+					add_region(si, bin, w);
+					if (sj != si) add_region(sj, bin, w);
+				}
+			}
+		}
+		__builtin_amdgcn_wave_barrier();
+	};
+
+	// append the lanes with in == true (pair: this lane's object, tile entry q)
+	unsigned head = 0, tail = 0;
+	auto append = [&](bool in, int q) {
+		const unsigned long long m = __ballot(in);
+		if (in) {
+			const unsigned off = __builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u));
+			ring[(tail + off) & (RING - 1)] = ((unsigned)q << 8) | packed_li;
+		}
+		tail += (unsigned)__popcll(m);
+	};
+
+	for (int r = 0; r < J.n_rows; ++r) {
+		if (!((W.row_mask >> r) & 1ull)) continue;
+		const int4 row = rows[J.row_off + r];
+		const int qx = cx + row.x, qy = cy + row.y;
+		if (qx < 0 || qy < 0 || qx >= J.dim[0] || qy >= d1) continue;
+		if (is_auto && (row.x < 0 || (row.x == 0 && row.y < 0))) continue;
+		int z0 = max(cz + row.z, 0), z1 = min(cz + row.w, d2 - 1);
+		if (is_auto && row.x == 0 && row.y == 0) z0 = max(z0, cz);
+		if (z0 > z1) continue;
+		const int base = (qx * d1 + qy) * d2;
+		const int jb = B.start[base + z0], je = B.start[base + z1 + 1];
+		for (int t0 = jb; t0 < je; t0 += TILE) {
+			const int nt = min(TILE, je - t0);
+			if (tid < nt) {
+				const int gj = t0 + tid;
+				int sl = 0;
+				if (jk) {
+					const int rj = B.jk[gj];
+					sl = -(rj + 1);
+					for (int s = 0; s < n_slots; ++s)
+						if (sreg[s] == rj) sl = s;
+				}
+				tfa[tid] = make_float4((float)(B.x[gj] - c0), (float)(B.y[gj] - c1), (float)(B.z[gj] - c2),
+				                       __int_as_float(gj));
+				tfb[tid] = make_float4((float)B.nx[gj], (float)B.ny[gj], (float)B.nz[gj], B.w[gj]);
+				tslot[tid] = sl;
+			}
+			__syncthreads();
+			if (wave_active) {
+				// screening, two tile entries per step; the ring holds < 64
+				// pairs between steps and at most 191 within one
+				for (int q = jg; q < nt; q += 2 * JG) {
+					const int q1 = q + JG;
+					const bool has1 = q1 < nt;
+					const float4 a0 = tfa[q];
+					const float4 a1 = tfa[has1 ? q1 : q];
+					const float dx0 = me.x - a0.x, dy0 = me.y - a0.y, dz0 = me.z - a0.z;
+					const float dx1 = me.x - a1.x, dy1 = me.y - a1.y, dz1 = me.z - a1.z;
+					const float s20 = fmaf(dz0, dz0, fmaf(dy0, dy0, dx0 * dx0));
+					const float s21 = fmaf(dz1, dz1, fmaf(dy1, dy1, dx1 * dx1));
+					const bool in0 = ivalid && (s20 >= s2_lo) && (s20 < s2_hi) && (!is_auto || __float_as_int(a0.w) > gi);
+					const bool in1 = has1 && ivalid && (s21 >= s2_lo) && (s21 < s2_hi) &&
+					                 (!is_auto || __float_as_int(a1.w) > gi);
+					append(in0, q);
+					append(in1, q1);
+					while (tail - head >= WAVE) {
+						consume(head, WAVE);
+						head += WAVE;
+					}
+				}
+				if (tail != head) {
+					consume(head, tail - head);
+					head = tail;
+				}
+			}
+			__syncthreads();
+		}
+	}
+
+	for (int b = tid; b < nb; b += BLOCK) {
+		HT v = F[b];
+		if (v != (HT)0) {
+			if (WEIGHTED) atomicAdd(&full_d[b], (double)v);
+			else atomicAdd(&full_u[b], (unsigned long long)v);
+			F[b] = (HT)0;
+		}
+	}
+	if (jk) {
+		for (int e = tid; e < n_slots * nb; e += BLOCK) {
+			HT v = H[e];
+			if (v != (HT)0) {
+				const int s_ = e / nb, b = e - s_ * nb;
+				const size_t o = (size_t)sreg[s_] * nb + b;
+				if (WEIGHTED) atomicAdd(&jk_d[o], (double)v);
+				else atomicAdd(&jk_u[o], (unsigned long long)v);
+				H[e] = (HT)0;
+			}
+		}
+	}
+	for (int off = 32; off; off >>= 1) {
+		n_acc += __shfl_xor(n_acc, off);
+		n_rc += __shfl_xor(n_rc, off);
+	}
+	if (lane == 0) {
+		if (n_acc) atomicAdd(&J.counters[0], (unsigned long long)n_acc);
+		if (n_rc) atomicAdd(&J.counters[1], (unsigned long long)n_rc);
+	}
+	__syncthreads();
+}
+
 // One launch: item = blockIdx.x (plain) or pulled from a global queue
 // (persistent, queue != nullptr).  Every block leaves when the queue is
 // drained, so the grid always finishes.
@@ -398,7 +639,10 @@ __global__ void __launch_bounds__(BLOCK) k_count(const JobDev *__restrict__ jobs
 		}
 		if (it >= n_items) break;
 		const WorkItem W = items[it];
-		process_item<KIND, FAST, WEIGHTED>(jobs[W.job], W, rows, slot_regions, smem, L);
+		if constexpr (FAST)
+			process_item_fast<KIND, WEIGHTED>(jobs[W.job], W, rows, slot_regions, smem, L);
+		else
+			process_item<KIND, false, WEIGHTED>(jobs[W.job], W, rows, slot_regions, smem, L);
 		if (!queue) break;
 	}
 }
@@ -408,6 +652,52 @@ __global__ void __launch_bounds__(BLOCK) k_count(const JobDev *__restrict__ jobs
 double now_ms() {
 	return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now().time_since_epoch()).count();
 }
+
+using KernelFn = void (*)(const JobDev *, const WorkItem *, int, unsigned *, const int4 *, const int *, const float *,
+                          int, Lds);
+
+struct PlanLaunch {
+	KernelFn fn;
+	int grid;
+	Lds L;
+	const WorkItem *items;
+	int n_items;
+	unsigned *queue;
+};
+
+struct PlanOut {
+	bool weighted;
+	size_t full_off, jk_off, full_n, jk_n;
+};
+
+// Everything one twopcf_count call needs on the device, kept for replays.
+struct Plan {
+	std::vector<uint64_t> cat_ids;
+	std::vector<void *> owned;
+	JobDev *d_jobs = nullptr;
+	int4 *d_rows = nullptr;
+	int *d_slots = nullptr;
+	float *d_tab = nullptr;
+	unsigned long long *d_pool_u = nullptr, *d_cnt = nullptr;
+	double *d_pool_d = nullptr;
+	unsigned int *d_queue = nullptr;
+	size_t pool_u = 0, pool_d = 0;
+	int n_jobs = 0, n_tables = 0, n_items = 0;
+	double prep_ms = 0;
+	std::vector<PlanLaunch> launches;
+	std::vector<PlanOut> outs;
+	hipGraph_t graph = nullptr;
+	hipGraphExec_t exec = nullptr;
+	void release() {
+		if (exec) (void)hipGraphExecDestroy(exec);
+		if (graph) (void)hipGraphDestroy(graph);
+		exec = nullptr;
+		graph = nullptr;
+		for (void *p : owned) (void)hipFree(p);
+		owned.clear();
+	}
+	~Plan() { release(); }
+};
 
 }  // namespace
 
@@ -420,6 +710,8 @@ struct twopcf_ctx {
 	size_t lds_max = 65536;
 	std::string name;
 	std::vector<twopcf_cat *> cats;
+	std::map<std::string, std::unique_ptr<Plan>> plans;
+	uint64_t next_cat_id = 1;
 };
 
 struct SortedCat {
@@ -452,6 +744,7 @@ struct GridKey {
 
 struct twopcf_cat {
 	twopcf_ctx *ctx = nullptr;
+	uint64_t id = 0;
 	int64_t n = 0;
 	int n_words = 0;
 	double *x = nullptr, *y = nullptr, *z = nullptr, *nx = nullptr, *ny = nullptr, *nz = nullptr;
@@ -641,6 +934,7 @@ int twopcf_ctx_create(int device, twopcf_ctx **out) {
 int twopcf_ctx_destroy(twopcf_ctx *ctx) {
 	if (!ctx) return TWOPCF_E_ARG;
 	(void)hipSetDevice(ctx->device);
+	ctx->plans.clear();
 	while (!ctx->cats.empty()) twopcf_catalog_free(ctx->cats.back());
 	if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
 	if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
@@ -673,6 +967,7 @@ int twopcf_catalog_upload(twopcf_ctx *ctx, const twopcf_cat_desc *d, twopcf_cat 
 	HIPCK(ctx, hipSetDevice(ctx->device));
 	auto *c = new twopcf_cat();
 	c->ctx = ctx;
+	c->id = ctx->next_cat_id++;
 	c->n = d->n;
 	c->n_words = d->n_words;
 	const int64_t n = d->n;
@@ -732,6 +1027,13 @@ int twopcf_catalog_upload(twopcf_ctx *ctx, const twopcf_cat_desc *d, twopcf_cat 
 int twopcf_catalog_free(twopcf_cat *c) {
 	if (!c) return TWOPCF_E_ARG;
 	twopcf_ctx *ctx = c->ctx;
+	if (ctx) {
+		for (auto it = ctx->plans.begin(); it != ctx->plans.end();) {
+			const auto &ids = it->second->cat_ids;
+			if (std::find(ids.begin(), ids.end(), c->id) != ids.end()) it = ctx->plans.erase(it);
+			else ++it;
+		}
+	}
 	for (auto &kv : c->grids) kv.second.release();
 	for (void *p : {(void *)c->x, (void *)c->y, (void *)c->z, (void *)c->nx, (void *)c->ny, (void *)c->nz,
 	                (void *)c->w, (void *)c->iw, (void *)c->jk, (void *)c->masks})
@@ -743,15 +1045,17 @@ int twopcf_catalog_free(twopcf_cat *c) {
 
 int64_t twopcf_catalog_size(const twopcf_cat *c) { return c ? c->n : -1; }
 
-int twopcf_count(twopcf_ctx *ctx, const twopcf_job *jobs, int n_jobs, int method, const twopcf_tuning *tuning,
-                 twopcf_timing *timing) {
-	const double t_start = now_ms();
-	if (!ctx || (!jobs && n_jobs > 0) || n_jobs < 0) return fail(ctx, TWOPCF_E_ARG, "null argument");
-	if (method != TWOPCF_PLAIN_FP64 && method != TWOPCF_FP32_RECHECK) return fail(ctx, TWOPCF_E_ARG, "bad method");
-	HIPCK(ctx, hipSetDevice(ctx->device));
+}  // extern "C"
+
+namespace {
+
+// Build the device-side plan of one twopcf_count call: grids, tables, work
+// items, output pools and the launch list.  Cached per context.
+int build_plan(twopcf_ctx *ctx, const twopcf_job *jobs, int n_jobs, int method, const twopcf_tuning *tuning,
+               Plan &plan) {
 	const bool fast = method == TWOPCF_FP32_RECHECK;
 	int subdiv = fast ? 2 : 1;
-	int max_slots = 4;
+	int max_slots = 8;
 	if (tuning) {
 		if (fast && tuning->cell_subdiv > 0) subdiv = std::min(tuning->cell_subdiv, 4);
 		if (tuning->max_slots > 0) max_slots = std::min(tuning->max_slots, MAX_SLOTS);
@@ -867,7 +1171,7 @@ int twopcf_count(twopcf_ctx *ctx, const twopcf_job *jobs, int n_jobs, int method
 	// table sizes are known up front: every job pushes at most 3 * (n1 + 1)
 	// + (n2 + 1) floats
 	size_t tab_floats = 1;
-	for (auto &H : hj) tab_floats += 3 * (H.ax1.n + 1) + (H.ax2.n + 1) + 8;
+	for (auto &H : hj) tab_floats += 7 * (H.ax1.n + 1) + (H.ax2.n + 1) + 16;
 	const int lds_fixed = (int)(((std::max(tile_bytes(true), tile_bytes(false)) + 15) & ~15) + ((tab_floats * 4 + 15) & ~15) +
 	                            ((MAX_SLOTS * 4 + 15) & ~15));
 	const size_t lds_budget = ctx->lds_max;
@@ -937,6 +1241,14 @@ int twopcf_count(twopcf_ctx *ctx, const twopcf_job *jobs, int n_jobs, int method
 			}
 			D.tA_off = push(A_);
 			D.tB_off = push(B_);
+			while (tables.size() % 4) tables.push_back(0.0f);
+			std::vector<float> Q(4 * H.t_s2.size(), 0.0f);
+			for (size_t k = 0; k < H.t_s2.size(); ++k) {
+				Q[4 * k] = H.t_s2[k];
+				Q[4 * k + 1] = A_[k];
+				Q[4 * k + 2] = B_[k];
+			}
+			D.tq_off = push(Q);
 			D.s_min = H.ax1.min;
 			D.s_log = H.ax1.log ? 1 : 0;
 			D.s_ilb_ln2 = (float)((double)H.ax1.ilb * std::log(2.0));
@@ -1066,14 +1378,11 @@ int twopcf_count(twopcf_ctx *ctx, const twopcf_job *jobs, int n_jobs, int method
 	unsigned long long *d_pool_u = nullptr, *d_cnt = nullptr;
 	double *d_pool_d = nullptr;
 	unsigned int *d_queue = nullptr;
-	std::vector<void *> owned;
-	auto release = [&]() {
-		for (void *p : owned) (void)hipFree(p);
-	};
+	auto release = [&]() { plan.release(); };
 	auto alloc = [&](void **p, size_t bytes) -> int {
 		hipError_t e = hipMalloc(p, std::max<size_t>(bytes, 16));
 		if (e != hipSuccess) return fail(ctx, TWOPCF_E_HIP, std::string("hipMalloc: ") + hipGetErrorString(e));
-		owned.push_back(*p);
+		plan.owned.push_back(*p);
 		return 0;
 	};
 	int rc = 0;
@@ -1124,16 +1433,23 @@ int twopcf_count(twopcf_ctx *ctx, const twopcf_job *jobs, int n_jobs, int method
 		release();
 		return rc;
 	}
-	// clear outputs with a fill kernel (graph-safe), not a memset
-	auto fill = [&](unsigned long long *p, size_t n) {
-		if (n) k_fill_u64<<<(unsigned)((n + 255) / 256), 256, 0, ctx->stream>>>(p, (int64_t)n, 0ull);
-	};
-	fill(d_pool_u, pool_u);
-	fill((unsigned long long *)d_pool_d, pool_d);   // +0.0 has all bits clear
-	fill(d_cnt, 2 * (size_t)n_jobs);
-	fill((unsigned long long *)d_queue, 32);
-	HIPCK(ctx, hipGetLastError());
 	const double t_prep1 = now_ms();
+	plan.prep_ms = t_prep1 - t_prep0;
+	plan.d_jobs = d_jobs;
+	plan.d_rows = d_rows;
+	plan.d_slots = d_slots;
+	plan.d_tab = d_tab;
+	plan.d_pool_u = d_pool_u;
+	plan.d_pool_d = d_pool_d;
+	plan.d_cnt = d_cnt;
+	plan.d_queue = d_queue;
+	plan.pool_u = pool_u;
+	plan.pool_d = pool_d;
+	plan.n_jobs = n_jobs;
+	plan.n_tables = (int)tables.size();
+	plan.n_items = (int)items.size();
+	for (int q = 0; q < n_jobs; ++q)
+		plan.outs.push_back(PlanOut{prep[q].weighted, prep[q].full_off, prep[q].jk_off, prep[q].full_n, prep[q].jk_n});
 
 	// ---- launches ----
 	// LDS: tile + tables + slot regions + histograms
@@ -1149,8 +1465,6 @@ int twopcf_count(twopcf_ctx *ctx, const twopcf_job *jobs, int n_jobs, int method
 		L.total = L.hist + nb * (1 + slots) * (weighted ? 8 : 4);
 		return L;
 	};
-	HIPCK(ctx, hipEventRecord(ctx->ev0, ctx->stream));
-	int n_launch = 0;
 	auto launch = [&](int kind, bool weighted, const WorkItem *it, int n_it, int nb, int slots, unsigned *queue,
 	                  int grid) -> int {
 		const bool fast_kernel = fast && kind != K_SIGMA_PI;
@@ -1172,10 +1486,7 @@ int twopcf_count(twopcf_ctx *ctx, const twopcf_job *jobs, int n_jobs, int method
 #undef PICK
 		HIPCK(ctx, hipFuncSetAttribute((const void *)fn, hipFuncAttributeMaxDynamicSharedMemorySize, L.total));
 		if (n_it <= 0) return 0;
-		hipLaunchKernelGGL(fn, dim3(grid), dim3(BLOCK), L.total, ctx->stream, d_jobs, it, n_it, queue, d_rows,
-		                   d_slots, d_tab, (int)tables.size(), L);
-		HIPCK(ctx, hipGetLastError());
-		++n_launch;
+		plan.launches.push_back(PlanLaunch{fn, grid, L, it, n_it, queue});
 		return 0;
 	};
 	if (!fast) {
@@ -1231,22 +1542,109 @@ int twopcf_count(twopcf_ctx *ctx, const twopcf_job *jobs, int n_jobs, int method
 			++qi;
 		}
 	}
+	return 0;
+}
+
+// Enqueue one execution of a plan on the context stream: clear the outputs
+// with fill kernels (a graph may hold them; no memset nodes) and launch.
+int enqueue_plan(twopcf_ctx *ctx, Plan &plan) {
+	auto fill = [&](unsigned long long *p, size_t n) {
+		if (n) k_fill_u64<<<(unsigned)((n + 255) / 256), 256, 0, ctx->stream>>>(p, (int64_t)n, 0ull);
+	};
+	fill(plan.d_pool_u, plan.pool_u);
+	fill((unsigned long long *)plan.d_pool_d, plan.pool_d);   // +0.0 has all bits clear
+	fill(plan.d_cnt, 2 * (size_t)plan.n_jobs);
+	fill((unsigned long long *)plan.d_queue, 32);
+	HIPCK(ctx, hipGetLastError());
+	for (auto &l : plan.launches) {
+		hipLaunchKernelGGL(l.fn, dim3(l.grid), dim3(BLOCK), l.L.total, ctx->stream, plan.d_jobs, l.items, l.n_items,
+		                   l.queue, plan.d_rows, plan.d_slots, plan.d_tab, plan.n_tables, l.L);
+		HIPCK(ctx, hipGetLastError());
+	}
+	return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int twopcf_count(twopcf_ctx *ctx, const twopcf_job *jobs, int n_jobs, int method, const twopcf_tuning *tuning,
+                 twopcf_timing *timing) {
+	const double t_start = now_ms();
+	if (!ctx || (!jobs && n_jobs > 0) || n_jobs < 0) return fail(ctx, TWOPCF_E_ARG, "null argument");
+	if (method != TWOPCF_PLAIN_FP64 && method != TWOPCF_FP32_RECHECK) return fail(ctx, TWOPCF_E_ARG, "bad method");
+	for (int q = 0; q < n_jobs; ++q)
+		if (!jobs[q].a || !jobs[q].full) return fail(ctx, TWOPCF_E_ARG, "job needs a catalogue and a full[] output");
+	HIPCK(ctx, hipSetDevice(ctx->device));
+	// plan cache key: method, tuning and every field that shapes the plan
+	std::string key;
+	auto put = [&](const void *p, size_t n) { key.append((const char *)p, n); };
+	put(&method, sizeof method);
+	twopcf_tuning tu{};
+	if (tuning) tu = *tuning;
+	put(&tu, sizeof tu);
+	std::vector<uint64_t> ids;
+	for (int q = 0; q < n_jobs; ++q) {
+		const twopcf_job &J = jobs[q];
+		const uint64_t ia = J.a->id, ib = J.b ? J.b->id : 0;
+		ids.push_back(ia);
+		if (J.b) ids.push_back(ib);
+		put(&J.kind, sizeof J.kind);
+		put(&ia, 8);
+		put(&ib, 8);
+		put(J.ax, sizeof J.ax);
+		put(&J.n_jk, sizeof J.n_jk);
+		put(&J.weight_mode, sizeof J.weight_mode);
+		put(&J.n_runs, sizeof J.n_runs);
+	}
+	Plan *plan = nullptr;
+	bool built = false;
+	auto it = ctx->plans.find(key);
+	if (it != ctx->plans.end()) {
+		plan = it->second.get();
+	} else {
+		auto np = std::make_unique<Plan>();
+		int rc = build_plan(ctx, jobs, n_jobs, method, tuning, *np);
+		if (rc) return rc;
+		np->cat_ids = ids;
+		// capture the launch sequence once; replays run the whole batch as one graph
+		if (!getenv("TWOPCF_NO_GRAPH")) {
+			hipGraph_t g = nullptr;
+			HIPCK(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+			rc = enqueue_plan(ctx, *np);
+			hipError_t e = hipStreamEndCapture(ctx->stream, &g);
+			if (rc) return rc;
+			if (e != hipSuccess) return fail(ctx, TWOPCF_E_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+			np->graph = g;
+			HIPCK(ctx, hipGraphInstantiate(&np->exec, g, nullptr, nullptr, 0));
+		}
+		plan = np.get();
+		ctx->plans[key] = std::move(np);
+		built = true;
+	}
+	HIPCK(ctx, hipEventRecord(ctx->ev0, ctx->stream));
+	if (plan->exec) {
+		HIPCK(ctx, hipGraphLaunch(plan->exec, ctx->stream));
+	} else {
+		int rc = enqueue_plan(ctx, *plan);
+		if (rc) return rc;
+	}
 	HIPCK(ctx, hipEventRecord(ctx->ev1, ctx->stream));
 
 	// ---- results ----
+	const size_t pool_u = plan->pool_u, pool_d = plan->pool_d;
 	std::vector<unsigned long long> hu(pool_u), hcnt(2 * (size_t)n_jobs);
 	std::vector<double> hd(pool_d);
-	if (pool_u) HIPCK(ctx, hipMemcpyAsync(hu.data(), d_pool_u, pool_u * 8, hipMemcpyDeviceToHost, ctx->stream));
-	if (pool_d) HIPCK(ctx, hipMemcpyAsync(hd.data(), d_pool_d, pool_d * 8, hipMemcpyDeviceToHost, ctx->stream));
-	if (n_jobs) HIPCK(ctx, hipMemcpyAsync(hcnt.data(), d_cnt, hcnt.size() * 8, hipMemcpyDeviceToHost, ctx->stream));
+	if (pool_u) HIPCK(ctx, hipMemcpyAsync(hu.data(), plan->d_pool_u, pool_u * 8, hipMemcpyDeviceToHost, ctx->stream));
+	if (pool_d) HIPCK(ctx, hipMemcpyAsync(hd.data(), plan->d_pool_d, pool_d * 8, hipMemcpyDeviceToHost, ctx->stream));
+	if (n_jobs) HIPCK(ctx, hipMemcpyAsync(hcnt.data(), plan->d_cnt, hcnt.size() * 8, hipMemcpyDeviceToHost, ctx->stream));
 	HIPCK(ctx, hipStreamSynchronize(ctx->stream));
 	float kms = 0.0f;
 	HIPCK(ctx, hipEventElapsedTime(&kms, ctx->ev0, ctx->ev1));
-	release();
 	uint64_t tot_acc = 0, tot_rc = 0;
 	for (int q = 0; q < n_jobs; ++q) {
 		const twopcf_job &J = jobs[q];
-		Prepared &P = prep[q];
+		const PlanOut &P = plan->outs[q];
 		for (size_t b = 0; b < P.full_n; ++b)
 			J.full[b] = P.weighted ? hd[P.full_off + b] : (double)hu[P.full_off + b];
 		if (J.jk) {
@@ -1263,10 +1661,10 @@ int twopcf_count(twopcf_ctx *ctx, const twopcf_job *jobs, int n_jobs, int method
 	}
 	if (timing) {
 		timing->kernel_ms = kms;
-		timing->prep_ms = t_prep1 - t_prep0;
+		timing->prep_ms = built ? plan->prep_ms : 0.0;
 		timing->total_ms = now_ms() - t_start;
-		timing->n_launches = n_launch;
-		timing->n_work_items = (int)items.size();
+		timing->n_launches = (int)plan->launches.size();
+		timing->n_work_items = plan->n_items;
 		timing->n_pairs = tot_acc;
 		timing->n_rechecked = tot_rc;
 	}
