@@ -49,6 +49,13 @@ def build(verbose_isa=False):
     run([os.path.join(ROCM, "bin", "hipcc"), f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp,
          dev_o, host_o, "-lm"])
     os.replace(tmp, LIB)
+    # the reference-interface host layer and the parameter-file driver
+    host = os.path.join(CSRC, "host")
+    drv = os.path.join(HERE, "twopcf")
+    run(["g++", "-O2", "-std=c++17", "-ffp-contract=off", "-fno-fast-math", "-Wall",
+         os.path.join(host, "twopcf_ref_api.cpp"), os.path.join(host, "main.cpp"),
+         "-L", HERE, "-ltwopcf_hip", f"-Wl,-rpath,{HERE}", "-Wl,-rpath,$ORIGIN", "-o", drv + ".tmp"])
+    os.replace(drv + ".tmp", drv)
     return LIB
 
 

@@ -1,0 +1,23 @@
+// twopcf — the parameter-file driver (main.cpp:12-75 of the reference) on the
+// GPU counting library.  Usage: twopcf <param-file>
+#include <iostream>
+
+#include "twopcf_ref_api.hpp"
+
+int main(int argc, char *argv[]) {
+	std::cout << "--TWOPCF correlation function calculator (MI355X)--\n\n";
+	if (argc != 2) {
+		std::cout << "Incorrect number of arguments given, exiting\n";
+		return 1;
+	}
+	std::cout << "Correct number of arguments passed\n";
+	Params params(argv[1]);
+	GalaxyCatalogue *data = read_file(params.data_filename, params.data_file_type, params);
+	GalaxyCatalogue *randoms = read_file(params.random_filename, params.random_file_type, params);
+	if (params.plot_monopole) auto_correlate_monopole(data, randoms, params);
+	if (params.plot_sigma_pi) auto_correlate_sigma_pi(data, randoms, params);
+	if (params.plot_s_mu) auto_correlate_s_mu(data, randoms, params);
+	delete data;
+	delete randoms;
+	return 0;
+}

@@ -1,0 +1,653 @@
+// twopcf_ref_api.cpp — see twopcf_ref_api.hpp.  Compiled with
+// -ffp-contract=off so the host-side float/double arithmetic (Axis constants,
+// bin centres, normalisation, LS estimator) rounds as in the reference build.
+#include "twopcf_ref_api.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <fstream>
+#include <iostream>
+#include <sstream>
+
+// ------------------------------------------------------------------ context --
+
+twopcf_ctx *twopcf_default_ctx() {
+	static twopcf_ctx *ctx = nullptr;
+	if (!ctx) {
+		const char *dev = std::getenv("TWOPCF_DEVICE");
+		int rc = twopcf_ctx_create(dev ? std::atoi(dev) : 0, &ctx);
+		if (rc) {
+			std::fprintf(stderr, "twopcf: no usable GPU (twopcf_ctx_create returned %d); there is no CPU path\n", rc);
+			std::exit(1);
+		}
+	}
+	return ctx;
+}
+
+int twopcf_default_method() {
+	const char *m = std::getenv("TWOPCF_METHOD");
+	if (m && std::string(m) == "fp64") return TWOPCF_PLAIN_FP64;
+	return TWOPCF_FP32_RECHECK;
+}
+
+static void check(int rc, const char *what) {
+	if (rc) {
+		std::fprintf(stderr, "twopcf: %s failed (%d): %s\n", what, rc, twopcf_last_error(twopcf_default_ctx()));
+		std::exit(1);
+	}
+}
+
+// ------------------------------------------------------------------- galaxy --
+
+// galaxy.cpp:8-27 (the library's eq_to_cart is the bit-exact restatement)
+void Galaxy::set(double ra, double dec) {
+	double d = d_comov;
+	twopcf_eq_to_cart(1, &ra, &dec, &d, &x[0], &x[1], &x[2], &x_norm[0], &x_norm[1], &x_norm[2]);
+}
+
+GalaxyCatalogue::GalaxyCatalogue(int n, int n_jk) : n_jk_regions(n_jk), n_gals(n) { gals = new Galaxy[n > 0 ? n : 1]; }
+GalaxyCatalogue::~GalaxyCatalogue() { delete[] gals; }
+
+// --------------------------------------------------------------------- hist --
+
+// hist.cpp:15-36
+Axis::Axis(float input_min, float input_max, int input_n_bins, float log_base) {
+	min = input_min;
+	minus_min = -input_min;
+	max = input_max;
+	n_bins = input_n_bins;
+	step = (max - min) / (float)n_bins;
+	inv_step = (float)(1.0 / (double)step);
+	base = log_base;
+	if ((double)base > 1.02) {
+		inv_log_of_base = (float)(1.0 / (double)logf(log_base));
+		inv_log_step = (float)((std::pow((double)log_base, (double)input_n_bins) - 1.0) / (double)(input_max - input_min));
+		log_binning = true;
+	} else {
+		inv_log_of_base = 0.0f;
+		inv_log_step = 0.0f;
+		log_binning = false;
+	}
+}
+
+double Axis::bin_centre(const int b) { return (value_from_bin(b + 1.0) + value_from_bin((double)b)) / 2.0; }
+double Axis::bin_width(const int b) { return value_from_bin(b + 1.0) - value_from_bin((double)b); }
+
+float Axis::bin_num(const float value) {
+	if (log_binning) return inv_log_of_base * log1pf(inv_log_step * (value - min));
+	return (value + minus_min) * inv_step;
+}
+
+double Axis::value_from_bin(const double b) {
+	if (log_binning) return (std::exp(b / (double)inv_log_of_base) - 1.0) / (double)inv_log_step + (double)min;
+	return (double)min + b * (double)step;
+}
+
+twopcf_axis Axis::desc() const { return twopcf_axis{(double)min, (double)max, n_bins, (double)base}; }
+
+Hist::Hist(float mn, float mx, int n, float b) : ax(mn, mx, n, b) { bins = (double *)calloc(ax.n_bins, sizeof(double)); }
+Hist::Hist(const Hist &o) : ax(o.ax.min, o.ax.max, o.ax.n_bins, o.ax.base) {
+	bins = (double *)calloc(ax.n_bins, sizeof(double));
+	std::memcpy(bins, o.bins, sizeof(double) * ax.n_bins);
+}
+Hist::~Hist() { free(bins); }
+
+JKHists::JKHists(int n_jk, const Hist &t) : n_jk_regions(n_jk), full_hist(t) {
+	sub_hist = (double *)calloc((size_t)n_jk * full_hist.ax.n_bins, sizeof(double));
+}
+JKHists::~JKHists() { free(sub_hist); }
+
+// hist.cpp:270-288
+void JKHists::invert_jk_hists() {
+	const int nb = full_hist.ax.n_bins;
+	for (int r = 0; r < n_jk_regions; ++r)
+		for (int b = 0; b < nb; ++b) sub_hist[r * nb + b] = full_hist.bins[b] - sub_hist[r * nb + b];
+}
+void JKHists::multiply_hists(double f) {
+	const int nb = full_hist.ax.n_bins;
+	for (int b = 0; b < nb; ++b) full_hist.bins[b] *= f;
+	for (int k = 0; k < n_jk_regions * nb; ++k) sub_hist[k] *= f;
+}
+// hist.cpp:224-231
+void JKHists::divide_by_hist(const JKHists &o) {
+	const int nb = full_hist.ax.n_bins;
+	for (int b = 0; b < nb; ++b) full_hist.bins[b] /= o.full_hist.bins[b];
+	for (int k = 0; k < n_jk_regions * nb; ++k) sub_hist[k] /= o.sub_hist[k];
+}
+
+// Weight sums over the first n objects (hist.cpp:296-303, 518-525).
+static void weight_sums(const GalaxyCatalogue *cat, int n, int n_jk, double &sw, std::vector<double> &swj,
+                        std::vector<int> &nj) {
+	sw = 0.0;
+	swj.assign(n_jk, 0.0);
+	nj.assign(n_jk, 0);
+	for (int i = 0; i < n; ++i) {
+		sw += cat->gals[i].weight;
+		swj[cat->gals[i].jk_region] += cat->gals[i].weight;
+		++nj[cat->gals[i].jk_region];
+	}
+	for (int r = 0; r < n_jk; ++r) swj[r] = sw - swj[r];
+}
+
+// hist.cpp:290-321 / 512-543 (the region factor uses the count inside the
+// region, as the reference does)
+static void normalise_auto(double *full, double *sub, int nb, int n_jk, const GalaxyCatalogue *cat, int n) {
+	double sw;
+	std::vector<double> swj;
+	std::vector<int> nj;
+	weight_sums(cat, n, n_jk, sw, swj, nj);
+	sw = sw * sw * (1.0 - 1.0 / (double)n);
+	for (int r = 0; r < n_jk; ++r) swj[r] = swj[r] * swj[r] * (1.0 - 1.0 / nj[r]);
+	for (int b = 0; b < nb; ++b) full[b] /= sw;
+	for (int r = 0; r < n_jk; ++r)
+		for (int b = 0; b < nb; ++b) sub[r * nb + b] /= swj[r];
+}
+
+// hist.cpp:323-367 / 545-589
+static void normalise_cross(double *full, double *sub, int nb, int n_jk, const GalaxyCatalogue *c1, int n1,
+                            const GalaxyCatalogue *c2, int n2) {
+	double s1, s2;
+	std::vector<double> j1, j2;
+	std::vector<int> k1, k2;
+	weight_sums(c1, n1, n_jk, s1, j1, k1);
+	weight_sums(c2, n2, n_jk, s2, j2, k2);
+	s1 *= s2;
+	for (int r = 0; r < n_jk; ++r) j1[r] *= j2[r];
+	for (int b = 0; b < nb; ++b) full[b] /= s1;
+	for (int r = 0; r < n_jk; ++r)
+		for (int b = 0; b < nb; ++b) sub[r * nb + b] /= j1[r];
+}
+
+void JKHists::normalise(const GalaxyCatalogue *cat, int n) {
+	normalise_auto(full_hist.bins, sub_hist, full_hist.ax.n_bins, n_jk_regions, cat, n);
+}
+void JKHists::normalise(const GalaxyCatalogue *c1, int n1, const GalaxyCatalogue *c2, int n2) {
+	normalise_cross(full_hist.bins, sub_hist, full_hist.ax.n_bins, n_jk_regions, c1, n1, c2, n2);
+}
+
+// hist.cpp:369-391
+double JKHists::interp(const double value) {
+	Axis &a = full_hist.ax;
+	if (value < a.min) return full_hist.bins[0];
+	if (value > a.max) return full_hist.bins[a.n_bins - 1];
+	float bn = (float)((double)a.bin_num((float)value) - 0.5);
+	if (bn < 0) return full_hist.bins[0];
+	if (bn > a.n_bins - 1) return full_hist.bins[a.n_bins - 1];
+	const int k = (int)bn;
+	return full_hist.bins[k] + (full_hist.bins[k + 1] - full_hist.bins[k]) * (value - a.bin_centre(k)) /
+	                               (a.bin_centre(k + 1) - a.bin_centre(k));
+}
+
+Hist2D::Hist2D(float mn1, float mx1, int n1, float b1, float mn2, float mx2, int n2, float b2)
+	: ax1(mn1, mx1, n1, b1), ax2(mn2, mx2, n2, b2) {
+	total_bins = ax1.n_bins * ax2.n_bins;
+	bins = (double *)calloc(total_bins, sizeof(double));
+}
+Hist2D::Hist2D(const Hist2D &o)
+	: ax1(o.ax1.min, o.ax1.max, o.ax1.n_bins, o.ax1.base), ax2(o.ax2.min, o.ax2.max, o.ax2.n_bins, o.ax2.base) {
+	total_bins = ax1.n_bins * ax2.n_bins;
+	bins = (double *)calloc(total_bins, sizeof(double));
+	std::memcpy(bins, o.bins, sizeof(double) * total_bins);
+}
+Hist2D::~Hist2D() { free(bins); }
+
+JKHists2D::JKHists2D(int n_jk, const Hist2D &t) : n_jk_regions(n_jk), full_hist(t) {
+	sub_hist = (double *)calloc((size_t)n_jk * full_hist.total_bins, sizeof(double));
+}
+JKHists2D::~JKHists2D() { free(sub_hist); }
+void JKHists2D::invert_jk_hists() {
+	const int nb = full_hist.total_bins;
+	for (int r = 0; r < n_jk_regions; ++r)
+		for (int b = 0; b < nb; ++b) sub_hist[r * nb + b] = full_hist.bins[b] - sub_hist[r * nb + b];
+}
+void JKHists2D::multiply_hists(double f) {
+	const int nb = full_hist.total_bins;
+	for (int b = 0; b < nb; ++b) full_hist.bins[b] *= f;
+	for (int k = 0; k < n_jk_regions * nb; ++k) sub_hist[k] *= f;
+}
+void JKHists2D::normalise(const GalaxyCatalogue *cat, int n) {
+	normalise_auto(full_hist.bins, sub_hist, full_hist.total_bins, n_jk_regions, cat, n);
+}
+void JKHists2D::normalise(const GalaxyCatalogue *c1, int n1, const GalaxyCatalogue *c2, int n2) {
+	normalise_cross(full_hist.bins, sub_hist, full_hist.total_bins, n_jk_regions, c1, n1, c2, n2);
+}
+
+// -------------------------------------------------------------------- boxes --
+
+// Objects without a usable distance (d_comov < 0) go behind n_gals
+// (boxes.cpp:16-26); the first n_gals are uploaded to the GPU.
+static int partition_valid(Galaxy *g, int n) {
+	Galaxy *mid = std::stable_partition(g, g + n, [](const Galaxy &a) { return !(a.d_comov < 0.0); });
+	return (int)(mid - g);
+}
+
+static twopcf_cat *upload(Galaxy *g, int n, const std::vector<uint64_t> *masks, int n_words) {
+	std::vector<double> c[6];
+	for (auto &v : c) v.resize(n);
+	std::vector<float> w(n), iw(n);
+	std::vector<int32_t> jk(n);
+	for (int i = 0; i < n; ++i) {
+		for (int k = 0; k < 3; ++k) {
+			c[k][i] = g[i].x[k];
+			c[3 + k][i] = g[i].x_norm[k];
+		}
+		w[i] = g[i].weight;
+		iw[i] = g[i].input_weight;
+		jk[i] = g[i].jk_region;
+	}
+	twopcf_cat_desc d{};
+	d.n = n;
+	d.x = c[0].data();
+	d.y = c[1].data();
+	d.z = c[2].data();
+	d.nx = c[3].data();
+	d.ny = c[4].data();
+	d.nz = c[5].data();
+	d.w = w.data();
+	d.jk = jk.data();
+	d.iw = iw.data();
+	if (masks && n_words > 0) {
+		d.masks = masks->data();
+		d.n_words = n_words;
+	}
+	twopcf_cat *cat = nullptr;
+	check(twopcf_catalog_upload(twopcf_default_ctx(), &d, &cat), "twopcf_catalog_upload");
+	return cat;
+}
+
+GalaxyBoxes::GalaxyBoxes(Galaxy in[], int n, Galaxy *, int, double r) : rmax(r) {
+	n_gals = partition_valid(in, n);
+	cat = upload(in, n_gals, nullptr, 0);
+}
+
+GalaxyBoxes::GalaxyBoxes(GalaxyCatalogue *in, GalaxyCatalogue *, double r) : rmax(r), source(in) {
+	// keep the masks aligned with the partitioned objects
+	const int n = in->n_gals, nw = in->n_words;
+	std::vector<int> order(n);
+	for (int i = 0; i < n; ++i) order[i] = i;
+	auto mid = std::stable_partition(order.begin(), order.end(), [&](int i) { return !(in->gals[i].d_comov < 0.0); });
+	n_gals = (int)(mid - order.begin());
+	std::vector<Galaxy> g(n);
+	std::vector<uint64_t> m((size_t)n * nw);
+	for (int i = 0; i < n; ++i) {
+		g[i] = in->gals[order[i]];
+		for (int k = 0; k < nw; ++k) m[(size_t)i * nw + k] = in->masks[(size_t)order[i] * nw + k];
+	}
+	std::copy(g.begin(), g.end(), in->gals);
+	in->masks.swap(m);
+	cat = upload(in->gals, n_gals, &in->masks, nw);
+}
+
+GalaxyBoxes::~GalaxyBoxes() {
+	if (cat) twopcf_catalog_free(cat);
+}
+
+// -------------------------------------------------------------- correlators --
+
+namespace {
+
+struct Out {
+	double *full;
+	double *sub;
+	int nb, n_jk;
+	bool is_auto;
+};
+
+// Run a batch of counts in one call and leave each histogram as the
+// reference's pair-count functions do: jackknife sums inverted
+// (hist.cpp:464-471) and, for auto counts, multiplied by 2 (hist.cpp:473-482).
+void run(std::vector<twopcf_job> &jobs, std::vector<Out> &outs) {
+	std::vector<std::vector<double>> raw(jobs.size());
+	for (size_t q = 0; q < jobs.size(); ++q) {
+		raw[q].assign((size_t)outs[q].n_jk * outs[q].nb, 0.0);
+		jobs[q].full = outs[q].full;
+		jobs[q].jk = raw[q].data();
+	}
+	check(twopcf_count(twopcf_default_ctx(), jobs.data(), (int)jobs.size(), twopcf_default_method(), nullptr, nullptr),
+	      "twopcf_count");
+	for (size_t q = 0; q < jobs.size(); ++q) {
+		const Out &o = outs[q];
+		std::memcpy(o.sub, raw[q].data(), sizeof(double) * raw[q].size());
+		for (int r = 0; r < o.n_jk; ++r)
+			for (int b = 0; b < o.nb; ++b) o.sub[r * o.nb + b] = o.full[b] - o.sub[r * o.nb + b];
+		if (o.is_auto) {
+			for (int b = 0; b < o.nb; ++b) o.full[b] *= 2.0;
+			for (int k = 0; k < o.n_jk * o.nb; ++k) o.sub[k] *= 2.0;
+		}
+	}
+}
+
+twopcf_job job1(int kind, const GalaxyBoxes *a, const GalaxyBoxes *b, JKHists *h) {
+	twopcf_job j{};
+	j.kind = kind;
+	j.a = a->cat;
+	j.b = b ? b->cat : nullptr;
+	j.ax[0] = h->full_hist.ax.desc();
+	j.ax[1] = twopcf_axis{0.0, 1.0, 1, 0.0};
+	j.n_jk = h->n_jk_regions;
+	j.weight_mode = TWOPCF_W_PLAIN;
+	return j;
+}
+
+twopcf_job job2(int kind, const GalaxyBoxes *a, const GalaxyBoxes *b, JKHists2D *h) {
+	twopcf_job j{};
+	j.kind = kind;
+	j.a = a->cat;
+	j.b = b ? b->cat : nullptr;
+	j.ax[0] = h->full_hist.ax1.desc();
+	j.ax[1] = h->full_hist.ax2.desc();
+	j.n_jk = h->n_jk_regions;
+	j.weight_mode = TWOPCF_W_PLAIN;
+	return j;
+}
+
+Out out1(JKHists *h, bool a) { return Out{h->full_hist.bins, h->sub_hist, h->full_hist.ax.n_bins, h->n_jk_regions, a}; }
+Out out2(JKHists2D *h, bool a) {
+	return Out{h->full_hist.bins, h->sub_hist, h->full_hist.total_bins, h->n_jk_regions, a};
+}
+
+void count1(int kind, const GalaxyBoxes *a, const GalaxyBoxes *b, JKHists *h) {
+	std::vector<twopcf_job> j{job1(kind, a, b, h)};
+	std::vector<Out> o{out1(h, b == nullptr)};
+	run(j, o);
+}
+void count2(int kind, const GalaxyBoxes *a, const GalaxyBoxes *b, JKHists2D *h) {
+	std::vector<twopcf_job> j{job2(kind, a, b, h)};
+	std::vector<Out> o{out2(h, b == nullptr)};
+	run(j, o);
+}
+
+}  // namespace
+
+void auto_pair_count_monopole(const GalaxyBoxes *c, JKHists *h) { count1(TWOPCF_MONOPOLE, c, nullptr, h); }
+void cross_pair_count_monopole(const GalaxyBoxes *a, const GalaxyBoxes *b, JKHists *h) { count1(TWOPCF_MONOPOLE, a, b, h); }
+void auto_pair_count_sigma_pi(const GalaxyBoxes *c, JKHists2D *h) { count2(TWOPCF_SIGMA_PI, c, nullptr, h); }
+void cross_pair_count_sigma_pi(const GalaxyBoxes *a, const GalaxyBoxes *b, JKHists2D *h) { count2(TWOPCF_SIGMA_PI, a, b, h); }
+void auto_pair_count_s_mu(const GalaxyBoxes *c, JKHists2D *h) { count2(TWOPCF_S_MU, c, nullptr, h); }
+void cross_pair_count_s_mu(const GalaxyBoxes *a, const GalaxyBoxes *b, JKHists2D *h) { count2(TWOPCF_S_MU, a, b, h); }
+
+// correlators.cpp:127-154 — DD, DR and RR in one batched call
+void auto_correlate_monopole(GalaxyCatalogue *data, GalaxyCatalogue *randoms, Params p) {
+	std::cout << "Calculating autocorrelation monopole\n";
+	Hist t(p.monopole_min, p.monopole_max, p.monopole_n_bins, p.monopole_log_base);
+	JKHists DD(p.n_jk_regions, t), DR(p.n_jk_regions, t), RR(p.n_jk_regions, t);
+	GalaxyBoxes db(data, randoms, p.monopole_max);
+	GalaxyBoxes rb(randoms, data, p.monopole_max);
+	std::vector<twopcf_job> jobs{job1(TWOPCF_MONOPOLE, &db, nullptr, &DD), job1(TWOPCF_MONOPOLE, &db, &rb, &DR),
+	                             job1(TWOPCF_MONOPOLE, &rb, nullptr, &RR)};
+	std::vector<Out> outs{out1(&DD, true), out1(&DR, false), out1(&RR, true)};
+	run(jobs, outs);
+	DD.normalise(data, db.n_gals);
+	DR.normalise(data, db.n_gals, randoms, rb.n_gals);
+	RR.normalise(randoms, rb.n_gals);
+	output_xi_1D(p.monopole_filename, p.monopole_output_type, &DD, &DR, &RR, data, randoms);
+}
+
+// correlators.cpp:280-307
+void auto_correlate_sigma_pi(GalaxyCatalogue *data, GalaxyCatalogue *randoms, Params p) {
+	std::cout << "Calculating autocorrelation sigma-pi decomposition\n";
+	Hist2D t(p.sigma_min, p.sigma_max, p.sigma_n_bins, p.sigma_log_base, p.pi_min, p.pi_max, p.pi_n_bins,
+	         p.pi_log_base);
+	JKHists2D DD(p.n_jk_regions, t), DR(p.n_jk_regions, t), RR(p.n_jk_regions, t);
+	const double rmax = std::sqrt(p.sigma_max * p.sigma_max + p.pi_max * p.pi_max);
+	GalaxyBoxes db(data, randoms, rmax);
+	GalaxyBoxes rb(randoms, data, rmax);
+	std::vector<twopcf_job> jobs{job2(TWOPCF_SIGMA_PI, &db, nullptr, &DD), job2(TWOPCF_SIGMA_PI, &db, &rb, &DR),
+	                             job2(TWOPCF_SIGMA_PI, &rb, nullptr, &RR)};
+	std::vector<Out> outs{out2(&DD, true), out2(&DR, false), out2(&RR, true)};
+	run(jobs, outs);
+	DD.normalise(data, db.n_gals);
+	DR.normalise(data, db.n_gals, randoms, rb.n_gals);
+	RR.normalise(randoms, rb.n_gals);
+	output_xi_2D(p.sigma_pi_filename, p.sigma_pi_output_type, "sigma", "pi", &DD, &DR, &RR, data, randoms);
+}
+
+// correlators.cpp:432-458
+void auto_correlate_s_mu(GalaxyCatalogue *data, GalaxyCatalogue *randoms, Params p) {
+	std::cout << "Calculating autocorrelation s-mu decomposition\n";
+	Hist2D t(p.s_min, p.s_max, p.s_n_bins, p.s_log_base, 0.0, 1.0, p.mu_n_bins, 0);
+	JKHists2D DD(p.n_jk_regions, t), DR(p.n_jk_regions, t), RR(p.n_jk_regions, t);
+	GalaxyBoxes db(data, randoms, p.s_max);
+	GalaxyBoxes rb(randoms, data, p.s_max);
+	std::vector<twopcf_job> jobs{job2(TWOPCF_S_MU, &db, nullptr, &DD), job2(TWOPCF_S_MU, &db, &rb, &DR),
+	                             job2(TWOPCF_S_MU, &rb, nullptr, &RR)};
+	std::vector<Out> outs{out2(&DD, true), out2(&DR, false), out2(&RR, true)};
+	run(jobs, outs);
+	DD.normalise(data, db.n_gals);
+	DR.normalise(data, db.n_gals, randoms, rb.n_gals);
+	RR.normalise(randoms, rb.n_gals);
+	output_xi_2D(p.s_mu_filename, p.s_mu_output_type, "s", "mu", &DD, &DR, &RR, data, randoms);
+}
+
+// ----------------------------------------------------------------------- io --
+
+static std::string strip_spaces(std::string s) {
+	s.erase(std::remove(s.begin(), s.end(), ' '), s.end());
+	return s;
+}
+
+// io.cpp:16-237: "key = value", '#' comments, spaces removed
+Params::Params(std::string fn) : param_filename(fn) {
+	std::cout << "Reading parameter file: " << fn << "\n";
+	std::ifstream file(fn);
+	std::string line, key, value;
+	while (std::getline(file, line)) {
+		if (line.empty() || line[0] == '#' || line.find('=') == std::string::npos) continue;
+		std::stringstream nt(line);
+		std::getline(nt, line, '#');
+		std::stringstream ss(line);
+		std::getline(ss, key, '=');
+		std::getline(ss, value);
+		key = strip_spaces(key);
+		value = strip_spaces(value);
+		auto I = [&](int &v) { v = std::stoi(value); };
+		auto B = [&](bool &v) { v = std::stoi(value) != 0; };
+		auto D = [&](double &v) { v = std::stod(value); };
+		if (key == "data_filename") data_filename = value;
+		else if (key == "data_file_type") data_file_type = value;
+		else if (key == "random_filename") random_filename = value;
+		else if (key == "random_file_type") random_file_type = value;
+		else if (key == "coord_system") coord_system = value;
+		else if (key == "ra_x_dataset_name") ra_x_dataset_name = value;
+		else if (key == "dec_y_dataset_name") dec_y_dataset_name = value;
+		else if (key == "z_z_dataset_name") z_z_dataset_name = value;
+		else if (key == "weight_dataset_name") weight_dataset_name = value;
+		else if (key == "jk_dataset_name") jk_dataset_name = value;
+		else if (key == "use_weights") B(use_weights);
+		else if (key == "n_threads") I(n_threads);
+		else if (key == "n_jk_regions") I(n_jk_regions);
+		else if (key == "omega_m") D(omega_m);
+		else if (key == "h") D(h);
+		else if (key == "z_min") D(z_min);
+		else if (key == "z_max") D(z_max);
+		else if (key == "plot_monopole") B(plot_monopole);
+		else if (key == "monopole_filename") monopole_filename = value;
+		else if (key == "monopole_output_type") monopole_output_type = value;
+		else if (key == "monopole_log_base") D(monopole_log_base);
+		else if (key == "monopole_min") D(monopole_min);
+		else if (key == "monopole_max") D(monopole_max);
+		else if (key == "monopole_n_bins") I(monopole_n_bins);
+		else if (key == "plot_sigma_pi") B(plot_sigma_pi);
+		else if (key == "sigma_pi_filename") sigma_pi_filename = value;
+		else if (key == "sigma_pi_output_type") sigma_pi_output_type = value;
+		else if (key == "sigma_log_base") D(sigma_log_base);
+		else if (key == "sigma_min") D(sigma_min);
+		else if (key == "sigma_max") D(sigma_max);
+		else if (key == "sigma_n_bins") I(sigma_n_bins);
+		else if (key == "pi_log_base") D(pi_log_base);
+		else if (key == "pi_min") D(pi_min);
+		else if (key == "pi_max") D(pi_max);
+		else if (key == "pi_n_bins") I(pi_n_bins);
+		else if (key == "plot_s_mu") B(plot_s_mu);
+		else if (key == "s_mu_filename") s_mu_filename = value;
+		else if (key == "s_mu_output_type") s_mu_output_type = value;
+		else if (key == "s_log_base") D(s_log_base);
+		else if (key == "s_min") D(s_min);
+		else if (key == "s_max") D(s_max);
+		else if (key == "s_n_bins") I(s_n_bins);
+		else if (key == "mu_n_bins") I(mu_n_bins);
+		else if (key == "calculate_angular_dd") B(calculate_angular_dd);
+		else if (key == "angular_dd_filename") angular_dd_filename = value;
+		else if (key == "calculate_angular_dd_invpweights") B(calculate_angular_dd_invpweights);
+		else if (key == "angular_dd_invpweights_filename") angular_dd_invpweights_filename = value;
+		else if (key == "calculate_angular_dr") B(calculate_angular_dr);
+		else if (key == "angular_dr_filename") angular_dr_filename = value;
+		else if (key == "calculate_angular_dr_invpweights") B(calculate_angular_dr_invpweights);
+		else if (key == "angular_dr_invpweights_filename") angular_dr_invpweights_filename = value;
+		else if (key == "theta_max") D(theta_max);
+		else if (key == "theta_n_bins") I(theta_n_bins);
+		else if (key == "theta_log_base") D(theta_log_base);
+		else if (key == "healpix_order") I(healpix_order);
+		else if (key == "bitwise_weight_dataset_name") bitwise_weight_dataset_name = value;
+		else if (key == "dither_weight_dataset_name") dither_weight_dataset_name = value;
+		else if (key == "n_bitwise_runs") I(n_bitwise_runs);
+		else std::cout << "Key: " << key << " KEY NOT FOUND\n";
+	}
+	if (n_jk_regions <= 1) n_jk_regions = 1;
+}
+
+// Next non-empty field of an ascii line, spaces removed (io.cpp:287-303).
+static std::string next_field(std::stringstream &ss, char sep) {
+	std::string part;
+	do {
+		if (!std::getline(ss, part, sep)) return std::string();
+	} while (part.empty());
+	return strip_spaces(part);
+}
+
+// io.cpp:240-415
+GalaxyCatalogue *read_data_ascii(std::string filename, const Params &p) {
+	std::ifstream file(filename);
+	if (!file.is_open()) {
+		std::cout << "File not open\n";
+		return new GalaxyCatalogue(0, p.n_jk_regions);
+	}
+	std::vector<std::string> lines;
+	std::string line;
+	while (std::getline(file, line))
+		if (!line.empty() && line[0] != '#') lines.push_back(line);
+	const int n = (int)lines.size();
+	auto *cat = new GalaxyCatalogue(n, p.n_jk_regions);
+	const bool eq = p.coord_system == "equatorial";
+	if (!eq && p.coord_system != "cartesian") {
+		std::cout << "Coord system not found\n";
+		return cat;
+	}
+	std::vector<double> c1(n), c2(n), c3(n);
+	for (int i = 0; i < n; ++i) {
+		const std::string &l = lines[i];
+		char sep = ' ';
+		if (l.find('\t') != std::string::npos) sep = '\t';
+		if (l.find(',') != std::string::npos) sep = ',';
+		std::stringstream ss(l);
+		c1[i] = std::stod(next_field(ss, sep));
+		c2[i] = std::stod(next_field(ss, sep));
+		c3[i] = std::stod(next_field(ss, sep));
+		Galaxy &g = cat->gals[i];
+		g.weight = p.use_weights ? std::stof(next_field(ss, sep)) : 1.0f;
+		g.jk_region = p.n_jk_regions > 1 ? std::stoi(next_field(ss, sep)) : 0;
+		g.input_weight = g.weight;
+		if (g.jk_region < 0 || g.jk_region >= p.n_jk_regions) {
+			std::fprintf(stderr, "twopcf: %s line %d: jackknife region %d outside [0, %d)\n", filename.c_str(), i + 1,
+			             g.jk_region, p.n_jk_regions);
+			std::exit(1);
+		}
+	}
+	if (eq) {
+		std::vector<double> d(n);
+		check(twopcf_comoving_distance(p.omega_m, p.h, p.z_min, p.z_max, n, c3.data(), d.data()),
+		      "twopcf_comoving_distance");
+		for (int i = 0; i < n; ++i) {
+			cat->gals[i].d_comov = d[i];
+			cat->gals[i].set(c1[i], c2[i]);
+		}
+	} else {
+		for (int i = 0; i < n; ++i) {
+			Galaxy &g = cat->gals[i];
+			g.x[0] = c1[i];
+			g.x[1] = c2[i];
+			g.x[2] = c3[i];
+			g.d_comov = std::sqrt(g.x[0] * g.x[0] + g.x[1] * g.x[1] + g.x[2] * g.x[2]);
+			for (int k = 0; k < 3; ++k) g.x_norm[k] = g.x[k] / g.d_comov;
+		}
+	}
+	return cat;
+}
+
+// io.cpp:601-627
+GalaxyCatalogue *read_file(std::string filename, std::string file_type, const Params &p) {
+	std::cout << "\nReading file: " << filename << "\n";
+	if (file_type == "ascii") return read_data_ascii(filename, p);
+	std::cout << "Can't find file type for " << filename << " (this build reads ascii catalogues)\n";
+	return new GalaxyCatalogue(0, p.n_jk_regions);
+}
+
+// io.cpp:630-641
+std::string generate_sub_filename(std::string filename, long long sub_number) {
+	size_t dot = filename.find_last_of('.');
+	if (dot == std::string::npos) return filename + std::to_string(sub_number);
+	std::string ext = filename.substr(dot);
+	std::string r = filename;
+	r.erase(dot);
+	r.append(std::to_string(sub_number));
+	r.append(ext);
+	return r;
+}
+
+// io.cpp:643-651
+double ls_estimator(double DD, double DR, double RR) {
+	if (RR == 0) return 0;
+	return DD / RR - 2.0 * DR / RR + 1.0;
+}
+
+// io.cpp:724-737 (default stream precision, tab separated)
+static void single_1d(std::string fn, Axis *ax, double *DD, double *DR, double *RR) {
+	std::ofstream f(fn);
+	for (int i = 0; i < ax->n_bins; ++i) {
+		f << i << "\t" << ax->bin_centre(i) << "\t" << ax->bin_width(i) << "\t" << ls_estimator(DD[i], DR[i], RR[i])
+		  << "\t" << DD[i] << "\t" << DR[i] << "\t" << RR[i] << "\n";
+	}
+}
+
+// io.cpp:836-859
+static void single_2d(std::string fn, Axis *a1, Axis *a2, double *DD, double *DR, double *RR) {
+	std::ofstream f(fn);
+	for (int i = 0; i < a1->n_bins; ++i)
+		for (int j = 0; j < a2->n_bins; ++j) {
+			int b = i * a2->n_bins + j;
+			f << i << "\t" << j << "\t" << a1->bin_centre(i) << "\t" << a2->bin_centre(j) << "\t" << a1->bin_width(i)
+			  << "\t" << a2->bin_width(j) << "\t" << ls_estimator(DD[b], DR[b], RR[b]) << "\t" << DD[b] << "\t" << DR[b]
+			  << "\t" << RR[b] << "\n";
+		}
+}
+
+// io.cpp:741-834 (ascii branch)
+void output_xi_1D(std::string fn, std::string type, JKHists *DD, JKHists *DR, JKHists *RR, GalaxyCatalogue *,
+                  GalaxyCatalogue *) {
+	if (type != "ascii") {
+		std::cout << "Output file type not found\n";
+		return;
+	}
+	const int nb = DD->full_hist.ax.n_bins;
+	single_1d(fn, &DD->full_hist.ax, DD->full_hist.bins, DR->full_hist.bins, RR->full_hist.bins);
+	for (int r = 0; r < DD->n_jk_regions; ++r)
+		single_1d(generate_sub_filename(fn, r), &DD->full_hist.ax, &DD->sub_hist[r * nb], &DR->sub_hist[r * nb],
+		          &RR->sub_hist[r * nb]);
+}
+
+// io.cpp:861-975 (ascii branch)
+void output_xi_2D(std::string fn, std::string type, std::string, std::string, JKHists2D *DD, JKHists2D *DR,
+                  JKHists2D *RR, GalaxyCatalogue *, GalaxyCatalogue *) {
+	if (type != "ascii") {
+		std::cout << "File type not found\n";
+		return;
+	}
+	const int nb = DD->full_hist.total_bins;
+	single_2d(fn, &DD->full_hist.ax1, &DD->full_hist.ax2, DD->full_hist.bins, DR->full_hist.bins, RR->full_hist.bins);
+	for (int r = 0; r < DD->n_jk_regions; ++r)
+		single_2d(generate_sub_filename(fn, r), &DD->full_hist.ax1, &DD->full_hist.ax2, &DD->sub_hist[r * nb],
+		          &DR->sub_hist[r * nb], &RR->sub_hist[r * nb]);
+}

@@ -1,0 +1,189 @@
+// twopcf_ref_api.hpp — the reference's host-side interface (galaxy.h, hist.h,
+// boxes.h, io.h, correlators.h) restated on top of libtwopcf_hip.
+//
+// The names, argument meanings and post-conditions follow the reference so
+// that a caller of auto_/cross_pair_count_* or auto_correlate_* can switch
+// without changes; the pair loops themselves run on the GPU through the C ABI
+// in include/twopcf_hip.h.  Differences from the reference, on purpose:
+//   * GalaxyBoxes does not cell-sort the caller's catalogue (the GPU grids its
+//     own copy); it only moves objects with d_comov < 0 behind n_gals, as
+//     boxes.cpp:16-26 does, which is all normalise() relies on.
+//   * Params members are zero-initialised instead of left undefined when a
+//     key is missing (io.cpp:16-237 leaves them uninitialised).
+//   * Only ascii catalogues and ascii output are supported (the _HAVE_HDF5
+//     parts need libhdf5).
+#pragma once
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../../include/twopcf_hip.h"
+
+// galaxy.h:16-35 (the bitmask lives in GalaxyCatalogue::masks, n_words per object)
+class Galaxy {
+public:
+	double x[3];
+	float weight;
+	int jk_region;
+	double d_comov;
+	double x_norm[3];
+	float input_weight;
+	void set(double ra, double dec);   // galaxy.cpp:8-27
+};
+
+class GalaxyCatalogue {
+public:
+	int n_jk_regions;
+	int n_gals;
+	Galaxy *gals;
+	int n_words = 0;
+	std::vector<uint64_t> masks;   // n_gals * n_words
+	GalaxyCatalogue(int n_gals, int n_jk_regions);
+	~GalaxyCatalogue();
+	GalaxyCatalogue(const GalaxyCatalogue &) = delete;
+	GalaxyCatalogue &operator=(const GalaxyCatalogue &) = delete;
+};
+
+// hist.h:5-19
+class Axis {
+public:
+	int n_bins;
+	bool log_binning;
+	float min, minus_min, max, step, inv_step, base, inv_log_of_base, inv_log_step;
+	Axis(float input_min, float input_max, int input_n_bins, float log_base);
+	double bin_centre(const int bin_num);
+	double bin_width(const int bin_num);
+	double value_from_bin(const double bin_num);
+	float bin_num(const float value);
+	twopcf_axis desc() const;
+};
+
+class Hist {
+public:
+	Axis ax;
+	double *bins;
+	Hist(float input_min, float input_max, int input_n_bins, float log_base);
+	Hist(const Hist &copy_obj);
+	~Hist();
+};
+
+class JKHists {
+public:
+	int n_jk_regions;
+	Hist full_hist;
+	double *sub_hist;
+	JKHists(int jk_regions, const Hist &template_hist);
+	~JKHists();
+	void invert_jk_hists();
+	void multiply_hists(double factor);
+	void divide_by_hist(const JKHists &input_hist);
+	double get(const int bin_num) { return full_hist.bins[bin_num]; }
+	double get(const int jk_region, const int bin_num) { return sub_hist[jk_region * full_hist.ax.n_bins + bin_num]; }
+	void normalise(const GalaxyCatalogue *cat, int n_gals);
+	void normalise(const GalaxyCatalogue *cat_1, int n_gals_1, const GalaxyCatalogue *cat_2, int n_gals_2);
+	double interp(const double value);
+};
+
+class Hist2D {
+public:
+	Axis ax1, ax2;
+	double *bins;
+	int total_bins;
+	Hist2D(float input_min_1, float input_max_1, int input_n_bins_1, float axis_1_log_base, float input_min_2,
+	       float input_max_2, int input_n_bins_2, float axis_2_log_base);
+	Hist2D(const Hist2D &copy_obj);
+	~Hist2D();
+};
+
+class JKHists2D {
+public:
+	int n_jk_regions;
+	Hist2D full_hist;
+	double *sub_hist;
+	JKHists2D(int jk_regions, const Hist2D &template_hist);
+	~JKHists2D();
+	void invert_jk_hists();
+	void multiply_hists(double factor);
+	double get(const int b1, const int b2) { return full_hist.bins[b1 * full_hist.ax2.n_bins + b2]; }
+	void normalise(const GalaxyCatalogue *cat, int n_gals);
+	void normalise(const GalaxyCatalogue *cat_1, int n_gals_1, const GalaxyCatalogue *cat_2, int n_gals_2);
+};
+
+// boxes.h:16-36: here the device-resident catalogue of the first n_gals objects
+class GalaxyBoxes {
+public:
+	double rmax;
+	int n_gals;
+	twopcf_cat *cat = nullptr;
+	const GalaxyCatalogue *source = nullptr;
+	GalaxyBoxes(Galaxy input_galaxies[], int input_n_gals, Galaxy other_cat[], int other_cat_n_gals,
+	            double input_rmax);
+	GalaxyBoxes(GalaxyCatalogue *input, GalaxyCatalogue *other, double input_rmax);
+	~GalaxyBoxes();
+	GalaxyBoxes(const GalaxyBoxes &) = delete;
+	GalaxyBoxes &operator=(const GalaxyBoxes &) = delete;
+};
+
+// io.h:10-86
+class Params {
+public:
+	std::string param_filename, data_filename, data_file_type, random_filename, random_file_type, coord_system;
+	std::string ra_x_dataset_name, dec_y_dataset_name, z_z_dataset_name, weight_dataset_name, jk_dataset_name;
+	bool use_weights = false;
+	int n_threads = 0;
+	int n_jk_regions = 1;
+	double omega_m = 0, h = 0, z_min = 0, z_max = 0;
+	bool plot_monopole = false;
+	std::string monopole_filename, monopole_output_type;
+	double monopole_log_base = 0, monopole_min = 0, monopole_max = 0;
+	int monopole_n_bins = 0;
+	bool plot_sigma_pi = false;
+	std::string sigma_pi_filename, sigma_pi_output_type;
+	double sigma_log_base = 0, sigma_min = 0, sigma_max = 0;
+	int sigma_n_bins = 0;
+	double pi_log_base = 0, pi_min = 0, pi_max = 0;
+	int pi_n_bins = 0;
+	bool plot_s_mu = false;
+	std::string s_mu_filename, s_mu_output_type;
+	double s_log_base = 0, s_min = 0, s_max = 0;
+	int s_n_bins = 0, mu_n_bins = 0;
+	bool calculate_angular_dd = false, calculate_angular_dd_invpweights = false;
+	bool calculate_angular_dr = false, calculate_angular_dr_invpweights = false;
+	std::string angular_dd_filename, angular_dd_invpweights_filename, angular_dr_filename,
+	    angular_dr_invpweights_filename;
+	double theta_max = 0;
+	int theta_n_bins = 0;
+	double theta_log_base = 0;
+	int healpix_order = 0;
+	std::string bitwise_weight_dataset_name, dither_weight_dataset_name;
+	int n_bitwise_runs = 0;
+	Params(std::string input_filename);
+};
+
+GalaxyCatalogue *read_data_ascii(std::string filename, const Params &params);
+GalaxyCatalogue *read_file(std::string filename, std::string file_type, const Params &params);
+double ls_estimator(double DD, double DR, double RR);
+std::string generate_sub_filename(std::string filename, long long sub_number);
+void output_xi_1D(std::string filename, std::string file_type, JKHists *DD, JKHists *DR, JKHists *RR,
+                  GalaxyCatalogue *data_cat, GalaxyCatalogue *random_cat);
+void output_xi_2D(std::string filename, std::string file_type, std::string axis_1_name, std::string axis_2_name,
+                  JKHists2D *DD, JKHists2D *DR, JKHists2D *RR, GalaxyCatalogue *data_cat,
+                  GalaxyCatalogue *random_cat);
+
+// correlators.h:7-31 — same names and post-conditions: histograms hold the
+// inverted jackknife sums, multiplied by 2 for auto counts, not normalised.
+void auto_pair_count_monopole(const GalaxyBoxes *cat, JKHists *hist);
+void cross_pair_count_monopole(const GalaxyBoxes *cat_1, const GalaxyBoxes *cat_2, JKHists *hist);
+void auto_pair_count_sigma_pi(const GalaxyBoxes *cat, JKHists2D *hist);
+void cross_pair_count_sigma_pi(const GalaxyBoxes *cat_1, const GalaxyBoxes *cat_2, JKHists2D *hist);
+void auto_pair_count_s_mu(const GalaxyBoxes *cat, JKHists2D *hist);
+void cross_pair_count_s_mu(const GalaxyBoxes *cat_1, const GalaxyBoxes *cat_2, JKHists2D *hist);
+void auto_correlate_monopole(GalaxyCatalogue *data, GalaxyCatalogue *randoms, Params params);
+void auto_correlate_sigma_pi(GalaxyCatalogue *data, GalaxyCatalogue *randoms, Params params);
+void auto_correlate_s_mu(GalaxyCatalogue *data, GalaxyCatalogue *randoms, Params params);
+
+// The context every entry point uses (device TWOPCF_DEVICE, default 0;
+// counting method TWOPCF_METHOD = fp32 (default) or fp64).  Aborts with a
+// message when no GPU is usable: there is no CPU fallback.
+twopcf_ctx *twopcf_default_ctx();
+int twopcf_default_method();
