@@ -1,0 +1,91 @@
+"""GPU parity of the inverse-p pipeline (correlators_invpweight.cpp): angular
+DD/DR with parent and inverse-p weights, the angular up-weight tables built
+from them as calc_angular_upweight and the 3-D correlators do
+(correlators_invpweight.cpp:985-1033, 584-588), then the 3-D s-mu DD with
+bitwise weights, DR with the up-weight, and plain RR.  2048-bit masks (32
+words).  Pair counts must be identical; weighted sums agree to 1e-12
+relative (their summation order differs from the oracle's)."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+N_RUNS = 2048
+WORDS = 32
+
+
+def make(L, n, seed, ones=False, ra_l=(180.0, 190.0), dec_l=(10.0, 20.0), z_l=(0.1, 0.15)):
+    from two_pcf_amd import synth
+    rng = np.random.Generator(np.random.PCG64(seed))
+    if ones:
+        ra, dec = rng.uniform(*ra_l, n), np.degrees(np.arcsin(rng.uniform(np.sin(np.radians(dec_l[0])),
+                                                                            np.sin(np.radians(dec_l[1])), n)))
+        z = rng.uniform(*z_l, n)
+    else:
+        ra, dec, z = synth.neyman_scott_equatorial(max(1, n // 25), 25.0, 4.0, ra_l, dec_l, z_l, seed=seed)
+    m = ra.size
+    d = L.comoving_distance(0.25, 1.0, 0.0, 1.0, z)
+    x, y, zz, nx, ny, nz = L.eq_to_cart(ra, dec, d)
+    if ones:
+        masks = np.full((m, WORDS), np.uint64(0xFFFFFFFFFFFFFFFF), dtype=np.uint64)
+    else:
+        p = rng.uniform(0.3, 0.95, m)
+        bits = rng.random((m, WORDS * 64)) < p[:, None]
+        masks = np.packbits(bits.astype(np.uint8), axis=1, bitorder="little").view(np.uint64).reshape(m, WORDS)
+    iw = rng.uniform(0.5, 1.5, m).astype(np.float32)
+    # inverse-probability weight at read time (io.cpp:578-582)
+    tot = np.array([sum(int(int(v)).bit_count() for v in row) for row in masks])
+    tot[tot == 0] = 1
+    w = np.where(tot != N_RUNS, (iw.astype(np.float64) * (N_RUNS / tot)).astype(np.float32), iw)
+    jk = (ra > 0.5 * (ra_l[0] + ra_l[1])).astype(np.int32)
+    return dict(x=x, y=y, z=zz, nx=nx, ny=ny, nz=nz, w=w, iw=iw, masks=masks, jk=jk)
+
+
+def test_invp_pipeline(oracle_lib):
+    from two_pcf_amd import lib as L
+    o = oracle_lib
+    D = make(L, 2500, 31)
+    R = make(L, 5000, 32, ones=True)
+    oc = {k: o.Catalog(v["x"], v["y"], v["z"], v["nx"], v["ny"], v["nz"], v["w"], v["jk"], iw=v["iw"], masks=v["masks"])
+          for k, v in (("d", D), ("r", R))}
+    th = 0.03
+    ang = (0.0, float(1.0 - np.cos(th)), 16, 1.6)
+    oang = o.axis(*ang)
+    with L.Context(0) as ctx:
+        dc = {k: ctx.upload(v["x"], v["y"], v["z"], v["nx"], v["ny"], v["nz"], v["w"], v["jk"], iw=v["iw"],
+                            masks=v["masks"]) for k, v in (("d", D), ("r", R))}
+        # angular: parent DD (iw), inverse-p DD, parent DR (iw), inverse-p DR (w)
+        spec = [("d", None, L.W_ANG_IW), ("d", None, L.W_ANG_INVP), ("d", "r", L.W_ANG_IW), ("d", "r", L.W_ANG_W)]
+        for method in (0, 1):
+            print(f"[gpu] angular method={method}", flush=True)
+            jobs = [dict(kind=L.ANGULAR, a=dc[a], b=dc[b] if b else None, axes=[ang], n_jk=2, weight_mode=m,
+                         n_runs=N_RUNS) for a, b, m in spec]
+            res, _ = ctx.count(jobs, method=method)
+            for r, (a, b, m) in zip(res, spec):
+                wo = o.WOpt(m, N_RUNS, None, o.Axis())
+                full, sub, acc = o.count(3, oc[a], oc[b] if b else None, oang, None, 2, wopt=wo)
+                assert r["n_pairs"] == acc
+                np.testing.assert_allclose(r["full"], full, rtol=1e-12, atol=0)
+                np.testing.assert_allclose(r["jk"], sub, rtol=1e-12, atol=0)
+        # up-weight tables: DD/DDinvp and DR/DRinvp per bin (ratios of finished hists)
+        fin = [L.finish(b is None, r["full"], r["jk"])[0] for r, (a, b, m) in zip(res, spec)]
+        with np.errstate(invalid="ignore", divide="ignore"):
+            up_dd = fin[0] / fin[1]
+            up_dr = fin[2] / fin[3]
+        up_dd = np.where(np.isfinite(up_dd), up_dd, 1.0)
+        up_dr = np.where(np.isfinite(up_dr), up_dr, 1.0)
+        axes = [(0.0, 40.0, 20, 1.3), (0.0, 1.0, 10, 0.0)]
+        ax1, ax2 = o.axis(*axes[0]), o.axis(*axes[1])
+        spec3 = [("d", None, L.W_INVP_3D, up_dd), ("d", "r", L.W_ANGUP_3D, up_dr), ("r", None, L.W_PLAIN, None)]
+        for method in (0, 1):
+            print(f"[gpu] 3-D inverse-p method={method}", flush=True)
+            jobs = [dict(kind=L.S_MU, a=dc[a], b=dc[b] if b else None, axes=axes, n_jk=2, weight_mode=m,
+                         n_runs=N_RUNS, up_bins=u, up_axis=ang if u is not None else None)
+                    for a, b, m, u in spec3]
+            res3, _ = ctx.count(jobs, method=method)
+            for r, (a, b, m, u) in zip(res3, spec3):
+                wo = o.WOpt(m, N_RUNS, u.ctypes.data if u is not None else None, oang)
+                full, sub, acc = o.count(2, oc[a], oc[b] if b else None, ax1, ax2, 2, wopt=wo)
+                assert r["n_pairs"] == acc
+                np.testing.assert_allclose(r["full"], full, rtol=1e-12, atol=0)
+                np.testing.assert_allclose(r["jk"], sub, rtol=1e-12, atol=0)

@@ -114,6 +114,28 @@ bool value_thresholds(const AxisF &a, std::vector<float> &t) {
 	return true;
 }
 
+bool interp_thresholds(const AxisF &a, std::vector<float> &t) {
+	const uint32_t lo = fkey(-std::numeric_limits<float>::max());
+	const uint32_t hi = fkey(std::numeric_limits<float>::max());
+	t.assign(a.n + 1, 0.0f);
+	for (int k = 0; k <= a.n; ++k) {
+		auto pred = [&](float v) {
+			const float bn = (float)((double)a.index(v) - 0.5);
+			return k < a.n ? (bn >= (float)k) : (bn > (float)(a.n - 1));
+		};
+		uint32_t key = first_true(lo, hi, pred);
+		if (key > hi) {
+			t[k] = std::numeric_limits<float>::infinity();
+			continue;
+		}
+		if (!check_switch(key, lo, hi, pred)) return false;
+		t[k] = keyf(key);
+	}
+	for (int k = 1; k <= a.n; ++k)
+		if (t[k] < t[k - 1]) return false;
+	return true;
+}
+
 bool s2_thresholds(int kind, const AxisF &a1, const AxisF &a2, std::vector<float> &t) {
 	float mn, mx;
 	range_sq(kind, a1, a2, &mn, &mx);

@@ -32,6 +32,11 @@ void range_sq(int kind, const AxisF &a1, const AxisF &a2, float *min_r_sq, float
 // Returns false if the index is not monotone near a threshold.
 bool value_thresholds(const AxisF &a, std::vector<float> &t);
 
+// Segment thresholds of JKHists::interp (hist.cpp:378-389): with
+// bn = float(double(index(float v)) - 0.5), t[k] (k < n) is the first float
+// with bn >= k and t[n] the first with bn > n - 1.
+bool interp_thresholds(const AxisF &a, std::vector<float> &t);
+
 // s^2-domain thresholds of a monopole / s-mu s axis, folding in the range
 // test and sqrtf (correlators.cpp:315-317, 338).  Same convention.
 bool s2_thresholds(int kind, const AxisF &a1, const AxisF &a2, std::vector<float> &t);

@@ -64,6 +64,16 @@ __device__ __forceinline__ float ref_pi(float d0, float d1, float d2, const doub
 	return p;
 }
 
+// 1 - cos(theta) as the angular correlators form it: double, running
+// subtraction of the three products (correlators_invpweight.cpp:738-741).
+__device__ __forceinline__ double ref_one_minus_cos(const double *n1, const double *n2) {
+	double v = 1.0;
+	v -= n1[0] * n2[0];
+	v -= n1[1] * n2[1];
+	v -= n1[2] * n2[2];
+	return v;
+}
+
 // One pair through the reference pipeline.  t_s2: s^2 thresholds (monopole,
 // s-mu); t1/t2: value thresholds of the sigma / pi axes (sigma-pi, log axes);
 // a1/a2: linear-axis constants.  Returns true and the flattened bin if the
@@ -72,6 +82,10 @@ template <int KIND>
 __device__ __forceinline__ bool exact_bin(const double *x1, const double *n1, const double *x2, const double *n2,
                                           float min_r_sq, float max_r_sq, const float *t_s2, const float *t1,
                                           const float *t2, const AxisDev &a1, const AxisDev &a2, int &bin) {
+	if constexpr (KIND == K_ANGULAR) {
+		// value rounded to float for Axis::bin_num (hist.cpp:246-250); no range test
+		return tab_bin(t1, a1.n, (float)ref_one_minus_cos(n1, n2), bin);
+	}
 	float d0 = ref_diff(x1[0], x2[0]);
 	float d1 = ref_diff(x1[1], x2[1]);
 	float d2 = ref_diff(x1[2], x2[2]);

@@ -76,6 +76,11 @@ struct JobDev {
 	int s_log;
 	float mu_c1, mu_c2;                              // fast mu margin: delta = c1 + c2 * rsq(s2)
 	int row_off, n_rows;                             // stencil rows in the row pool
+	int wmode, n_runs, n_words;                      // pair-weight rule (TWOPCF_W_*)
+	int up_n;                                        // angular up-weight table
+	double up_min, up_max;
+	const double *up_bins, *up_centre;
+	const float *up_thr;                             // [up_n + 1] segment thresholds
 	unsigned long long *full_u, *jk_u;
 	double *full_d, *jk_d;
 	unsigned long long *counters;                    // [0] accepted, [1] rechecked
@@ -159,6 +164,61 @@ template <>
 struct HistT<true> {
 	using T = double;
 };
+
+// JKHists::interp (hist.cpp:369-391) on the up-weight table.  The segment
+// (int)(bin_num(float(v)) - 0.5) comes from host-built float thresholds
+// (glibc log1pf is not evaluated on the device); the interpolation is the
+// reference's double expression in the same order.
+__device__ double interp_up(const JobDev &J, double v) {
+	const int n = J.up_n;
+	if (v < J.up_min) return J.up_bins[0];
+	if (v > J.up_max) return J.up_bins[n - 1];
+	const float fv = (float)v;
+	int cnt = 0;
+	for (int k = 0; k <= n; ++k) cnt += (J.up_thr[k] <= fv);
+	const int seg = cnt - 1;
+	if (seg < 0) return J.up_bins[0];
+	if (seg >= n - 1) return J.up_bins[n - 1];
+	const double *b = J.up_bins, *c = J.up_centre;
+	return b[seg] + (b[seg + 1] - b[seg]) * (v - c[seg]) / (c[seg + 1] - c[seg]);
+}
+
+__device__ __forceinline__ float popc_weight(const JobDev &J, int gi, int gj) {
+	const uint64_t *m1 = J.A.masks + (size_t)gi * J.n_words, *m2 = J.B.masks + (size_t)gj * J.n_words;
+	int c = 0;
+	for (int k = 0; k < J.n_words; ++k) c += __popcll(m1[k] & m2[k]);
+	float pc = (float)c;
+	if (pc < 0.5f) pc += 1.0f;
+	return pc;
+}
+
+// Pair weight under the job's rule, in the reference's arithmetic
+// (correlators.cpp:340; correlators_invpweight.cpp:507-533, 574, 743-764,
+// 853, 901).  Every rule is symmetric in the two objects.
+__device__ float pair_weight(const JobDev &J, int gi, int gj, const double *ni, const double *nj) {
+	switch (J.wmode) {
+	case TWOPCF_W_INVP_3D: {
+		const float pc = popc_weight(J, gi, gj);
+		double t = interp_up(J, ref_one_minus_cos(ni, nj)) * (double)J.n_runs;
+		t = t / (double)pc;
+		return J.A.iw[gi] * J.B.iw[gj] * (float)t;
+	}
+	case TWOPCF_W_ANGUP_3D: {
+		const float p = J.A.w[gi] * J.B.w[gj];
+		return (float)((double)p * interp_up(J, ref_one_minus_cos(ni, nj)));
+	}
+	case TWOPCF_W_ANG_IW:
+		return J.A.iw[gi] * J.B.iw[gj];
+	case TWOPCF_W_ANG_INVP: {
+		const float pc = popc_weight(J, gi, gj);
+		return J.A.iw[gi] * J.B.iw[gj] * ((float)J.n_runs / pc);
+	}
+	case TWOPCF_W_ANG_W:
+	case TWOPCF_W_PLAIN:
+	default:
+		return J.A.w[gi] * J.B.w[gj];
+	}
+}
 
 template <int KIND, bool FAST, bool WEIGHTED>
 __device__ void process_item(const JobDev &J, const WorkItem &W, const int4 *__restrict__ rows,
@@ -334,7 +394,7 @@ __device__ void process_item(const JobDev &J, const WorkItem &W, const int4 *__r
 						if (!exact_bin<KIND>(xi, ni, e, e + 3, J.min_r_sq, J.max_r_sq, t_s2, t1, t2, J.ax1, J.ax2, bin))
 							continue;
 						++n_acc;
-						add(bin, wi * tw[q], tslot[q]);
+						add(bin, J.wmode == TWOPCF_W_PLAIN ? wi * tw[q] : pair_weight(J, gi, gj, ni, e + 3), tslot[q]);
 					}
 				}
 			}
@@ -498,7 +558,13 @@ __device__ void process_item_fast(const JobDev &J, const WorkItem &W, const int4
 			}
 			if (ok) {
 				++n_acc;
-				const float w = ia.w * b.w;
+				float w = ia.w * b.w;
+				if (WEIGHTED && J.wmode != TWOPCF_W_PLAIN) {
+					const int g1 = W.i_begin + l2, g2 = __float_as_int(a.w);
+					const double ni[3] = {A.nx[g1], A.ny[g1], A.nz[g1]};
+					const double nj[3] = {B.nx[g2], B.ny[g2], B.nz[g2]};
+					w = pair_weight(J, g1, g2, ni, nj);
+				}
 				if (WEIGHTED) atomicAdd((double *)&F[bin], (double)w);
 				else atomicAdd((unsigned int *)&F[bin], 1u);
 				if (jk) {
@@ -736,9 +802,10 @@ struct GridKey {
 	double h;
 	int dim[3];
 	int n_region;
+	int on_norm;   // cells over the unit vectors (angular counts) instead of positions
 	bool operator<(const GridKey &o) const {
-		return std::tie(lo[0], lo[1], lo[2], h, dim[0], dim[1], dim[2], n_region) <
-		       std::tie(o.lo[0], o.lo[1], o.lo[2], o.h, o.dim[0], o.dim[1], o.dim[2], o.n_region);
+		return std::tie(lo[0], lo[1], lo[2], h, dim[0], dim[1], dim[2], n_region, on_norm) <
+		       std::tie(o.lo[0], o.lo[1], o.lo[2], o.h, o.dim[0], o.dim[1], o.dim[2], o.n_region, o.on_norm);
 	}
 };
 
@@ -806,7 +873,8 @@ int grid_catalog(twopcf_ctx *ctx, twopcf_cat *c, const GridKey &g, SortedCat **o
 	while (end_bit < 64 && (1ull << end_bit) <= max_key) ++end_bit;
 	const int nblk = (int)((n + 255) / 256);
 	if (n > 0) {
-		k_cell_keys<<<nblk, 256, 0, ctx->stream>>>(c->x, c->y, c->z, c->jk, n, g.lo[0], g.lo[1], g.lo[2], g.h,
+		k_cell_keys<<<nblk, 256, 0, ctx->stream>>>(g.on_norm ? c->nx : c->x, g.on_norm ? c->ny : c->y,
+		                                           g.on_norm ? c->nz : c->z, c->jk, n, g.lo[0], g.lo[1], g.lo[2], g.h,
 		                                           g.dim[0], g.dim[1], g.dim[2], g.n_region, keys, vals);
 		HIPCK(ctx, hipGetLastError());
 		HIPCK(ctx, hipcub::DeviceRadixSort::SortPairs(nullptr, tmp_bytes, keys, keys_s, vals, perm, (int)n, 0,
@@ -1068,7 +1136,9 @@ int build_plan(twopcf_ctx *ctx, const twopcf_job *jobs, int n_jobs, int method, 
 		bool swapped;
 		AxisF ax1, ax2;
 		double rmax;
-		std::vector<float> t_s2, t1, t2, tA, tB;
+		std::vector<float> t_s2, t1, t2, tA, tB, up_thr;
+		std::vector<double> up_bins, up_centre;
+		AxisF up_ax;
 		float min_r_sq, max_r_sq;
 		int nb;
 		bool weighted;
@@ -1080,9 +1150,25 @@ int build_plan(twopcf_ctx *ctx, const twopcf_job *jobs, int n_jobs, int method, 
 		HostJob &H = hj[q];
 		H.j = &J;
 		if (!J.a || !J.full) return fail(ctx, TWOPCF_E_ARG, "job needs a catalogue and a full[] output");
-		if (J.kind < TWOPCF_MONOPOLE || J.kind > TWOPCF_S_MU)
-			return fail(ctx, TWOPCF_E_ARG, "correlator kind not supported by this build");
-		if (J.weight_mode != TWOPCF_W_PLAIN) return fail(ctx, TWOPCF_E_ARG, "weight mode not supported by this build");
+		if (J.kind < TWOPCF_MONOPOLE || J.kind > TWOPCF_ANGULAR) return fail(ctx, TWOPCF_E_ARG, "bad correlator kind");
+		if (J.weight_mode < TWOPCF_W_PLAIN || J.weight_mode > TWOPCF_W_ANG_W)
+			return fail(ctx, TWOPCF_E_ARG, "bad weight mode");
+		const bool needs_masks = J.weight_mode == TWOPCF_W_INVP_3D || J.weight_mode == TWOPCF_W_ANG_INVP;
+		const bool needs_up = J.weight_mode == TWOPCF_W_INVP_3D || J.weight_mode == TWOPCF_W_ANGUP_3D;
+		if (needs_masks) {
+			const twopcf_cat *bb = J.b ? J.b : J.a;
+			if (J.a->n_words < 1 || J.a->n_words != bb->n_words)
+				return fail(ctx, TWOPCF_E_ARG, "inverse-p weights need bit masks of equal width in both catalogues");
+			if (J.n_runs <= 0) return fail(ctx, TWOPCF_E_ARG, "n_runs must be positive");
+		}
+		if (needs_up) {
+			if (!J.up_bins || J.up_axis.n_bins < 2 || J.up_axis.n_bins > MAX_AXIS_BINS)
+				return fail(ctx, TWOPCF_E_ARG, "angular up-weight table missing or too short");
+			H.up_ax = AxisF((float)J.up_axis.min, (float)J.up_axis.max, J.up_axis.n_bins, (float)J.up_axis.log_base);
+			if (!interp_thresholds(H.up_ax, H.up_thr)) return fail(ctx, TWOPCF_E_AXIS, "up-weight axis not monotone");
+			H.up_bins.assign(J.up_bins, J.up_bins + J.up_axis.n_bins);
+			for (int k = 0; k < H.up_ax.n; ++k) H.up_centre.push_back(H.up_ax.centre(k));
+		}
 		if (J.n_jk < 1) return fail(ctx, TWOPCF_E_ARG, "n_jk must be >= 1");
 		twopcf_cat *a = const_cast<twopcf_cat *>(J.a), *b = const_cast<twopcf_cat *>(J.b);
 		for (twopcf_cat *c : {a, b}) {
@@ -1102,7 +1188,14 @@ int build_plan(twopcf_ctx *ctx, const twopcf_job *jobs, int n_jobs, int method, 
 		else H.ax2 = AxisF(0.0f, 1.0f, 1, 0.0f);
 		H.nb = H.ax1.n * (two_axes ? H.ax2.n : 1);
 		range_sq(J.kind, H.ax1, H.ax2, &H.min_r_sq, &H.max_r_sq);
-		if (J.kind == TWOPCF_SIGMA_PI) {
+		if (J.kind == TWOPCF_ANGULAR) {
+			// cells over the unit vectors, side = the largest chord whose
+			// 1 - cos can still land in the histogram
+			if (!value_thresholds(H.ax1, H.t1)) return fail(ctx, TWOPCF_E_AXIS, "axis index not monotone");
+			const double vmax = std::max(0.0, (double)H.t1[H.ax1.n]);
+			H.rmax = std::sqrt(2.0 * std::min(vmax, 2.0) * (1.0 + 1e-6) + 1e-14);
+			H.max_r_sq = (float)(H.rmax * H.rmax);
+		} else if (J.kind == TWOPCF_SIGMA_PI) {
 			H.rmax = std::sqrt(J.ax[0].max * J.ax[0].max + J.ax[1].max * J.ax[1].max);
 			if (!value_thresholds(H.ax1, H.t1) || !value_thresholds(H.ax2, H.t2))
 				return fail(ctx, TWOPCF_E_AXIS, "axis index not monotone");
@@ -1114,7 +1207,7 @@ int build_plan(twopcf_ctx *ctx, const twopcf_job *jobs, int n_jobs, int method, 
 		// The range test is on float s^2; the grid must cover every float s^2
 		// below max_r_sq.
 		H.rmax = std::max(H.rmax, std::sqrt((double)H.max_r_sq)) * (1.0 + 1e-6);
-		H.weighted = !(a->unit_w && (!b || b->unit_w));
+		H.weighted = !(a->unit_w && (!b || b->unit_w)) || J.weight_mode != TWOPCF_W_PLAIN;
 		H.swapped = b && b->n > a->n;
 		H.a = H.swapped ? b : a;
 		H.b = H.swapped ? a : b;
@@ -1132,11 +1225,13 @@ int build_plan(twopcf_ctx *ctx, const twopcf_job *jobs, int n_jobs, int method, 
 		double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
 		for (auto &G : hj) {
 			if (G.rmax != H.rmax) continue;
+			if ((G.j->kind == TWOPCF_ANGULAR) != (H.j->kind == TWOPCF_ANGULAR)) continue;
 			for (twopcf_cat *c : {G.a, G.b}) {
 				if (!c || c->n == 0) continue;
+				const bool ang = H.j->kind == TWOPCF_ANGULAR;
 				for (int k = 0; k < 3; ++k) {
-					lo[k] = std::min(lo[k], c->lo[k]);
-					hi[k] = std::max(hi[k], c->hi[k]);
+					lo[k] = std::min(lo[k], ang ? c->nlo[k] : c->lo[k]);
+					hi[k] = std::max(hi[k], ang ? c->nhi[k] : c->hi[k]);
 				}
 			}
 		}
@@ -1153,6 +1248,7 @@ int build_plan(twopcf_ctx *ctx, const twopcf_job *jobs, int n_jobs, int method, 
 		}
 		if (ncell > (1LL << 28)) return fail(ctx, TWOPCF_E_LIMIT, "too many cells");
 		H.g.n_region = H.j->n_jk > 1 ? H.j->n_jk : 1;
+		H.g.on_norm = H.j->kind == TWOPCF_ANGULAR ? 1 : 0;
 		if (!rows_of.count(H.rmax)) {
 			auto rows = stencil_rows(subdiv);
 			rows_of[H.rmax] = {(int)row_pool.size(), (int)rows.size()};
@@ -1221,7 +1317,9 @@ int build_plan(twopcf_ctx *ctx, const twopcf_job *jobs, int n_jobs, int method, 
 			tables.insert(tables.end(), v.begin(), v.end());
 			return off;
 		};
-		if (H.j->kind == TWOPCF_SIGMA_PI) {
+		if (H.j->kind == TWOPCF_ANGULAR) {
+			D.t1_off = push(H.t1);
+		} else if (H.j->kind == TWOPCF_SIGMA_PI) {
 			D.t1_off = push(H.t1);
 			D.t2_off = push(H.t2);
 		} else {
@@ -1262,6 +1360,31 @@ int build_plan(twopcf_ctx *ctx, const twopcf_job *jobs, int n_jobs, int method, 
 		auto rr = rows_of[H.rmax];
 		D.row_off = rr.first;
 		D.n_rows = rr.second;
+		D.wmode = H.j->weight_mode;
+		D.n_runs = H.j->n_runs;
+		D.n_words = H.a->n_words;
+		if (!H.up_bins.empty()) {
+			double *ub = nullptr, *uc = nullptr;
+			float *ut = nullptr;
+			const size_t n = H.up_bins.size();
+			for (void **p : {(void **)&ub, (void **)&uc, (void **)&ut}) {
+				hipError_t e = hipMalloc(p, (n + 1) * 8);
+				if (e != hipSuccess) {
+					plan.release();
+					return fail(ctx, TWOPCF_E_HIP, std::string("hipMalloc: ") + hipGetErrorString(e));
+				}
+				plan.owned.push_back(*p);
+			}
+			HIPCK(ctx, hipMemcpy(ub, H.up_bins.data(), n * 8, hipMemcpyHostToDevice));
+			HIPCK(ctx, hipMemcpy(uc, H.up_centre.data(), n * 8, hipMemcpyHostToDevice));
+			HIPCK(ctx, hipMemcpy(ut, H.up_thr.data(), (n + 1) * 4, hipMemcpyHostToDevice));
+			D.up_bins = ub;
+			D.up_centre = uc;
+			D.up_thr = ut;
+			D.up_n = (int)n;
+			D.up_min = (double)H.up_ax.min;
+			D.up_max = (double)H.up_ax.max;
+		}
 		P.full_n = H.nb;
 		P.jk_n = H.j->n_jk > 1 ? (size_t)H.j->n_jk * H.nb : 0;
 		if (H.weighted) {
@@ -1467,7 +1590,7 @@ int build_plan(twopcf_ctx *ctx, const twopcf_job *jobs, int n_jobs, int method, 
 	};
 	auto launch = [&](int kind, bool weighted, const WorkItem *it, int n_it, int nb, int slots, unsigned *queue,
 	                  int grid) -> int {
-		const bool fast_kernel = fast && kind != K_SIGMA_PI;
+		const bool fast_kernel = fast && (kind == K_MONO || kind == K_S_MU);
 		Lds L = layout(fast_kernel, weighted, nb, slots);
 		if ((size_t)L.total > ctx->lds_max)
 			return fail(ctx, TWOPCF_E_LIMIT, "histogram does not fit in LDS (" + std::to_string(L.total) + " B)");
@@ -1477,11 +1600,13 @@ int build_plan(twopcf_ctx *ctx, const twopcf_job *jobs, int n_jobs, int method, 
 		if (fast) {
 			if (kind == K_MONO) { if (weighted) PICK(K_MONO, true, true); else PICK(K_MONO, true, false); }
 			else if (kind == K_S_MU) { if (weighted) PICK(K_S_MU, true, true); else PICK(K_S_MU, true, false); }
-			else { if (weighted) PICK(K_SIGMA_PI, false, true); else PICK(K_SIGMA_PI, false, false); }
+			else if (kind == K_SIGMA_PI) { if (weighted) PICK(K_SIGMA_PI, false, true); else PICK(K_SIGMA_PI, false, false); }
+			else { if (weighted) PICK(K_ANGULAR, false, true); else PICK(K_ANGULAR, false, false); }
 		} else {
 			if (kind == K_MONO) { if (weighted) PICK(K_MONO, false, true); else PICK(K_MONO, false, false); }
 			else if (kind == K_S_MU) { if (weighted) PICK(K_S_MU, false, true); else PICK(K_S_MU, false, false); }
-			else { if (weighted) PICK(K_SIGMA_PI, false, true); else PICK(K_SIGMA_PI, false, false); }
+			else if (kind == K_SIGMA_PI) { if (weighted) PICK(K_SIGMA_PI, false, true); else PICK(K_SIGMA_PI, false, false); }
+			else { if (weighted) PICK(K_ANGULAR, false, true); else PICK(K_ANGULAR, false, false); }
 		}
 #undef PICK
 		HIPCK(ctx, hipFuncSetAttribute((const void *)fn, hipFuncAttributeMaxDynamicSharedMemorySize, L.total));
@@ -1531,7 +1656,7 @@ int build_plan(twopcf_ctx *ctx, const twopcf_job *jobs, int n_jobs, int method, 
 			int slots = 0;
 			for (size_t k = std::get<2>(sp); k < std::get<2>(sp) + std::get<3>(sp); ++k)
 				slots = std::max(slots, flat[k].n_slots);
-			Lds L = layout(kind != K_SIGMA_PI, weighted, nb, slots);
+			Lds L = layout(kind == K_MONO || kind == K_S_MU, weighted, nb, slots);
 			const int per_cu = std::max(1, std::min(8, (int)(ctx->lds_max / std::max(L.total, 1))));
 			const int grid = std::min<long long>((long long)ctx->n_cu * per_cu, (long long)std::get<3>(sp));
 			if ((rc = launch(kind, weighted, d_items + std::get<2>(sp), (int)std::get<3>(sp), nb, slots, d_queue + qi,
@@ -1596,6 +1721,8 @@ int twopcf_count(twopcf_ctx *ctx, const twopcf_job *jobs, int n_jobs, int method
 		put(&J.n_jk, sizeof J.n_jk);
 		put(&J.weight_mode, sizeof J.weight_mode);
 		put(&J.n_runs, sizeof J.n_runs);
+		put(&J.up_axis, sizeof J.up_axis);
+		if (J.up_bins && J.up_axis.n_bins > 0) put(J.up_bins, sizeof(double) * J.up_axis.n_bins);
 	}
 	Plan *plan = nullptr;
 	bool built = false;
