@@ -145,8 +145,10 @@ def main():
     ctx = L.Context(0)
     info = ctx.info()
     log("device", info)
+    t_up = time.perf_counter()
     dc = {t: ctx.upload(a["x"], a["y"], a["z"], a["nx"], a["ny"], a["nz"], a["w"], a["jk"])
           for t, a in cats.items()}
+    upload_ms = 1000.0 * (time.perf_counter() - t_up)
     jobs = [dict(kind=L.S_MU, a=dc["data"], b=None, axes=AXES, n_jk=n_jk),
             dict(kind=L.S_MU, a=dc["data"], b=dc["randoms"], axes=AXES, n_jk=n_jk),
             dict(kind=L.S_MU, a=dc["randoms"], b=None, axes=AXES, n_jk=n_jk)]
@@ -157,8 +159,11 @@ def main():
         tuning["max_slots"] = args.slots
 
     ref = None
+    first_call_ms = None
     for i in range(args.warmup):
         ref, t = ctx.count(jobs, method=L.FP32_RECHECK, tuning=tuning or None)
+        if first_call_ms is None:
+            first_call_ms = t["total_ms"]   # gridding + planning + graph capture + one run
         log(f"warmup {i}: kernel {t['kernel_ms']:.1f} ms, total {t['total_ms']:.1f} ms, "
             f"prep {t['prep_ms']:.1f} ms, items {t['n_work_items']}, rechecked {t['n_rechecked']}")
     kms = []
@@ -205,6 +210,9 @@ def main():
         "kernel_ms": float(np.mean(kms)),
         "kernel_pairs_per_s": pairs / kernel_s,
         "rechecked_pairs": rechecked,
+        "upload_ms": upload_ms,
+        "first_call_ms": first_call_ms,
+        "pcie_inclusive_pairs_per_s": (pairs / ((upload_ms + first_call_ms) / 1000.0)) if first_call_ms else None,
         "plain_fp64": plain,
         "speedup_vs_plain_fp64": (plain["kernel_ms"] / float(np.mean(kms))) if plain else None,
         "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",

@@ -174,3 +174,27 @@ def test_graph_replays_match_eager(ctx, oracle_lib):
         ctx.count([dict(kind=0, a=dc["data"], b=None, axes=[(0.0, 50.0, 10, 1.0)])], method=0)  # eager work between
     for c in dc.values():
         c.free()
+
+
+@pytest.mark.parametrize("subdiv", [1, 3, 4])
+def test_grid_independence(ctx, subdiv):
+    """The batched method's cell size only changes which candidates are
+    examined: every subdivision gives the reference's counts."""
+    from two_pcf_amd import lib as L
+    case = load_case("eq_clustered")
+    print(f"[gpu] grid independence subdiv={subdiv}", flush=True)
+    cats = lib_cats(L, ctx, case.inputs, case.coord)
+    for corr in ("monopole", "s_mu"):
+        if corr not in case.corrs():
+            continue
+        axes = case.axes(corr)
+        jobs = [dict(kind=KIND[corr], a=cats["data"], b=None, axes=axes, n_jk=case.n_jk),
+                dict(kind=KIND[corr], a=cats["data"], b=cats["randoms"], axes=axes, n_jk=case.n_jk)]
+        res, _ = ctx.count(jobs, method=1, tuning={"cell_subdiv": subdiv})
+        for h, r, j in zip(("DD", "DR"), res, jobs):
+            full, sub = L.finish(j["b"] is None, r["full"], r["jk"])
+            gf, gs = case.golden(corr, h)
+            np.testing.assert_array_equal(full, gf, err_msg=f"subdiv {subdiv} {corr} {h} full")
+            np.testing.assert_array_equal(sub, gs, err_msg=f"subdiv {subdiv} {corr} {h} sub")
+    for c in cats.values():
+        c.free()

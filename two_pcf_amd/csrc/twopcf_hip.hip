@@ -6,11 +6,12 @@
 //                region-sorted within a cell: hipCUB radix sort on the key
 //                cell * n_region + region)
 //   cell_start   int[n_cell + 1]
-// Work: a work item is (job, up to 64 first-side objects of one cell, a set
-// of neighbour-cell rows, up to S region slots).  A 256-thread block owns one
-// item at a time: each lane holds one first-side object, the second-side
-// objects of each row are staged through LDS in tiles of 256 and the four
-// waves share every tile.  Per-item histograms live in LDS (u32 counts or
+// Work: a work item is (job, up to 256 first-side objects of one cell, a set
+// of neighbour-cell rows, up to S region slots).  A 1024-thread block (16
+// waves: 4 groups over the first-side objects x 4 groups over each tile) owns
+// one item at a time: each lane holds one first-side object, the second-side
+// objects of each row are staged through LDS in tiles of 512 and every tile
+// is shared by all 16 waves.  Per-item histograms live in LDS (u32 counts or
 // f64 sums) and are flushed once per item with global atomics.
 #include <hip/hip_runtime.h>
 #include <hipcub/hipcub.hpp>
