@@ -131,7 +131,7 @@ typedef struct {
 
 /* Tuning knobs of TWOPCF_FP32_RECHECK (0 = default / autotuned). */
 typedef struct {
-	int32_t cell_subdiv;     /* cells per rmax along each axis (1..4)       */
+	int32_t cell_subdiv;     /* cells per rmax along each axis (1..3; larger values mean 3) */
 	int32_t max_slots;       /* jackknife region slots per work item (1..16) */
 	int32_t i_chunk;         /* objects per work item on the first side     */
 } twopcf_tuning;

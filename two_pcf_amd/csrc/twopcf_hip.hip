@@ -25,6 +25,7 @@
 #include <memory>
 #include <string>
 #include <tuple>
+#include <type_traits>
 #include <vector>
 
 #include "../../include/twopcf_hip.h"
@@ -43,14 +44,16 @@ constexpr int ICHUNK = WAVE * IG;           // first-side objects per work item
 constexpr int TILE_FAST = 512;              // staged second-side objects (fp32 path)
 constexpr int TILE_EXACT = 512;             // staged second-side objects (fp64 path)
 constexpr int RING = 256;                   // per-wave queue of in-range pairs (fp32 path)
+constexpr int RQ = 128;                     // per-wave queue of pairs to re-evaluate exactly (fp32 path)
 __host__ __device__ constexpr int tile_of(bool fast) { return fast ? TILE_FAST : TILE_EXACT; }
 // tile + (fp32 path) first-side chunk and per-wave pair queues
 __host__ __device__ constexpr int tile_bytes(bool fast) {
-	return fast ? TILE_FAST * (2 * 16 + 4) + ICHUNK * 32 + (BLOCK / WAVE) * RING * 4
+	return fast ? TILE_FAST * (2 * 16 + 4) + ICHUNK * 32 + (BLOCK / WAVE) * (RING * 4 + RQ * 8)
 	            : TILE_EXACT * (6 * 8 + 4 + 4 + 4);
 }
 constexpr int MAX_SLOTS = 16;
-constexpr int MAX_ROWS = 64;
+constexpr int MAX_ROWS = 64;                // stencil rows per work item (a u64 row mask)
+constexpr int MAX_SUBDIV = 3;               // cells per rmax; 4 needs more than MAX_ROWS rows
 constexpr int MAX_AXIS_BINS = 4096;
 
 // ------------------------------------------------------------ device data --
@@ -441,26 +444,27 @@ __device__ void process_item(const JobDev &J, const WorkItem &W, const int4 *__r
 // FP32 s^2 from cell-local offsets; pairs that may be in range are appended
 // to the wave's LDS ring.  Binning: whenever 64 pairs are queued, each lane
 // takes one, recomputes the same s^2, bins it and proves the bin with the
-// host-computed margins, or re-evaluates it with the exact pipeline.
+// host-computed margins.  Pairs the margins cannot decide go to the wave's
+// recheck queue (global indices), which is re-evaluated 64 at a time with
+// the exact pipeline, away from the screening loop.
 template <int KIND, bool WEIGHTED>
 __device__ void process_item_fast(const JobDev &J, const WorkItem &W, const int4 *__restrict__ rows,
                                   const int *__restrict__ slot_regions, char *smem, const Lds &L) {
 	using HT = typename HistT<WEIGHTED>::T;
 	constexpr int TILE = TILE_FAST;
-	const int tid = threadIdx.x, lane = tid & (WAVE - 1), wave = tid / WAVE;
-	const int ig = wave % IG, jg = wave / IG;
-	// job constants in registers (the job table lives in global memory)
+	const int tid = threadIdx.x, lane = tid & (WAVE - 1);
+	// wave index, made visibly uniform so that the loops below run on the
+	// scalar unit
+	const int wave = __builtin_amdgcn_readfirstlane(tid / WAVE);
+	// wave groups: ig_n groups over the first-side objects (one object per
+	// lane), jg_n groups over each staged tile; small items use fewer
+	// first-side groups so that no wave idles
+	const int ig_n = W.i_count > 2 * WAVE ? IG : (W.i_count > WAVE ? 2 : 1);
+	const int jg_n = (BLOCK / WAVE) / ig_n;
+	const int ig = wave % ig_n, jg = wave / ig_n;
 	const int nb = J.nb, n1 = J.n1, n2 = J.n2;
 	const bool is_auto = J.is_auto != 0;
 	const bool jk = J.n_jk > 1;
-	const bool s_log = J.s_log != 0;
-	const float s_min = J.s_min, s_ilb_ln2 = J.s_ilb_ln2, s_ils = J.s_ils, s_inv_step = J.s_inv_step;
-	const float mu_c1 = J.mu_c1, mu_c2 = J.mu_c2;
-	const float min_r_sq = J.min_r_sq, max_r_sq = J.max_r_sq;
-	const AxisDev ax1 = J.ax1, ax2 = J.ax2;
-	const CatDev A = J.A, B = J.B;
-	unsigned long long *const full_u = J.full_u, *const jk_u = J.jk_u;
-	double *const full_d = J.full_d, *const jk_d = J.jk_d;
 	const int n_slots = W.n_slots;
 
 	float4 *tfa = (float4 *)(smem + L.tile);   // (o0, o1, o2, gj)
@@ -469,6 +473,7 @@ __device__ void process_item_fast(const JobDev &J, const WorkItem &W, const int4
 	float4 *cia = (float4 *)(tslot + TILE);    // first side: (o0, o1, o2, w)
 	float4 *cib = cia + ICHUNK;                // (n0, n1, n2, region code)
 	unsigned *ring = (unsigned *)(cib + ICHUNK) + wave * RING;
+	int2 *rq = (int2 *)((unsigned *)(cib + ICHUNK) + (BLOCK / WAVE) * RING) + wave * RQ;   // (l2, gj)
 	float *tab = (float *)(smem + L.tables);
 	int *sreg = (int *)(smem + L.slots);
 	HT *F = (HT *)(smem + L.hist);
@@ -481,27 +486,27 @@ __device__ void process_item_fast(const JobDev &J, const WorkItem &W, const int4
 	__syncthreads();
 	if (tid < ICHUNK) {
 		const int g = W.i_begin + min(tid, W.i_count - 1);
-		const int r = A.jk[g];
+		const int r = J.A.jk[g];
 		int code = 0;
 		if (jk) {
 			code = -(r + 1);
 			for (int s = 0; s < n_slots; ++s)
 				if (sreg[s] == r) code = s;
 		}
-		cia[tid] = make_float4((float)(A.x[g] - c0), (float)(A.y[g] - c1), (float)(A.z[g] - c2), A.w[g]);
-		cib[tid] = make_float4((float)A.nx[g], (float)A.ny[g], (float)A.nz[g], __int_as_float(code));
+		cia[tid] = make_float4((float)(J.A.x[g] - c0), (float)(J.A.y[g] - c1), (float)(J.A.z[g] - c2), J.A.w[g]);
+		cib[tid] = make_float4((float)J.A.nx[g], (float)J.A.ny[g], (float)J.A.nz[g], __int_as_float(code));
 	}
 	__syncthreads();
 	const int li = ig * WAVE + lane;
-	const bool ivalid = li < W.i_count;
-	const bool wave_active = ig * WAVE < W.i_count;
+	const unsigned long long valid_mask = __builtin_amdgcn_ballot_w64(li < W.i_count);
 	const int gi = W.i_begin + min(li, W.i_count - 1);
 	const float4 me = cia[li];
 
-	const float *t_s2 = tab + J.t_s2_off;
 	const float4 *tq = (const float4 *)(tab + J.tq_off);
-	const float s2_lo = tab[J.tB_off];        // below: certainly out of range
-	const float s2_hi = tab[J.tA_off + n1];   // at or above: certainly out of range
+	// below s2_lo: certainly out of range; at or above s2_hi: certainly out
+	const float s2_lo = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(tab[J.tB_off])));
+	const float s2_hi = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(tab[J.tA_off + n1])));
+	const bool check_lo = s2_lo > 0.0f;       // with s_min = 0 every s^2 >= 0 passes
 	const unsigned packed_li = (unsigned)li;
 
 	unsigned int n_acc = 0, n_rc = 0;
@@ -512,28 +517,83 @@ __device__ void process_item_fast(const JobDev &J, const WorkItem &W, const int4
 			else atomicAdd((unsigned int *)&H[code * nb + bin], 1u);
 		} else {
 			const size_t o = (size_t)(-code - 1) * nb + bin;
-			if (WEIGHTED) atomicAdd(&jk_d[o], (double)w);
-			else atomicAdd(&jk_u[o], 1ull);
+			if (WEIGHTED) atomicAdd(&J.jk_d[o], (double)w);
+			else atomicAdd(&J.jk_u[o], 1ull);
 		}
 	};
+	auto add = [&](int bin, float w, int si, int sj) {
+		if (WEIGHTED) atomicAdd((double *)&F[bin], (double)w);
+		else atomicAdd((unsigned int *)&F[bin], 1u);
+		if (jk) {
+			add_region(si, bin, w);
+			if (sj != si) add_region(sj, bin, w);
+		}
+	};
+	auto weight = [&](int l2, int g2, float w) {
+		if (WEIGHTED && J.wmode != TWOPCF_W_PLAIN) {
+			const int g1 = W.i_begin + l2;
+			const double ni[3] = {J.A.nx[g1], J.A.ny[g1], J.A.nz[g1]};
+			const double nj[3] = {J.B.nx[g2], J.B.ny[g2], J.B.nz[g2]};
+			w = pair_weight(J, g1, g2, ni, nj);
+		}
+		return w;
+	};
 
-	// bin 64 queued pairs (lanes >= n idle)
+	// recheck queue: the exact pipeline on up to RQ pairs, 64 lanes at a time
+	unsigned rq_n = 0;
+	auto flush_rq = [&]() {
+		__builtin_amdgcn_wave_barrier();
+		for (unsigned b = 0; b < rq_n; b += WAVE) {
+			if (b + lane < rq_n) {
+				const int2 e = rq[b + lane];
+				const int g1 = W.i_begin + e.x, g2 = e.y;
+				const double xi[3] = {J.A.x[g1], J.A.y[g1], J.A.z[g1]};
+				const double ni[3] = {J.A.nx[g1], J.A.ny[g1], J.A.nz[g1]};
+				const double xj[3] = {J.B.x[g2], J.B.y[g2], J.B.z[g2]};
+				const double nj[3] = {J.B.nx[g2], J.B.ny[g2], J.B.nz[g2]};
+				const float *t_s2 = tab + J.t_s2_off;
+				int bin;
+				if (exact_bin<KIND>(xi, ni, xj, nj, J.min_r_sq, J.max_r_sq, t_s2, nullptr, nullptr, J.ax1, J.ax2, bin)) {
+					++n_acc;
+					int sj = 0;
+					if (jk) {
+						const int rj = J.B.jk[g2];
+						sj = -(rj + 1);
+						for (int s = 0; s < n_slots; ++s)
+							if (sreg[s] == rj) sj = s;
+					}
+					const float w = weight(e.x, g2, cia[e.x].w * J.B.w[g2]);
+					add(bin, w, __float_as_int(cib[e.x].w), sj);
+				}
+			}
+		}
+		n_rc += rq_n;
+		rq_n = 0;
+		__builtin_amdgcn_wave_barrier();
+	};
+
+	const bool s_log = J.s_log != 0;
+	const float s_min = J.s_min, s_ilb_ln2 = J.s_ilb_ln2, s_ils = J.s_ils, s_inv_step = J.s_inv_step;
+	const float mu_c1 = J.mu_c1, mu_c2 = J.mu_c2;
+	// bin n queued pairs (lanes >= n idle)
 	auto consume = [&](unsigned head, unsigned n) {
 		__builtin_amdgcn_wave_barrier();
+		bool sure = false;
+		int bin = 0, l2 = 0, q = 0;
 		if (lane < (int)n) {
 			const unsigned e = ring[(head + lane) & (RING - 1)];
-			const int l2 = e & 255, q = e >> 8;
+			l2 = e & 255;
+			q = e >> 8;
 			const float4 ia = cia[l2], ib = cib[l2], a = tfa[q], b = tfb[q];
 			const float dx = ia.x - a.x, dy = ia.y - a.y, dz = ia.z - a.z;
 			const float s2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
-			// s bin: closed-form estimate, checked against the exact thresholds,
-			// proved by the error margins; anything else goes to the exact path
+			// s bin: closed-form estimate, checked against the exact
+			// thresholds, proved by the error margins
 			const float sv = __builtin_amdgcn_sqrtf(s2) - s_min;
 			int k = s_log ? (int)(s_ilb_ln2 * __builtin_amdgcn_logf(fmaf(s_ils, sv, 1.0f))) : (int)(sv * s_inv_step);
 			k = min(max(k, 0), n1 - 1);
 			const float4 q0 = tq[k], q1 = tq[k + 1];
-			bool sure = (s2 >= q0.y) && (s2 < q1.z);   // T_k + E <= s2 < T_k+1 - E
-			int bin;
+			sure = (s2 >= q0.y) && (s2 < q1.z);   // T_k + E <= s2 < T_k+1 - E
 			if (KIND == K_S_MU) {
 				const float p = 0.5f * fmaf(dz, ib.z + b.z, fmaf(dy, ib.y + b.y, dx * (ib.x + b.x)));
 				const float rs = __builtin_amdgcn_rsqf(s2);
@@ -547,42 +607,28 @@ __device__ void process_item_fast(const JobDev &J, const WorkItem &W, const int4
 			} else {
 				bin = k;
 			}
-			bool ok = true;
-			if (!sure) {
-				++n_rc;
-				const int g1 = W.i_begin + l2, g2 = __float_as_int(a.w);
-				const double xi[3] = {A.x[g1], A.y[g1], A.z[g1]};
-				const double ni[3] = {A.nx[g1], A.ny[g1], A.nz[g1]};
-				const double xj[3] = {B.x[g2], B.y[g2], B.z[g2]};
-				const double nj[3] = {B.nx[g2], B.ny[g2], B.nz[g2]};
-				ok = exact_bin<KIND>(xi, ni, xj, nj, min_r_sq, max_r_sq, t_s2, nullptr, nullptr, ax1, ax2, bin);
-			}
-			if (ok) {
+			if (sure) {
 				++n_acc;
-				float w = ia.w * b.w;
-				if (WEIGHTED && J.wmode != TWOPCF_W_PLAIN) {
-					const int g1 = W.i_begin + l2, g2 = __float_as_int(a.w);
-					const double ni[3] = {A.nx[g1], A.ny[g1], A.nz[g1]};
-					const double nj[3] = {B.nx[g2], B.ny[g2], B.nz[g2]};
-					w = pair_weight(J, g1, g2, ni, nj);
-				}
-				if (WEIGHTED) atomicAdd((double *)&F[bin], (double)w);
-				else atomicAdd((unsigned int *)&F[bin], 1u);
-				if (jk) {
-					const int si = __float_as_int(ib.w), sj = tslot[q];
-					add_region(si, bin, w);
-					if (sj != si) add_region(sj, bin, w);
-				}
+				add(bin, weight(l2, __float_as_int(a.w), ia.w * b.w), __float_as_int(ib.w), tslot[q]);
 			}
 		}
+		const unsigned long long mr = __builtin_amdgcn_ballot_w64(lane < (int)n && !sure);
+		if (mr) {
+			if (__builtin_amdgcn_inverse_ballot_w64(mr)) {
+				const unsigned off = __builtin_amdgcn_mbcnt_hi((unsigned)(mr >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)mr, 0u));
+				rq[rq_n + off] = make_int2(l2, __float_as_int(tfa[q].w));
+			}
+			rq_n += (unsigned)__popcll(mr);
+		}
 		__builtin_amdgcn_wave_barrier();
+		if (rq_n >= WAVE) flush_rq();
 	};
 
-	// append the lanes with in == true (pair: this lane's object, tile entry q)
+	// append the lanes of mask m (pair: the lane's object, tile entry q)
 	unsigned head = 0, tail = 0;
-	auto append = [&](bool in, int q) {
-		const unsigned long long m = __ballot(in);
-		if (in) {
+	auto append = [&](unsigned long long m, int q) {
+		if (m == 0ull) return;
+		if (__builtin_amdgcn_inverse_ballot_w64(m)) {
 			const unsigned off = __builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u));
 			ring[(tail + off) & (RING - 1)] = ((unsigned)q << 8) | packed_li;
 		}
@@ -599,29 +645,35 @@ __device__ void process_item_fast(const JobDev &J, const WorkItem &W, const int4
 		if (is_auto && row.x == 0 && row.y == 0) z0 = max(z0, cz);
 		if (z0 > z1) continue;
 		const int base = (qx * d1 + qy) * d2;
-		const int jb = B.start[base + z0], je = B.start[base + z1 + 1];
+		const int jb = __builtin_amdgcn_readfirstlane(J.B.start[base + z0]);
+		const int je = __builtin_amdgcn_readfirstlane(J.B.start[base + z1 + 1]);
+		// only the row through the first-side cell of an auto count holds
+		// pairs of the excluded half (cell-major order)
+		const int own_end = is_auto && row.x == 0 && row.y == 0
+		                        ? __builtin_amdgcn_readfirstlane(J.B.start[W.i_cell + 1]) : jb;
 		for (int t0 = jb; t0 < je; t0 += TILE) {
 			const int nt = min(TILE, je - t0);
 			if (tid < nt) {
 				const int gj = t0 + tid;
 				int sl = 0;
 				if (jk) {
-					const int rj = B.jk[gj];
+					const int rj = J.B.jk[gj];
 					sl = -(rj + 1);
 					for (int s = 0; s < n_slots; ++s)
 						if (sreg[s] == rj) sl = s;
 				}
-				tfa[tid] = make_float4((float)(B.x[gj] - c0), (float)(B.y[gj] - c1), (float)(B.z[gj] - c2),
+				tfa[tid] = make_float4((float)(J.B.x[gj] - c0), (float)(J.B.y[gj] - c1), (float)(J.B.z[gj] - c2),
 				                       __int_as_float(gj));
-				tfb[tid] = make_float4((float)B.nx[gj], (float)B.ny[gj], (float)B.nz[gj], B.w[gj]);
+				tfb[tid] = make_float4((float)J.B.nx[gj], (float)J.B.ny[gj], (float)J.B.nz[gj], J.B.w[gj]);
 				tslot[tid] = sl;
 			}
 			__syncthreads();
-			if (wave_active) {
+			if (valid_mask) {
+				const bool order = t0 < own_end;
 				// screening, two tile entries per step; the ring holds < 64
 				// pairs between steps and at most 191 within one
-				for (int q = jg; q < nt; q += 2 * JG) {
-					const int q1 = q + JG;
+				for (int q = jg; q < nt; q += 2 * jg_n) {
+					const int q1 = q + jg_n;
 					const bool has1 = q1 < nt;
 					const float4 a0 = tfa[q];
 					const float4 a1 = tfa[has1 ? q1 : q];
@@ -629,11 +681,18 @@ __device__ void process_item_fast(const JobDev &J, const WorkItem &W, const int4
 					const float dx1 = me.x - a1.x, dy1 = me.y - a1.y, dz1 = me.z - a1.z;
 					const float s20 = fmaf(dz0, dz0, fmaf(dy0, dy0, dx0 * dx0));
 					const float s21 = fmaf(dz1, dz1, fmaf(dy1, dy1, dx1 * dx1));
-					const bool in0 = ivalid && (s20 >= s2_lo) && (s20 < s2_hi) && (!is_auto || __float_as_int(a0.w) > gi);
-					const bool in1 = has1 && ivalid && (s21 >= s2_lo) && (s21 < s2_hi) &&
-					                 (!is_auto || __float_as_int(a1.w) > gi);
-					append(in0, q);
-					append(in1, q1);
+					unsigned long long m0 = __builtin_amdgcn_ballot_w64(s20 < s2_hi) & valid_mask;
+					unsigned long long m1 = __builtin_amdgcn_ballot_w64(s21 < s2_hi) & (has1 ? valid_mask : 0ull);
+					if (check_lo) {
+						m0 &= __builtin_amdgcn_ballot_w64(s20 >= s2_lo);
+						m1 &= __builtin_amdgcn_ballot_w64(s21 >= s2_lo);
+					}
+					if (order) {
+						m0 &= __builtin_amdgcn_ballot_w64(__float_as_int(a0.w) > gi);
+						m1 &= __builtin_amdgcn_ballot_w64(__float_as_int(a1.w) > gi);
+					}
+					append(m0, q);
+					append(m1, q1);
 					while (tail - head >= WAVE) {
 						consume(head, WAVE);
 						head += WAVE;
@@ -647,12 +706,13 @@ __device__ void process_item_fast(const JobDev &J, const WorkItem &W, const int4
 			__syncthreads();
 		}
 	}
+	if (rq_n) flush_rq();
 
 	for (int b = tid; b < nb; b += BLOCK) {
 		HT v = F[b];
 		if (v != (HT)0) {
-			if (WEIGHTED) atomicAdd(&full_d[b], (double)v);
-			else atomicAdd(&full_u[b], (unsigned long long)v);
+			if (WEIGHTED) atomicAdd(&J.full_d[b], (double)v);
+			else atomicAdd(&J.full_u[b], (unsigned long long)v);
 			F[b] = (HT)0;
 		}
 	}
@@ -662,8 +722,8 @@ __device__ void process_item_fast(const JobDev &J, const WorkItem &W, const int4
 			if (v != (HT)0) {
 				const int s_ = e / nb, b = e - s_ * nb;
 				const size_t o = (size_t)sreg[s_] * nb + b;
-				if (WEIGHTED) atomicAdd(&jk_d[o], (double)v);
-				else atomicAdd(&jk_u[o], (unsigned long long)v);
+				if (WEIGHTED) atomicAdd(&J.jk_d[o], (double)v);
+				else atomicAdd(&J.jk_u[o], (unsigned long long)v);
 				H[e] = (HT)0;
 			}
 		}
@@ -701,7 +761,7 @@ __global__ void __launch_bounds__(BLOCK) k_count(const JobDev *__restrict__ jobs
 		if (queue) {
 			if (threadIdx.x == 0) s_item = (int)atomicAdd(queue, 1u);
 			__syncthreads();
-			it = s_item;
+			it = __builtin_amdgcn_readfirstlane(s_item);
 			__syncthreads();
 		}
 		if (it >= n_items) break;
@@ -1126,7 +1186,7 @@ int build_plan(twopcf_ctx *ctx, const twopcf_job *jobs, int n_jobs, int method, 
 	int subdiv = fast ? 2 : 1;
 	int max_slots = 8;
 	if (tuning) {
-		if (fast && tuning->cell_subdiv > 0) subdiv = std::min(tuning->cell_subdiv, 4);
+		if (fast && tuning->cell_subdiv > 0) subdiv = std::min(tuning->cell_subdiv, MAX_SUBDIV);
 		if (tuning->max_slots > 0) max_slots = std::min(tuning->max_slots, MAX_SLOTS);
 	}
 
