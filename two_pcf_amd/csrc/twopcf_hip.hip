@@ -707,6 +707,7 @@ __device__ void process_item_fast(const JobDev &J, const WorkItem &W, const int4
 		}
 	}
 	if (rq_n) flush_rq();
+	__syncthreads();   // every wave's rechecked pairs are in the LDS histograms
 
 	for (int b = tid; b < nb; b += BLOCK) {
 		HT v = F[b];
