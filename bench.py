@@ -29,6 +29,11 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 FLOP_PER_PAIR = 26            # FP32 flop per accepted s-mu pair
+# HBM bytes per launch of the counting kernel on the default workload (scale
+# 1, 100 regions, default tuning), from two rocprofv3 --pmc runs:
+# FETCH_SIZE x 2 + WRITE_SIZE x 1 (factors from a 1 GiB copy), KiB -> B.
+# Source: profiles/k_count_fp32_jackknife_batch_hbm.md
+TRAFFIC_BYTES = {(1.0, 100): (8.261e7 * 2 + 3.755e6) * 1024}
 PEAK_FP32_TFLOPS = 157.3      # MI355X FP32 vector peak
 AXES = [(0.0, 100.0, 40, 1.3), (0.0, 1.0, 50, 0.0)]
 
@@ -122,7 +127,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--scale", type=float, default=1.0)
     ap.add_argument("--plain-steps", type=int, default=1)
-    ap.add_argument("--cpu-frac", type=float, default=0.12)
+    ap.add_argument("--cpu-frac", type=float, default=0.2)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--subdiv", type=int, default=0)
     ap.add_argument("--slots", type=int, default=0)
@@ -216,7 +221,8 @@ def main():
         "plain_fp64": plain,
         "speedup_vs_plain_fp64": (plain["kernel_ms"] / float(np.mean(kms))) if plain else None,
         "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
-                     "frac": achieved / PEAK_FP32_TFLOPS, "traffic": None,
+                     "frac": achieved / PEAK_FP32_TFLOPS,
+                     "traffic": TRAFFIC_BYTES.get((args.scale, n_jk)) if not tuning else None,
                      "note": "FP32 vector ALU; 26 algorithmic flop per accepted pair"},
     }
     if not args.no_cpu:

@@ -545,6 +545,7 @@ __device__ void process_item_fast(const JobDev &J, const WorkItem &W, const int4
 		__builtin_amdgcn_wave_barrier();
 		for (unsigned b = 0; b < rq_n; b += WAVE) {
 			if (b + lane < rq_n) {
+				++n_rc;
 				const int2 e = rq[b + lane];
 				const int g1 = W.i_begin + e.x, g2 = e.y;
 				const double xi[3] = {J.A.x[g1], J.A.y[g1], J.A.z[g1]};
@@ -567,7 +568,6 @@ __device__ void process_item_fast(const JobDev &J, const WorkItem &W, const int4
 				}
 			}
 		}
-		n_rc += rq_n;
 		rq_n = 0;
 		__builtin_amdgcn_wave_barrier();
 	};
