@@ -29,11 +29,16 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 FLOP_PER_PAIR = 26            # FP32 flop per accepted s-mu pair
+# per accepted pair and correlator kind: monopole 15 (d 3, s^2 5, range 2,
+# sqrt 1, log index 3, weight 1), s-mu 26 (adds n1+n2 3, pi dot 5, x1/2 1,
+# mu divide 1, mu index 1, less nothing)
+FLOP_BY_KIND = {0: 15, 2: 26}
 # HBM bytes per launch of the counting kernel on the default workload (scale
 # 1, 100 regions, default tuning), from two rocprofv3 --pmc runs:
 # FETCH_SIZE x 2 + WRITE_SIZE x 1 (factors from a 1 GiB copy), KiB -> B.
 # Source: profiles/k_count_fp32_jackknife_batch_hbm.md
 TRAFFIC_BYTES = {(1.0, 100): (8.261e7 * 2 + 3.755e6) * 1024}
+DEFAULT_TUNING = dict(cell_subdiv=2, max_slots=8, i_chunk=256)
 PEAK_FP32_TFLOPS = 157.3      # MI355X FP32 vector peak
 AXES = [(0.0, 100.0, 40, 1.3), (0.0, 1.0, 50, 0.0)]
 
@@ -120,6 +125,63 @@ def copy_calibration(mib):
     print(json.dumps({"copy_calibration_mib": mib}), flush=True)
 
 
+def workload_jackknife(args, L, ctx):
+    cats, n_jk = make_catalogs(args.scale)
+    if args.n_jk:
+        n_jk = args.n_jk
+    t_up = time.perf_counter()
+    dc = {t: ctx.upload(a["x"], a["y"], a["z"], a["nx"], a["ny"], a["nz"], a["w"], a["jk"])
+          for t, a in cats.items()}
+    upload_ms = 1000.0 * (time.perf_counter() - t_up)
+    jobs = [dict(kind=L.S_MU, a=dc["data"], b=None, axes=AXES, n_jk=n_jk),
+            dict(kind=L.S_MU, a=dc["data"], b=dc["randoms"], axes=AXES, n_jk=n_jk),
+            dict(kind=L.S_MU, a=dc["randoms"], b=None, axes=AXES, n_jk=n_jk)]
+    config = {"workload": "jackknife batch: 2M clustered data + 10M randoms, 100 jk regions, "
+                          "s-mu 40 log bins (base 1.3, s<100) x 50 mu, DD+DR+RR in one launch",
+              "n_data": int(cats["data"]["x"].size), "n_randoms": int(cats["randoms"]["x"].size),
+              "scale": args.scale, "n_jk": n_jk}
+    return dict(jobs=jobs, upload_ms=upload_ms, config=config, traffic_key=(args.scale, n_jk),
+                cpu=lambda: cpu_baseline(cats, n_jk, args.cpu_frac))
+
+
+MONO_AXES = [(0.0, 100.0, 30, 1.3)]
+
+
+def workload_multi(args, L, ctx):
+    """64 independent cartesian mocks, sizes log-uniform in 20k-2M (scaled by
+    --scale), Neyman-Scott at density 3e-4 (Mpc/h)^-3, 5x uniform randoms
+    each; monopole (30 log bins base 1.3 to 100) and s-mu (40 x 50) DD, DR,
+    RR of every mock in one call: one persistent launch per correlator kind,
+    captured in one hipGraph."""
+    from two_pcf_amd import synth
+    t0 = time.time()
+    mocks = synth.uneven_mocks(n_mocks=64, n_min=max(50, int(20_000 * args.scale)),
+                               n_max=max(100, int(2_000_000 * args.scale)))
+    nd = sum(m["data"].shape[0] for m in mocks)
+    nr = sum(m["randoms"].shape[0] for m in mocks)
+    log(f"mocks: 64, {nd} data + {nr} randoms ({time.time() - t0:.1f} s)")
+    up_ms = 0.0
+    jobs = []
+    for m in mocks:
+        dc = {}
+        for tag in ("data", "randoms"):
+            p = m[tag]
+            x, y, z = (np.ascontiguousarray(p[:, k]) for k in range(3))
+            _, nx, ny, nz = L.cart_norm(x, y, z)
+            t_up = time.perf_counter()
+            dc[tag] = ctx.upload(x, y, z, nx, ny, nz)
+            up_ms += 1000.0 * (time.perf_counter() - t_up)
+        for kind, axes in ((L.MONOPOLE, MONO_AXES), (L.S_MU, AXES)):
+            jobs += [dict(kind=kind, a=dc["data"], b=None, axes=axes, n_jk=1),
+                     dict(kind=kind, a=dc["data"], b=dc["randoms"], axes=axes, n_jk=1),
+                     dict(kind=kind, a=dc["randoms"], b=None, axes=axes, n_jk=1)]
+    config = {"workload": "uneven multi-catalogue batch: 64 cartesian Neyman-Scott mocks, sizes log-uniform "
+                          "in 20k-2M, density 3e-4, 5x randoms; monopole (30 log bins) + s-mu (40x50) "
+                          "DD+DR+RR of every mock in one call",
+              "n_data": nd, "n_randoms": nr, "scale": args.scale, "n_jobs": len(jobs)}
+    return dict(jobs=jobs, upload_ms=up_ms, config=config, traffic_key=None, cpu=None)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -132,6 +194,10 @@ def main():
     ap.add_argument("--subdiv", type=int, default=0)
     ap.add_argument("--slots", type=int, default=0)
     ap.add_argument("--n-jk", type=int, default=0, help="override the number of jackknife regions (1: none)")
+    ap.add_argument("--no-autotune", action="store_true", help="use the default tuning instead of the autotuner's")
+    ap.add_argument("--workload", choices=["jackknife", "multi"], default="jackknife",
+                    help="jackknife: the 2M+10M s-mu jackknife batch (default); multi: 64 uneven "
+                         "cartesian mocks, monopole + s-mu of each in one call")
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="only run a device copy of this size (counter calibration)")
     args = ap.parse_args()
@@ -144,24 +210,21 @@ def main():
         sys.exit(2)
 
     from two_pcf_amd import lib as L
-    cats, n_jk = make_catalogs(args.scale)
-    if args.n_jk:
-        n_jk = args.n_jk
     ctx = L.Context(0)
-    info = ctx.info()
-    log("device", info)
-    t_up = time.perf_counter()
-    dc = {t: ctx.upload(a["x"], a["y"], a["z"], a["nx"], a["ny"], a["nz"], a["w"], a["jk"])
-          for t, a in cats.items()}
-    upload_ms = 1000.0 * (time.perf_counter() - t_up)
-    jobs = [dict(kind=L.S_MU, a=dc["data"], b=None, axes=AXES, n_jk=n_jk),
-            dict(kind=L.S_MU, a=dc["data"], b=dc["randoms"], axes=AXES, n_jk=n_jk),
-            dict(kind=L.S_MU, a=dc["randoms"], b=None, axes=AXES, n_jk=n_jk)]
+    log("device", ctx.info())
+    W = workload_multi(args, L, ctx) if args.workload == "multi" else workload_jackknife(args, L, ctx)
+    jobs, upload_ms = W["jobs"], W["upload_ms"]
     tuning = {}
     if args.subdiv:
         tuning["cell_subdiv"] = args.subdiv
     if args.slots:
         tuning["max_slots"] = args.slots
+    tuned = None
+    if not tuning and not args.no_autotune:
+        t0 = time.perf_counter()
+        tu, est = ctx.autotune(jobs)
+        tuned = dict(choice=tu, estimated_kernel_ms=est, autotune_s=time.perf_counter() - t0)
+        log(f"autotune: {tu} (estimated kernel {est:.1f} ms, {tuned['autotune_s']:.1f} s)")
 
     ref = None
     first_call_ms = None
@@ -199,18 +262,18 @@ def main():
         plain = dict(kernel_ms=float(np.mean(pk)), pairs_per_s=pairs / (float(np.mean(pk)) / 1000.0),
                      counts_identical=bool(same))
 
-    achieved = pairs * FLOP_PER_PAIR / kernel_s / 1e12
+    flop = sum(r["n_pairs"] * FLOP_BY_KIND[j["kind"]] for r, j in zip(res, jobs))
+    achieved = flop / kernel_s / 1e12
+    default_tuning = not tuning and (tuned is None or tuned["choice"] == DEFAULT_TUNING)
+    config = dict(W["config"])
+    config.update({"parallelism": "single GPU", "tuning": tuning or (tuned if tuned else "default")})
     line = {
         "metric": "pairs/s s-mu DD+DR+RR: FP32+edge-recheck batched vs FP64 kernel, exact counts",
         "value": value, "unit": "pairs/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": ms_step, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "f32+f64-recheck",
         "data": "synthetic (seeded Neyman-Scott + uniform randoms)",
-        "config": {"workload": "jackknife batch: 2M clustered data + 10M randoms, 100 jk regions, "
-                               "s-mu 40 log bins (base 1.3, s<100) x 50 mu, DD+DR+RR in one launch",
-                   "n_data": int(cats["data"]["x"].size), "n_randoms": int(cats["randoms"]["x"].size),
-                   "scale": args.scale, "n_jk": n_jk, "parallelism": "single GPU",
-                   "tuning": tuning or "default"},
+        "config": config,
         "pairs_per_step": pairs,
         "kernel_ms": float(np.mean(kms)),
         "kernel_pairs_per_s": pairs / kernel_s,
@@ -222,12 +285,12 @@ def main():
         "speedup_vs_plain_fp64": (plain["kernel_ms"] / float(np.mean(kms))) if plain else None,
         "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / PEAK_FP32_TFLOPS,
-                     "traffic": TRAFFIC_BYTES.get((args.scale, n_jk)) if not tuning else None,
-                     "note": "FP32 vector ALU; 26 algorithmic flop per accepted pair"},
+                     "traffic": TRAFFIC_BYTES.get(W["traffic_key"]) if default_tuning else None,
+                     "note": "FP32 vector ALU; algorithmic flop per accepted pair: s-mu 26, monopole 15"},
     }
-    if not args.no_cpu:
+    if not args.no_cpu and W["cpu"] is not None:
         try:
-            line["cpu_baseline"] = cpu_baseline(cats, n_jk, args.cpu_frac)
+            line["cpu_baseline"] = W["cpu"]()
         except Exception as e:  # reported, not fatal
             line["cpu_baseline"] = {"error": str(e)}
     else:

@@ -129,11 +129,13 @@ typedef struct {
 	uint64_t n_rechecked;    /* exact re-evaluations           */
 } twopcf_timing;
 
-/* Tuning knobs of TWOPCF_FP32_RECHECK (0 = default / autotuned). */
+/* Tuning knobs of TWOPCF_FP32_RECHECK.  A zero field takes the default
+ * (2, 8, 256).  A NULL tuning takes the choice twopcf_autotune made for the
+ * same job set, or the defaults.  No knob changes a result. */
 typedef struct {
 	int32_t cell_subdiv;     /* cells per rmax along each axis (1..3; larger values mean 3) */
 	int32_t max_slots;       /* jackknife region slots per work item (1..16) */
-	int32_t i_chunk;         /* objects per work item on the first side     */
+	int32_t i_chunk;         /* first-side objects per work item (64..256, multiple of 64) */
 } twopcf_tuning;
 
 int twopcf_abi_version(void);
@@ -150,6 +152,17 @@ int64_t twopcf_catalog_size(const twopcf_cat *cat);
  * tuning may be NULL.  timing may be NULL. */
 int twopcf_count(twopcf_ctx *ctx, const twopcf_job *jobs, int n_jobs, int method,
                  const twopcf_tuning *tuning, twopcf_timing *timing);
+
+/* Autotuner of the batched method (no counterpart in the reference, whose
+ * cell size is fixed at rmax): times trial runs of the job set on a sample of
+ * its work queue, choosing cell subdivision, region slots and first-side
+ * chunk size, and records the choice in the context for this job set (the
+ * same catalogues, axes, regions and weight rule), where twopcf_count uses it
+ * when called with tuning == NULL.  Writes the choice to *best and the
+ * estimated full-batch kernel time to *best_ms (either may be NULL).  The
+ * jobs' output buffers are not written. */
+int twopcf_autotune(twopcf_ctx *ctx, const twopcf_job *jobs, int n_jobs,
+                    twopcf_tuning *best, double *best_ms);
 
 /* Host-only helpers (no GPU needed). ------------------------------------ */
 

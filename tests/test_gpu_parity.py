@@ -198,3 +198,68 @@ def test_grid_independence(ctx, subdiv):
             np.testing.assert_array_equal(sub, gs, err_msg=f"subdiv {subdiv} {corr} {h} sub")
     for c in cats.values():
         c.free()
+
+
+def test_autotune_keeps_counts(ctx, oracle_lib):
+    """The autotuner's choice is used by later calls without a tuning and
+    gives the same counts as the defaults and the plain method."""
+    from two_pcf_amd import lib as L
+    cats, n_jk = _jk_batch(0.005, 13)
+    axes = [(0.0, 100.0, 40, 1.3), (0.0, 1.0, 50, 0.0)]
+    dc = {t: ctx.upload(a["x"], a["y"], a["z"], a["nx"], a["ny"], a["nz"], a["w"], a["jk"]) for t, a in cats.items()}
+    jobs = [dict(kind=2, a=dc["data"], b=None, axes=axes, n_jk=n_jk),
+            dict(kind=2, a=dc["data"], b=dc["randoms"], axes=axes, n_jk=n_jk),
+            dict(kind=2, a=dc["randoms"], b=None, axes=axes, n_jk=n_jk)]
+    print("[gpu] autotune", flush=True)
+    tu, est = ctx.autotune(jobs)
+    assert 1 <= tu["cell_subdiv"] <= 3 and 1 <= tu["max_slots"] <= 16 and tu["i_chunk"] in (128, 256)
+    assert est > 0
+    tuned, _ = ctx.count(jobs, method=1)
+    dflt, _ = ctx.count(jobs, method=1, tuning={})
+    plain, _ = ctx.count(jobs, method=0)
+    for a, b, c in zip(tuned, dflt, plain):
+        np.testing.assert_array_equal(a["full"], c["full"])
+        np.testing.assert_array_equal(a["jk"], c["jk"])
+        np.testing.assert_array_equal(b["full"], c["full"])
+        np.testing.assert_array_equal(b["jk"], c["jk"])
+    for c in dc.values():
+        c.free()
+
+
+@pytest.mark.parametrize("method", [0, 1])
+def test_uneven_multi_catalogue_batch(ctx, oracle_lib, method):
+    """Many independent cartesian mocks of uneven size, monopole and s-mu
+    DD/DR/RR of every mock in one call (one persistent launch per kind):
+    every job equals the oracle's count of that mock alone."""
+    from two_pcf_amd import lib as L
+    from two_pcf_amd import synth
+    o = oracle_lib
+    mocks = synth.uneven_mocks(n_mocks=6, n_min=300, n_max=3000, density=3e-4, seed=5)
+    mono = [(0.0, 100.0, 30, 1.3)]
+    smu = [(0.0, 100.0, 40, 1.3), (0.0, 1.0, 50, 0.0)]
+    jobs, want = [], []
+    dcs = []
+    for m in mocks:
+        dcat = {}
+        for tag in ("data", "randoms"):
+            p = m[tag]
+            _, nx, ny, nz = L.cart_norm(p[:, 0], p[:, 1], p[:, 2])
+            dcat[tag] = (p[:, 0].copy(), p[:, 1].copy(), p[:, 2].copy(), nx, ny, nz)
+        up = {t: ctx.upload(*v) for t, v in dcat.items()}
+        dcs.extend(up.values())
+        oc = {t: o.Catalog(*v, np.ones(v[0].size, np.float32), np.zeros(v[0].size, np.int32)) for t, v in dcat.items()}
+        for kind, axes in ((0, mono), (2, smu)):
+            ax1 = o.axis(*axes[0])
+            ax2 = o.axis(*axes[1]) if len(axes) > 1 else None
+            for a, b in (("data", None), ("data", "randoms"), ("randoms", None)):
+                jobs.append(dict(kind=kind, a=up[a], b=up[b] if b else None, axes=axes, n_jk=1))
+                want.append(o.count(kind, oc[a], oc[b] if b else None, ax1, ax2, 1))
+    print(f"[gpu] multi-catalogue batch method={method}: {len(jobs)} jobs", flush=True)
+    res, t = ctx.count(jobs, method=method)
+    if method == 1:
+        assert t["n_launches"] == 2   # one persistent launch per kind
+    for r, (full, _, acc) in zip(res, want):
+        np.testing.assert_array_equal(r["full"], full)
+        assert r["n_pairs"] == acc
+    for c in dcs:
+        c.free()

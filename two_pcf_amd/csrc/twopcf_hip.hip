@@ -21,6 +21,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <initializer_list>
 #include <map>
 #include <memory>
 #include <string>
@@ -839,6 +840,7 @@ struct twopcf_ctx {
 	std::string name;
 	std::vector<twopcf_cat *> cats;
 	std::map<std::string, std::unique_ptr<Plan>> plans;
+	std::map<std::string, twopcf_tuning> tuned;   // autotuned choice per job set
 	uint64_t next_cat_id = 1;
 };
 
@@ -1181,14 +1183,18 @@ namespace {
 
 // Build the device-side plan of one twopcf_count call: grids, tables, work
 // items, output pools and the launch list.  Cached per context.
+// sample_stride > 1 keeps every sample_stride-th item of the cost-sorted
+// queue (autotuning trials; the outputs are then partial).
 int build_plan(twopcf_ctx *ctx, const twopcf_job *jobs, int n_jobs, int method, const twopcf_tuning *tuning,
-               Plan &plan) {
+               Plan &plan, int sample_stride = 1) {
 	const bool fast = method == TWOPCF_FP32_RECHECK;
 	int subdiv = fast ? 2 : 1;
 	int max_slots = 8;
+	int i_chunk = ICHUNK;
 	if (tuning) {
 		if (fast && tuning->cell_subdiv > 0) subdiv = std::min(tuning->cell_subdiv, MAX_SUBDIV);
 		if (tuning->max_slots > 0) max_slots = std::min(tuning->max_slots, MAX_SLOTS);
+		if (fast && tuning->i_chunk > 0) i_chunk = std::max(WAVE, std::min(ICHUNK, tuning->i_chunk / WAVE * WAVE));
 	}
 
 	// ---- validate and build per-job host descriptions ----
@@ -1495,8 +1501,8 @@ int build_plan(twopcf_ctx *ctx, const twopcf_job *jobs, int n_jobs, int method, 
 				ri.push_back({r, jb, je});
 			}
 			if (ri.empty()) continue;
-			for (int i0 = ib; i0 < ie; i0 += ICHUNK) {
-				const int ni = std::min(ICHUNK, ie - i0);
+			for (int i0 = ib; i0 < ie; i0 += i_chunk) {
+				const int ni = std::min(i_chunk, ie - i0);
 				// group rows by region budget
 				std::vector<int> regs_i;
 				if (jk) {
@@ -1603,6 +1609,11 @@ int build_plan(twopcf_ctx *ctx, const twopcf_job *jobs, int n_jobs, int method, 
 		std::stable_sort(ord.begin(), ord.end(), [&](size_t x, size_t y) { return item_cost[x] > item_cost[y]; });
 		std::vector<WorkItem> s(items.size());
 		for (size_t k = 0; k < ord.size(); ++k) s[k] = items[ord[k]];
+		if (sample_stride > 1) {
+			std::vector<WorkItem> t;
+			for (size_t k = 0; k < s.size(); k += (size_t)sample_stride) t.push_back(s[k]);
+			s.swap(t);
+		}
 		items.swap(s);
 	}
 	auto h2d = [&](void *dst, const void *src, size_t bytes) -> int {
@@ -1732,6 +1743,32 @@ int build_plan(twopcf_ctx *ctx, const twopcf_job *jobs, int n_jobs, int method, 
 	return 0;
 }
 
+// Every input that shapes a plan except the tuning: method, catalogues,
+// axes, regions, weight rule and up-weight table of every job.
+std::string job_set_key(const twopcf_job *jobs, int n_jobs, int method, std::vector<uint64_t> *ids) {
+	std::string key;
+	auto put = [&](const void *p, size_t n) { key.append((const char *)p, n); };
+	put(&method, sizeof method);
+	for (int q = 0; q < n_jobs; ++q) {
+		const twopcf_job &J = jobs[q];
+		const uint64_t ia = J.a->id, ib = J.b ? J.b->id : 0;
+		if (ids) {
+			ids->push_back(ia);
+			if (J.b) ids->push_back(ib);
+		}
+		put(&J.kind, sizeof J.kind);
+		put(&ia, 8);
+		put(&ib, 8);
+		put(J.ax, sizeof J.ax);
+		put(&J.n_jk, sizeof J.n_jk);
+		put(&J.weight_mode, sizeof J.weight_mode);
+		put(&J.n_runs, sizeof J.n_runs);
+		put(&J.up_axis, sizeof J.up_axis);
+		if (J.up_bins && J.up_axis.n_bins > 0) put(J.up_bins, sizeof(double) * J.up_axis.n_bins);
+	}
+	return key;
+}
+
 // Enqueue one execution of a plan on the context stream: clear the outputs
 // with fill kernels (a graph may hold them; no memset nodes) and launch.
 int enqueue_plan(twopcf_ctx *ctx, Plan &plan) {
@@ -1763,29 +1800,20 @@ int twopcf_count(twopcf_ctx *ctx, const twopcf_job *jobs, int n_jobs, int method
 	for (int q = 0; q < n_jobs; ++q)
 		if (!jobs[q].a || !jobs[q].full) return fail(ctx, TWOPCF_E_ARG, "job needs a catalogue and a full[] output");
 	HIPCK(ctx, hipSetDevice(ctx->device));
-	// plan cache key: method, tuning and every field that shapes the plan
-	std::string key;
-	auto put = [&](const void *p, size_t n) { key.append((const char *)p, n); };
-	put(&method, sizeof method);
-	twopcf_tuning tu{};
-	if (tuning) tu = *tuning;
-	put(&tu, sizeof tu);
+	// no tuning given: the autotuned choice for this job set, if any
 	std::vector<uint64_t> ids;
-	for (int q = 0; q < n_jobs; ++q) {
-		const twopcf_job &J = jobs[q];
-		const uint64_t ia = J.a->id, ib = J.b ? J.b->id : 0;
-		ids.push_back(ia);
-		if (J.b) ids.push_back(ib);
-		put(&J.kind, sizeof J.kind);
-		put(&ia, 8);
-		put(&ib, 8);
-		put(J.ax, sizeof J.ax);
-		put(&J.n_jk, sizeof J.n_jk);
-		put(&J.weight_mode, sizeof J.weight_mode);
-		put(&J.n_runs, sizeof J.n_runs);
-		put(&J.up_axis, sizeof J.up_axis);
-		if (J.up_bins && J.up_axis.n_bins > 0) put(J.up_bins, sizeof(double) * J.up_axis.n_bins);
+	const std::string jkey = job_set_key(jobs, n_jobs, method, &ids);
+	twopcf_tuning tu{};
+	if (tuning) {
+		tu = *tuning;
+	} else if (method == TWOPCF_FP32_RECHECK) {
+		auto t = ctx->tuned.find(jkey);
+		if (t != ctx->tuned.end()) tu = t->second;
 	}
+	tuning = &tu;
+	// plan cache key: the job set and the tuning
+	std::string key = jkey;
+	key.append((const char *)&tu, sizeof tu);
 	Plan *plan = nullptr;
 	bool built = false;
 	auto it = ctx->plans.find(key);
@@ -1857,6 +1885,68 @@ int twopcf_count(twopcf_ctx *ctx, const twopcf_job *jobs, int n_jobs, int method
 		timing->n_pairs = tot_acc;
 		timing->n_rechecked = tot_rc;
 	}
+	return 0;
+}
+
+// Autotuning of the batched method: trials on every 8th item of the
+// cost-sorted queue, timed with events on the context stream, over cell
+// subdivision, then region slots (jobs with regions), then first-side chunk
+// size, one knob at a time from the defaults.  The choice is kept in the
+// context for this job set and used by twopcf_count when it gets no tuning.
+int twopcf_autotune(twopcf_ctx *ctx, const twopcf_job *jobs, int n_jobs, twopcf_tuning *best, double *best_ms) {
+	if (!ctx || !jobs || n_jobs <= 0) return fail(ctx, TWOPCF_E_ARG, "null argument");
+	for (int q = 0; q < n_jobs; ++q)
+		if (!jobs[q].a || !jobs[q].full) return fail(ctx, TWOPCF_E_ARG, "job needs a catalogue and a full[] output");
+	HIPCK(ctx, hipSetDevice(ctx->device));
+	const int method = TWOPCF_FP32_RECHECK;
+	const int stride = 8;
+	bool any_jk = false;
+	for (int q = 0; q < n_jobs; ++q) any_jk = any_jk || jobs[q].n_jk > 1;
+	auto trial = [&](const twopcf_tuning &t, double *ms) -> int {
+		Plan p;
+		int rc = build_plan(ctx, jobs, n_jobs, method, &t, p, stride);
+		if (rc) return rc;
+		double m = INFINITY;
+		for (int rep = 0; rep < 2; ++rep) {
+			HIPCK(ctx, hipEventRecord(ctx->ev0, ctx->stream));
+			if ((rc = enqueue_plan(ctx, p))) return rc;
+			HIPCK(ctx, hipEventRecord(ctx->ev1, ctx->stream));
+			HIPCK(ctx, hipStreamSynchronize(ctx->stream));
+			float k = 0.0f;
+			HIPCK(ctx, hipEventElapsedTime(&k, ctx->ev0, ctx->ev1));
+			m = std::min(m, (double)k);
+		}
+		*ms = m;
+		return 0;
+	};
+	twopcf_tuning cur{2, 8, ICHUNK};
+	double cur_ms = INFINITY;
+	int rc = trial(cur, &cur_ms);
+	if (rc) return rc;
+	auto try_knob = [&](int32_t twopcf_tuning::*f, std::initializer_list<int> vals) -> int {
+		const twopcf_tuning base = cur;
+		for (int v : vals) {
+			if (v == base.*f) continue;
+			twopcf_tuning t = base;
+			t.*f = v;
+			double ms;
+			int r = trial(t, &ms);
+			if (r == TWOPCF_E_LIMIT) continue;   // not feasible for these jobs (stencil, LDS)
+			if (r) return r;
+			if (ms < cur_ms) {
+				cur = t;
+				cur_ms = ms;
+			}
+		}
+		return 0;
+	};
+	ctx->err.clear();
+	if ((rc = try_knob(&twopcf_tuning::cell_subdiv, {1, 3}))) return rc;
+	if (any_jk && (rc = try_knob(&twopcf_tuning::max_slots, {4, 12, 16}))) return rc;
+	if ((rc = try_knob(&twopcf_tuning::i_chunk, {128}))) return rc;
+	ctx->tuned[job_set_key(jobs, n_jobs, method, nullptr)] = cur;
+	if (best) *best = cur;
+	if (best_ms) *best_ms = cur_ms * stride;
 	return 0;
 }
 

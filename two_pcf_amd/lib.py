@@ -18,7 +18,7 @@ EXPORTS = [
     "twopcf_abi_version", "twopcf_ctx_create", "twopcf_ctx_destroy", "twopcf_last_error",
     "twopcf_device_info", "twopcf_catalog_upload", "twopcf_catalog_free", "twopcf_catalog_size",
     "twopcf_count", "twopcf_s2_thresholds", "twopcf_axis_thresholds", "twopcf_comoving_distance",
-    "twopcf_eq_to_cart",
+    "twopcf_eq_to_cart", "twopcf_autotune",
 ]
 
 
@@ -82,6 +82,8 @@ def lib():
         L.twopcf_comoving_distance.argtypes = [C.c_double, C.c_double, C.c_double, C.c_double, C.c_int64,
                                                C.c_void_p, C.c_void_p]
         L.twopcf_eq_to_cart.argtypes = [C.c_int64] + [C.c_void_p] * 9
+        L.twopcf_autotune.argtypes = [C.c_void_p, C.POINTER(JobDesc), C.c_int, C.POINTER(Tuning),
+                                      C.POINTER(C.c_double)]
         _lib = L
     return _lib
 
@@ -196,7 +198,36 @@ class Context:
     def count(self, jobs, method=FP32_RECHECK, tuning=None):
         """jobs: list of dicts(kind, a, b, axes=[(min,max,n,base), ...], n_jk).
         Fills and returns, per job, dict(full, jk, n_pairs, n_rechecked);
-        plus the timing record."""
+        plus the timing record.  tuning None: the autotuned choice for this
+        job set if autotune() ran, else the defaults."""
+        descs, outs = self._descs(jobs)
+        t = Timing()
+        tu = None
+        if tuning is not None:
+            tu = Tuning(int(tuning.get("cell_subdiv", 0)), int(tuning.get("max_slots", 0)),
+                        int(tuning.get("i_chunk", 0)))
+        self.check(lib().twopcf_count(self.h, descs, len(jobs), method,
+                                      C.byref(tu) if tu is not None else None, C.byref(t)), "count")
+        res = []
+        for o in outs:
+            o.pop("_up", None)
+            o["n_pairs"] = int(o["n_pairs"][0])
+            o["n_rechecked"] = int(o["n_rechecked"][0])
+            res.append(o)
+        timing = {f: getattr(t, f) for f, _ in Timing._fields_}
+        return res, timing
+
+    def autotune(self, jobs):
+        """Choose the batched method's tuning for this job set (kept in the
+        context for later count() calls without a tuning).  Returns the
+        tuning dict and the estimated full-batch kernel time (ms)."""
+        descs, outs = self._descs(jobs)
+        best = Tuning()
+        ms = C.c_double()
+        self.check(lib().twopcf_autotune(self.h, descs, len(jobs), C.byref(best), C.byref(ms)), "autotune")
+        return dict(cell_subdiv=best.cell_subdiv, max_slots=best.max_slots, i_chunk=best.i_chunk), ms.value
+
+    def _descs(self, jobs):
         descs = (JobDesc * len(jobs))()
         outs = []
         for k, j in enumerate(jobs):
@@ -227,21 +258,7 @@ class Context:
             D.jk = o["jk"].ctypes.data
             D.n_pairs = o["n_pairs"].ctypes.data
             D.n_rechecked = o["n_rechecked"].ctypes.data
-        t = Timing()
-        tu = None
-        if tuning:
-            tu = Tuning(int(tuning.get("cell_subdiv", 0)), int(tuning.get("max_slots", 0)),
-                        int(tuning.get("i_chunk", 0)))
-        self.check(lib().twopcf_count(self.h, descs, len(jobs), method,
-                                      C.byref(tu) if tu is not None else None, C.byref(t)), "count")
-        res = []
-        for o in outs:
-            o.pop("_up", None)
-            o["n_pairs"] = int(o["n_pairs"][0])
-            o["n_rechecked"] = int(o["n_rechecked"][0])
-            res.append(o)
-        timing = {f: getattr(t, f) for f, _ in Timing._fields_}
-        return res, timing
+        return descs, outs
 
     def close(self):
         if self.h:

@@ -130,3 +130,20 @@ def uniform_monopole(seed=20260101):
     d = uniform_cube(50_000, 500.0, rng)
     r = uniform_cube(100_000, 500.0, rng)
     return dict(data=d, randoms=r)
+
+
+def uneven_mocks(n_mocks=64, n_min=20_000, n_max=2_000_000, density=3e-4, seed=20260103):
+    """Independent cartesian mocks of uneven size: N_k log-uniform in
+    [n_min, n_max], Neyman-Scott (N_k / 50 parents, Poisson(50) children,
+    5 Mpc/h) in a cube of side (N_k / density)^(1/3), each with 5 N_k uniform
+    randoms in the same cube; unit weights.  Returns a list of dicts with
+    `data` and `randoms` position arrays (n, 3) and the cube side."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    sizes = np.round(np.exp(rng.uniform(np.log(n_min), np.log(n_max), n_mocks))).astype(np.int64)
+    out = []
+    for k, n in enumerate(sizes):
+        side = float((n / density) ** (1.0 / 3.0))
+        d = neyman_scott_box(int(n), 50.0, 5.0, side, seed=seed + 17 * (k + 1))
+        r = uniform_cube(5 * int(n), side, np.random.Generator(np.random.PCG64(seed + 17 * (k + 1) + 1)))
+        out.append(dict(data=d, randoms=r, side=side))
+    return out
