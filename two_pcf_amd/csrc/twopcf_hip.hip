@@ -1327,6 +1327,7 @@ int build_plan(twopcf_ctx *ctx, const twopcf_job *jobs, int n_jobs, int method, 
 	// ---- grids, tables, outputs, work items ----
 	std::vector<Prepared> prep(n_jobs);
 	std::vector<float> tables;
+	std::map<std::string, int> table_at;
 	std::vector<int> slot_pool;
 	std::vector<WorkItem> items;
 	std::vector<double> item_cost;
@@ -1335,7 +1336,18 @@ int build_plan(twopcf_ctx *ctx, const twopcf_job *jobs, int n_jobs, int method, 
 	// table sizes are known up front: every job pushes at most 3 * (n1 + 1)
 	// + (n2 + 1) floats
 	size_t tab_floats = 1;
-	for (auto &H : hj) tab_floats += 7 * (H.ax1.n + 1) + (H.ax2.n + 1) + 16;
+	{
+		// per distinct (kind, axes): at most 7 * (n1 + 1) + (n2 + 1) floats
+		// plus alignment; jobs on the same axes share their tables
+		std::vector<std::string> seen;
+		for (auto &H : hj) {
+			std::string k((const char *)&H.j->kind, sizeof H.j->kind);
+			k.append((const char *)H.j->ax, sizeof H.j->ax);
+			if (std::find(seen.begin(), seen.end(), k) != seen.end()) continue;
+			seen.push_back(k);
+			tab_floats += 7 * (H.ax1.n + 1) + (H.ax2.n + 1) + 32;
+		}
+	}
 	const int lds_fixed = (int)(((std::max(tile_bytes(true), tile_bytes(false)) + 15) & ~15) + ((tab_floats * 4 + 15) & ~15) +
 	                            ((MAX_SLOTS * 4 + 15) & ~15));
 	const size_t lds_budget = ctx->lds_max;
@@ -1380,9 +1392,16 @@ int build_plan(twopcf_ctx *ctx, const twopcf_job *jobs, int n_jobs, int method, 
 		D.max_r_sq = H.max_r_sq;
 		D.ax1 = AxisDev{H.ax1.n, H.ax1.log ? 1 : 0, H.ax1.minus_min, H.ax1.inv_step};
 		D.ax2 = AxisDev{H.ax2.n, H.ax2.log ? 1 : 0, H.ax2.minus_min, H.ax2.inv_step};
+		// identical tables (jobs on the same axes) are stored once; every
+		// table starts 16-byte aligned
 		auto push = [&](const std::vector<float> &v) {
+			std::string k((const char *)v.data(), v.size() * sizeof(float));
+			auto f = table_at.find(k);
+			if (f != table_at.end()) return f->second;
+			while (tables.size() % 4) tables.push_back(0.0f);
 			int off = (int)tables.size();
 			tables.insert(tables.end(), v.begin(), v.end());
+			table_at[k] = off;
 			return off;
 		};
 		if (H.j->kind == TWOPCF_ANGULAR) {
@@ -1407,7 +1426,6 @@ int build_plan(twopcf_ctx *ctx, const twopcf_job *jobs, int n_jobs, int method, 
 			}
 			D.tA_off = push(A_);
 			D.tB_off = push(B_);
-			while (tables.size() % 4) tables.push_back(0.0f);
 			std::vector<float> Q(4 * H.t_s2.size(), 0.0f);
 			for (size_t k = 0; k < H.t_s2.size(); ++k) {
 				Q[4 * k] = H.t_s2[k];
