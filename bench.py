@@ -34,11 +34,10 @@ FLOP_PER_PAIR = 26            # FP32 flop per accepted s-mu pair
 # mu divide 1, mu index 1, less nothing)
 FLOP_BY_KIND = {0: 15, 2: 26}
 # HBM bytes per launch of the counting kernel on the default workload (scale
-# 1, 100 regions, default tuning), from two rocprofv3 --pmc runs:
-# FETCH_SIZE x 2 + WRITE_SIZE x 1 (factors from a 1 GiB copy), KiB -> B.
-# Source: profiles/k_count_fp32_jackknife_batch_hbm.md
-TRAFFIC_BYTES = {(1.0, 100): (8.261e7 * 2 + 3.755e6) * 1024}
-DEFAULT_TUNING = dict(cell_subdiv=2, max_slots=8, i_chunk=256)
+# 1, 100 regions) under the tuning the autotuner picks for it, from two
+# rocprofv3 --pmc runs: FETCH_SIZE x 2 + WRITE_SIZE x 1 (factors from a 1 GiB
+# copy), KiB -> B.  Source: profiles/k_count_fp32_jackknife_batch_hbm.md
+TRAFFIC_BYTES = {(1.0, 100, 2, 12, 256): (8.419e7 * 2 + 2.579e6) * 1024}
 PEAK_FP32_TFLOPS = 157.3      # MI355X FP32 vector peak
 AXES = [(0.0, 100.0, 40, 1.3), (0.0, 1.0, 50, 0.0)]
 
@@ -264,7 +263,10 @@ def main():
 
     flop = sum(r["n_pairs"] * FLOP_BY_KIND[j["kind"]] for r, j in zip(res, jobs))
     achieved = flop / kernel_s / 1e12
-    default_tuning = not tuning and (tuned is None or tuned["choice"] == DEFAULT_TUNING)
+    tu_used = tuned["choice"] if tuned else dict(cell_subdiv=tuning.get("cell_subdiv", 2),
+                                                   max_slots=tuning.get("max_slots", 8), i_chunk=256)
+    traffic_key = (W["traffic_key"] + (tu_used["cell_subdiv"], tu_used["max_slots"], tu_used["i_chunk"])
+                   if W["traffic_key"] else None)
     config = dict(W["config"])
     config.update({"parallelism": "single GPU", "tuning": tuning or (tuned if tuned else "default")})
     line = {
@@ -285,7 +287,7 @@ def main():
         "speedup_vs_plain_fp64": (plain["kernel_ms"] / float(np.mean(kms))) if plain else None,
         "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / PEAK_FP32_TFLOPS,
-                     "traffic": TRAFFIC_BYTES.get(W["traffic_key"]) if default_tuning else None,
+                     "traffic": TRAFFIC_BYTES.get(traffic_key),
                      "note": "FP32 vector ALU; algorithmic flop per accepted pair: s-mu 26, monopole 15"},
     }
     if not args.no_cpu and W["cpu"] is not None:
