@@ -204,8 +204,8 @@ def main():
         copy_calibration(args.calib_copy_mib)
         return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
-        print("bench.py measures one GPU only: this build has no multi-GPU path "
-              "(the sharding design is described in DESIGN.md)", file=sys.stderr)
+        print("bench.py measures one GPU only: there is no multi-GPU path "
+              "(--gpus must be 1)", file=sys.stderr)
         sys.exit(2)
 
     from two_pcf_amd import lib as L

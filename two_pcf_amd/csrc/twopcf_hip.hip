@@ -80,6 +80,7 @@ struct JobDev {
 	float s_min, s_ilb_ln2, s_ils, s_inv_step;       // fast s-index estimate
 	int s_log;
 	float mu_c1, mu_c2;                              // fast mu margin: delta = c1 + c2 * rsq(s2)
+	int f_shift;                                     // fp32 path: full histogram in 2^f_shift lane copies
 	int row_off, n_rows;                             // stencil rows in the row pool
 	int wmode, n_runs, n_words;                      // pair-weight rule (TWOPCF_W_*)
 	int up_n;                                        // angular up-weight table
@@ -477,8 +478,12 @@ __device__ void process_item_fast(const JobDev &J, const WorkItem &W, const int4
 	int2 *rq = (int2 *)((unsigned *)(cib + ICHUNK) + (BLOCK / WAVE) * RING) + wave * RQ;   // (l2, gj)
 	float *tab = (float *)(smem + L.tables);
 	int *sreg = (int *)(smem + L.slots);
+	// the full histogram: 2^f_shift copies, bin-major, lane l adding to copy
+	// l mod 2^f_shift (small histograms: lanes of one atomic never collide)
+	const int f_shift = J.f_shift;
+	const int f_lane = lane & ((1 << f_shift) - 1);
 	HT *F = (HT *)(smem + L.hist);
-	HT *H = F + nb;
+	HT *H = F + (nb << f_shift);
 
 	for (int k = tid; k < n_slots; k += BLOCK) sreg[k] = slot_regions[W.slot_base + k];
 	const int d1 = J.dim[1], d2 = J.dim[2];
@@ -523,8 +528,9 @@ __device__ void process_item_fast(const JobDev &J, const WorkItem &W, const int4
 		}
 	};
 	auto add = [&](int bin, float w, int si, int sj) {
-		if (WEIGHTED) atomicAdd((double *)&F[bin], (double)w);
-		else atomicAdd((unsigned int *)&F[bin], 1u);
+		const int fb = (bin << f_shift) + f_lane;
+		if (WEIGHTED) atomicAdd((double *)&F[fb], (double)w);
+		else atomicAdd((unsigned int *)&F[fb], 1u);
 		if (jk) {
 			add_region(si, bin, w);
 			if (sj != si) add_region(sj, bin, w);
@@ -711,11 +717,14 @@ __device__ void process_item_fast(const JobDev &J, const WorkItem &W, const int4
 	__syncthreads();   // every wave's rechecked pairs are in the LDS histograms
 
 	for (int b = tid; b < nb; b += BLOCK) {
-		HT v = F[b];
+		HT v = (HT)0;
+		for (int r = 0; r < (1 << f_shift); ++r) {
+			v += F[(b << f_shift) + r];
+			F[(b << f_shift) + r] = (HT)0;
+		}
 		if (v != (HT)0) {
 			if (WEIGHTED) atomicAdd(&J.full_d[b], (double)v);
 			else atomicAdd(&J.full_u[b], (unsigned long long)v);
-			F[b] = (HT)0;
 		}
 	}
 	if (jk) {
@@ -1014,12 +1023,15 @@ struct Prepared {
 	size_t full_n, jk_n;
 	size_t full_off, jk_off;   // offsets into the device output pools
 	bool weighted;
+	int f_elems = 0;           // LDS elements of the full histogram (lane copies included)
 };
 
-// Fast-path error model: see DESIGN.md "FP32 + edge recheck".  E(S) bounds
-// |s2_fp32 - s2_ref| for true squared separation S, with R the summed
-// per-coordinate distance of both objects from the first object's cell
-// centre.
+// Error model of the FP32 path.  Both objects are held as float offsets
+// from the centre of the first object's cell, each coordinate within R of
+// it, so each offset carries a rounding error of at most 2^-24 R.  The FP32
+// difference, the multiply-adds of s^2 and the reference's own rounding of
+// d = (float)(x1 - x2) and of s^2 = (d0^2 + d1^2) + d2^2 then keep the FP32
+// s^2 within E(S) of the reference's s^2 at true squared separation S.
 inline double e_s2(double S, double R) { return std::ldexp(1.0, -23) * (11.0 * S + 4.0 * R * std::sqrt(S)); }
 
 float round_up(double v) {
@@ -1446,6 +1458,11 @@ int build_plan(twopcf_ctx *ctx, const twopcf_job *jobs, int n_jobs, int method, 
 		auto rr = rows_of[H.rmax];
 		D.row_off = rr.first;
 		D.n_rows = rr.second;
+		// small histograms of the fp32 path get one copy per lane
+		D.f_shift = (fast && (H.j->kind == TWOPCF_MONOPOLE || H.j->kind == TWOPCF_S_MU) &&
+		             (size_t)H.nb * WAVE * (H.weighted ? 8 : 4) <= 16384)
+		                ? 6 : 0;
+		P.f_elems = H.nb << D.f_shift;
 		D.wmode = H.j->weight_mode;
 		D.n_runs = H.j->n_runs;
 		D.n_words = H.a->n_words;
@@ -1487,9 +1504,10 @@ int build_plan(twopcf_ctx *ctx, const twopcf_job *jobs, int n_jobs, int method, 
 		// grouped so that each item sees at most slot_cap regions; the slot
 		// cap also keeps (1 + slots) histograms inside the LDS budget
 		const int hist_bytes = H.nb * (H.weighted ? 8 : 4);
+		const int f_bytes = hist_bytes << D.f_shift;
 		const int slot_cap =
-		    std::max(1, std::min(max_slots, (int)((lds_budget - lds_fixed) / std::max(hist_bytes, 1)) - 1));
-		if ((long long)lds_fixed + 2LL * hist_bytes > (long long)ctx->lds_max)
+		    std::max(1, std::min(max_slots, (int)((lds_budget - lds_fixed - f_bytes) / std::max(hist_bytes, 1))));
+		if ((long long)lds_fixed + f_bytes + hist_bytes > (long long)ctx->lds_max)
 			return fail(ctx, TWOPCF_E_LIMIT, "histogram does not fit in LDS");
 		const int n_cell = H.g.dim[0] * H.g.dim[1] * H.g.dim[2];
 		const int d1 = H.g.dim[1], d2 = H.g.dim[2];
@@ -1670,19 +1688,20 @@ int build_plan(twopcf_ctx *ctx, const twopcf_job *jobs, int n_jobs, int method, 
 	int max_nb = 1, max_tab = (int)tables.size();
 	for (auto &H : hj) max_nb = std::max(max_nb, H.nb);
 	(void)max_tab_n1;
-	auto layout = [&](bool fast_kernel, bool weighted, int nb, int slots) {
+	// f_elems: elements of the largest full histogram (with its lane copies)
+	auto layout = [&](bool fast_kernel, bool weighted, int nb, int slots, int f_elems) {
 		Lds L;
 		L.tile = 0;
 		L.tables = (tile_bytes(fast_kernel) + 15) & ~15;
 		L.slots = L.tables + (((int)tables.size() * 4 + 15) & ~15);
 		L.hist = L.slots + ((MAX_SLOTS * 4 + 15) & ~15);
-		L.total = L.hist + nb * (1 + slots) * (weighted ? 8 : 4);
+		L.total = L.hist + (f_elems + nb * slots) * (weighted ? 8 : 4);
 		return L;
 	};
 	auto launch = [&](int kind, bool weighted, const WorkItem *it, int n_it, int nb, int slots, unsigned *queue,
-	                  int grid) -> int {
+	                  int grid, int f_elems) -> int {
 		const bool fast_kernel = fast && (kind == K_MONO || kind == K_S_MU);
-		Lds L = layout(fast_kernel, weighted, nb, slots);
+		Lds L = layout(fast_kernel, weighted, nb, slots, f_elems);
 		if ((size_t)L.total > ctx->lds_max)
 			return fail(ctx, TWOPCF_E_LIMIT, "histogram does not fit in LDS (" + std::to_string(L.total) + " B)");
 		void (*fn)(const JobDev *, const WorkItem *, int, unsigned *, const int4 *, const int *, const float *, int,
@@ -1714,7 +1733,7 @@ int build_plan(twopcf_ctx *ctx, const twopcf_job *jobs, int n_jobs, int method, 
 			int slots = 0;
 			for (size_t k = k0; k < k1; ++k) slots = std::max(slots, items[k].n_slots);
 			if ((rc = launch(hj[q].j->kind, prep[q].weighted, d_items + k0, (int)(k1 - k0), prep[q].nb, slots, nullptr,
-			                 (int)(k1 - k0)))) {
+			                 (int)(k1 - k0), prep[q].f_elems))) {
 				release();
 				return rc;
 			}
@@ -1741,17 +1760,20 @@ int build_plan(twopcf_ctx *ctx, const twopcf_job *jobs, int n_jobs, int method, 
 		for (auto &sp : spans) {
 			const int kind = std::get<0>(sp);
 			const bool weighted = std::get<1>(sp) != 0;
-			int nb = 1;
-			for (auto &H : hj)
-				if (H.j->kind == kind) nb = std::max(nb, H.nb);
+			int nb = 1, f_elems = 1;
+			for (int q = 0; q < n_jobs; ++q)
+				if (hj[q].j->kind == kind && prep[q].weighted == weighted) {
+					nb = std::max(nb, hj[q].nb);
+					f_elems = std::max(f_elems, prep[q].f_elems);
+				}
 			int slots = 0;
 			for (size_t k = std::get<2>(sp); k < std::get<2>(sp) + std::get<3>(sp); ++k)
 				slots = std::max(slots, flat[k].n_slots);
-			Lds L = layout(kind == K_MONO || kind == K_S_MU, weighted, nb, slots);
+			Lds L = layout(kind == K_MONO || kind == K_S_MU, weighted, nb, slots, f_elems);
 			const int per_cu = std::max(1, std::min(8, (int)(ctx->lds_max / std::max(L.total, 1))));
 			const int grid = std::min<long long>((long long)ctx->n_cu * per_cu, (long long)std::get<3>(sp));
 			if ((rc = launch(kind, weighted, d_items + std::get<2>(sp), (int)std::get<3>(sp), nb, slots, d_queue + qi,
-			                 std::max(grid, 1)))) {
+			                 std::max(grid, 1), f_elems))) {
 				release();
 				return rc;
 			}
