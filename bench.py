@@ -37,7 +37,8 @@ FLOP_BY_KIND = {0: 15, 2: 26}
 # 1, 100 regions) under the tuning the autotuner picks for it, from two
 # rocprofv3 --pmc runs: FETCH_SIZE x 2 + WRITE_SIZE x 1 (factors from a 1 GiB
 # copy), KiB -> B.  Source: profiles/k_count_fp32_jackknife_batch_hbm.md
-TRAFFIC_BYTES = {(1.0, 100, 2, 12, 256): (8.419e7 * 2 + 2.579e6) * 1024}
+TRAFFIC_BYTES = {(1.0, 100, 2, 8, 256): (8.261e7 * 2 + 3.755e6) * 1024,
+                 (1.0, 100, 2, 12, 256): (8.419e7 * 2 + 2.579e6) * 1024}
 PEAK_FP32_TFLOPS = 157.3      # MI355X FP32 vector peak
 AXES = [(0.0, 100.0, 40, 1.3), (0.0, 1.0, 50, 0.0)]
 

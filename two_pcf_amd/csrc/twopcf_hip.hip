@@ -1973,7 +1973,7 @@ int twopcf_autotune(twopcf_ctx *ctx, const twopcf_job *jobs, int n_jobs, twopcf_
 			int r = trial(t, &ms);
 			if (r == TWOPCF_E_LIMIT) continue;   // not feasible for these jobs (stencil, LDS)
 			if (r) return r;
-			if (ms < cur_ms) {
+			if (ms < 0.99 * cur_ms) {   // a change must gain more than 1 % (timing noise)
 				cur = t;
 				cur_ms = ms;
 			}
