@@ -44,3 +44,156 @@ def test_driver_fails_loudly_without_gpu(tmp_path):
     r = run_driver(CASES[0], tmp_path, "fp32")
     assert r.returncode != 0
     assert "no usable GPU" in r.stderr
+
+
+# ---------------------------------------------------------------- inverse-p --
+
+DRIVER_INVP = os.path.join(ROOT, "two_pcf_amd", "twopcf_invp")
+N_RUNS = 128
+
+
+def _invp_inputs(seed=41):
+    """Equatorial clustered data with 2-word bit masks and weights, randoms
+    with all-ones masks (their IIP weight is unchanged), 2 regions."""
+    import numpy as np
+    from two_pcf_amd import synth
+    rng = np.random.Generator(np.random.PCG64(seed))
+    ra_l, dec_l, z_l = (180.0, 190.0), (10.0, 20.0), (0.1, 0.15)
+    ra, dec, z = synth.neyman_scott_equatorial(60, 25.0, 4.0, ra_l, dec_l, z_l, seed=seed)
+    rra, rdec, rz = synth.randoms_equatorial(3000, ra_l, dec_l, z, seed=seed + 1)
+    out = {}
+    for tag, (a, b, c) in (("data", (ra, dec, z)), ("rand", (rra, rdec, rz))):
+        n = a.size
+        if tag == "data":
+            p = rng.uniform(0.3, 0.95, n)
+            bits = rng.random((n, N_RUNS)) < p[:, None]
+            masks = np.packbits(bits.astype(np.uint8), axis=1, bitorder="little").view(np.uint64).reshape(n, 2)
+        else:
+            masks = np.full((n, 2), np.uint64(0xFFFFFFFFFFFFFFFF), dtype=np.uint64)
+        w = rng.uniform(0.5, 1.5, n).astype(np.float32)
+        jk = (a > 185.0).astype(np.int32)
+        out[tag] = dict(ra=a, dec=b, z=c, w=w, jk=jk, masks=masks)
+    return out
+
+
+INVP_PARAMS = dict(
+    data_file_type="ascii", random_file_type="ascii", coord_system="equatorial",
+    use_weights=1, n_jk_regions=2, omega_m=0.25, h=1.0, z_min=0.0, z_max=1.0,
+    plot_monopole=1, monopole_filename="mono.txt", monopole_output_type="ascii",
+    monopole_log_base=1.3, monopole_min=0.0, monopole_max=40.0, monopole_n_bins=10,
+    plot_sigma_pi=0, plot_s_mu=1, s_mu_filename="smu.txt", s_mu_output_type="ascii",
+    s_log_base=1.3, s_min=0.0, s_max=40.0, s_n_bins=10, mu_n_bins=5,
+    calculate_angular_dd=1, calculate_angular_dd_invpweights=1,
+    calculate_angular_dr=1, calculate_angular_dr_invpweights=1,
+    angular_dd_filename="add.hdf5", angular_dd_invpweights_filename="addinv.hdf5",
+    angular_dr_filename="adr.hdf5", angular_dr_invpweights_filename="adrinv.hdf5",
+    theta_max=0.03, theta_n_bins=6, theta_log_base=1.0, healpix_order=5,
+    n_bitwise_runs=N_RUNS, bitwise_weight_dataset_name="BITWEIGHT",
+    data_filename="data.txt", random_filename="rand.txt",
+)
+
+
+def _expected_invp(o, inp):
+    """The inverse-p pipeline from the oracle: positions (cosmology table,
+    Galaxy::set), IIP weights (io.cpp:578-582), angular DD/DR with parent and
+    inverse-p weights, up-weights DD/DDinvp and DR/DRinvp (divide_by_hist),
+    3-D DD (inverse-p), DR (up-weight), RR (plain), invert/x2, normalise and
+    the LS estimator, formatted as the reference's ascii writer does."""
+    import numpy as np
+    p = INVP_PARAMS
+    table = o.cosmo_table(p["omega_m"], p["h"], p["z_min"], p["z_max"])
+    cats = {}
+    for tag, a in inp.items():
+        d = o.comov_dist(table, p["z_min"], a["z"])
+        x, y, z, nx, ny, nz = o.eq_to_cart(a["ra"], a["dec"], d)
+        tot = np.array([sum(int(int(v)).bit_count() for v in row) for row in a["masks"]])
+        tot[tot == 0] = 1
+        w = np.where(tot != N_RUNS, (a["w"].astype(np.float64) * (N_RUNS / tot)).astype(np.float32), a["w"])
+        cats[tag] = o.Catalog(x, y, z, nx, ny, nz, w, a["jk"], iw=a["w"], masks=a["masks"])
+    D, R = cats["data"], cats["rand"]
+    n_jk = p["n_jk_regions"]
+    ang = o.axis(0.0, float(1.0 - np.cos(p["theta_max"])), p["theta_n_bins"], p["theta_log_base"])
+    none = o.Axis()
+
+    def fin(kind, a, b, ax1, ax2, wo):
+        full, sub, _ = o.count(kind, a, b, ax1, ax2, n_jk, wopt=wo)
+        return o.finish(b is None, full, sub)
+
+    add = fin(3, D, None, ang, None, o.WOpt(3, N_RUNS, None, none))[0]
+    addi = fin(3, D, None, ang, None, o.WOpt(4, N_RUNS, None, none))[0]
+    adr = fin(3, D, R, ang, None, o.WOpt(3, N_RUNS, None, none))[0]
+    adri = fin(3, D, R, ang, None, o.WOpt(5, N_RUNS, None, none))[0]
+    up_dd = np.ascontiguousarray(add / addi)
+    up_dr = np.ascontiguousarray(adr / adri)
+    files = {}
+    for name, kind, axes in (("mono.txt", 0, [(0.0, 40.0, 10, 1.3)]),
+                             ("smu.txt", 2, [(0.0, 40.0, 10, 1.3), (0.0, 1.0, 5, 0.0)])):
+        ax1 = o.axis(*axes[0])
+        ax2 = o.axis(*axes[1]) if len(axes) > 1 else None
+        dd = fin(kind, D, None, ax1, ax2, o.WOpt(1, N_RUNS, up_dd.ctypes.data, ang))
+        dr = fin(kind, D, R, ax1, ax2, o.WOpt(2, N_RUNS, up_dr.ctypes.data, ang))
+        rr = fin(kind, R, None, ax1, ax2, None)
+        dd = o.normalise_auto(*dd, D)
+        dr = o.normalise_cross(*dr, D, R)
+        rr = o.normalise_auto(*rr, R)
+        L = o.lib()
+        c1 = [L.or_axis_bin_centre(o.C.byref(ax1), k) for k in range(ax1.n_bins)]
+        w1 = [L.or_axis_bin_width(o.C.byref(ax1), k) for k in range(ax1.n_bins)]
+        if ax2 is not None:
+            c2 = [L.or_axis_bin_centre(o.C.byref(ax2), k) for k in range(ax2.n_bins)]
+            w2 = [L.or_axis_bin_width(o.C.byref(ax2), k) for k in range(ax2.n_bins)]
+
+        def text(DD, DR, RR):
+            rows = []
+            for i in range(ax1.n_bins):
+                js = range(ax2.n_bins) if ax2 is not None else [None]
+                for j in js:
+                    b = i if j is None else i * ax2.n_bins + j
+                    xi = 0.0 if RR[b] == 0 else DD[b] / RR[b] - 2.0 * DR[b] / RR[b] + 1.0
+                    if j is None:
+                        vals = [i, c1[i], w1[i], xi, DD[b], DR[b], RR[b]]
+                    else:
+                        vals = [i, j, c1[i], c2[j], w1[i], w2[j], xi, DD[b], DR[b], RR[b]]
+                    rows.append(vals)
+            return rows
+
+        files[name] = text(dd[0], dr[0], rr[0])
+        stem, ext = name.rsplit(".", 1)
+        for r in range(n_jk):
+            files[f"{stem}{r}.{ext}"] = text(dd[1][r], dr[1][r], rr[1][r])
+    return files
+
+
+def _write_invp_inputs(tmp_path, inp):
+    for tag, a in inp.items():
+        with open(os.path.join(tmp_path, tag + ".txt"), "w") as f:
+            for i in range(a["ra"].size):
+                m = " ".join(str(int(v)) for v in a["masks"][i])
+                f.write("%.17g %.17g %.17g %.9g %d %s\n" % (a["ra"][i], a["dec"][i], a["z"][i], a["w"][i],
+                                                          a["jk"][i], m))
+    with open(os.path.join(tmp_path, "params.txt"), "w") as f:
+        f.write("".join(f"{k} = {v}\n" for k, v in INVP_PARAMS.items()))
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("method", ["fp64", "fp32"])
+def test_driver_invp_pipeline(method, tmp_path, oracle_lib):
+    """twopcf_invp (the inverse-p build of the driver) on ascii catalogues with
+    bit-mask columns: every output file equals the oracle's pipeline to the
+    6 significant digits of the ascii writer (a last-digit difference is
+    allowed: weighted sums are added in another order)."""
+    import numpy as np
+    print(f"[gpu] driver inverse-p {method}", flush=True)
+    inp = _invp_inputs()
+    _write_invp_inputs(tmp_path, inp)
+    env = dict(os.environ, TWOPCF_METHOD=method)
+    r = subprocess.run([DRIVER_INVP, "params.txt"], cwd=tmp_path, env=env, capture_output=True, text=True,
+                       timeout=300)
+    assert r.returncode == 0, r.stderr
+    want = _expected_invp(oracle_lib, inp)
+    for fn, rows in want.items():
+        with open(os.path.join(tmp_path, fn)) as f:
+            got = [[float(v) for v in line.split("\t")] for line in f.read().splitlines()]
+        assert len(got) == len(rows), fn
+        for g, w in zip(got, rows):
+            np.testing.assert_allclose(g, w, rtol=1.5e-5, atol=0, err_msg=fn)

@@ -10,6 +10,7 @@
 #include <cstring>
 #include <fstream>
 #include <iostream>
+#include <map>
 #include <sstream>
 
 // ------------------------------------------------------------------ context --
@@ -263,13 +264,14 @@ GalaxyBoxes::GalaxyBoxes(Galaxy in[], int n, Galaxy *, int, double r) : rmax(r) 
 	cat = upload(in, n_gals, nullptr, 0);
 }
 
-GalaxyBoxes::GalaxyBoxes(GalaxyCatalogue *in, GalaxyCatalogue *, double r) : rmax(r), source(in) {
-	// keep the masks aligned with the partitioned objects
+// Move the objects with d_comov < 0 behind the others, keeping the masks
+// aligned with their objects; returns the number in front.
+static int partition_catalogue(GalaxyCatalogue *in) {
 	const int n = in->n_gals, nw = in->n_words;
 	std::vector<int> order(n);
 	for (int i = 0; i < n; ++i) order[i] = i;
 	auto mid = std::stable_partition(order.begin(), order.end(), [&](int i) { return !(in->gals[i].d_comov < 0.0); });
-	n_gals = (int)(mid - order.begin());
+	const int n_valid = (int)(mid - order.begin());
 	std::vector<Galaxy> g(n);
 	std::vector<uint64_t> m((size_t)n * nw);
 	for (int i = 0; i < n; ++i) {
@@ -278,7 +280,29 @@ GalaxyBoxes::GalaxyBoxes(GalaxyCatalogue *in, GalaxyCatalogue *, double r) : rma
 	}
 	std::copy(g.begin(), g.end(), in->gals);
 	in->masks.swap(m);
-	cat = upload(in->gals, n_gals, &in->masks, nw);
+	return n_valid;
+}
+
+GalaxyBoxes::GalaxyBoxes(GalaxyCatalogue *in, GalaxyCatalogue *, double r) : rmax(r), source(in) {
+	n_gals = partition_catalogue(in);
+	cat = upload(in->gals, n_gals, &in->masks, in->n_words);
+}
+
+// boxes.cpp:215-240: with require_redshift the objects without a distance go
+// to the back and are left out; otherwise every object takes part (angular
+// counts use only the unit vectors)
+GalaxyPixels::GalaxyPixels(Galaxy in[], int n, int, double t, bool require_redshift) : theta_max(t) {
+	n_gals = require_redshift ? partition_valid(in, n) : n;
+	cat = upload(in, n_gals, nullptr, 0);
+}
+
+GalaxyPixels::GalaxyPixels(GalaxyCatalogue *in, int, double t, bool require_redshift) : theta_max(t) {
+	n_gals = require_redshift ? partition_catalogue(in) : in->n_gals;
+	cat = upload(in->gals, n_gals, &in->masks, in->n_words);
+}
+
+GalaxyPixels::~GalaxyPixels() {
+	if (cat) twopcf_catalog_free(cat);
 }
 
 GalaxyBoxes::~GalaxyBoxes() {
@@ -422,6 +446,262 @@ void auto_correlate_s_mu(GalaxyCatalogue *data, GalaxyCatalogue *randoms, Params
 	output_xi_2D(p.s_mu_filename, p.s_mu_output_type, "s", "mu", &DD, &DR, &RR, data, randoms);
 }
 
+// ------------------------------------------------------------- inverse-p --
+
+namespace {
+
+// Angular histograms by file name (in place of the HDF5 files)
+struct StoredHist {
+	float min, max, base;
+	int n_bins, n_jk;
+	std::vector<double> full, sub;
+};
+std::map<std::string, StoredHist> &angular_store() {
+	static std::map<std::string, StoredHist> m;
+	return m;
+}
+
+twopcf_job job_ang(int mode, const GalaxyPixels *a, const GalaxyPixels *b, JKHists *h, int n_runs) {
+	twopcf_job j{};
+	j.kind = TWOPCF_ANGULAR;
+	j.a = a->cat;
+	j.b = b ? b->cat : nullptr;
+	j.ax[0] = h->full_hist.ax.desc();
+	j.ax[1] = twopcf_axis{0.0, 1.0, 1, 0.0};
+	j.n_jk = h->n_jk_regions;
+	j.weight_mode = mode;
+	j.n_runs = n_runs;
+	return j;
+}
+
+void count_ang(int mode, const GalaxyPixels *a, const GalaxyPixels *b, JKHists *h, int n_runs) {
+	std::vector<twopcf_job> j{job_ang(mode, a, b, h, n_runs)};
+	std::vector<Out> o{out1(h, b == nullptr)};
+	run(j, o);
+}
+
+// The angular up-weight of the 3-D counts: the parent-weight angular
+// histogram divided by the inverse-p one (correlators_invpweight.cpp:20-24),
+// bin by bin; JKHists::interp reads its full histogram.
+struct UpWeight {
+	JKHists *h = nullptr;
+	explicit UpWeight(const std::string &num, const std::string &den) {
+		h = read_angular_hist(num);
+		JKHists *d = read_angular_hist(den);
+		h->divide_by_hist(*d);
+		delete d;
+	}
+	~UpWeight() { delete h; }
+	void apply(twopcf_job &j, int mode, int n_runs) const {
+		j.weight_mode = mode;
+		j.n_runs = n_runs;
+		j.up_bins = h->full_hist.bins;
+		j.up_axis = h->full_hist.ax.desc();
+	}
+};
+
+}  // namespace
+
+void store_angular_hist(const std::string &fn, const JKHists &h) {
+	StoredHist s;
+	s.min = h.full_hist.ax.min;
+	s.max = h.full_hist.ax.max;
+	s.base = h.full_hist.ax.base;
+	s.n_bins = h.full_hist.ax.n_bins;
+	s.n_jk = h.n_jk_regions;
+	s.full.assign(h.full_hist.bins, h.full_hist.bins + s.n_bins);
+	s.sub.assign(h.sub_hist, h.sub_hist + (size_t)s.n_jk * s.n_bins);
+	angular_store()[fn] = s;
+}
+
+JKHists *read_angular_hist(const std::string &fn) {
+	auto it = angular_store().find(fn);
+	if (it == angular_store().end()) {
+		std::fprintf(stderr, "twopcf: angular histogram %s was not computed in this run (set the matching "
+		                     "calculate_angular_* key; HDF5 input is not supported)\n", fn.c_str());
+		std::exit(1);
+	}
+	const StoredHist &s = it->second;
+	Hist t(s.min, s.max, s.n_bins, s.base);
+	JKHists *h = new JKHists(s.n_jk, t);
+	std::copy(s.full.begin(), s.full.end(), h->full_hist.bins);
+	std::copy(s.sub.begin(), s.sub.end(), h->sub_hist);
+	return h;
+}
+
+// correlators_invpweight.cpp:870-936, 938-983, 713-821, 823-868
+void auto_pair_count_angular(const GalaxyPixels *c, JKHists *h) { count_ang(TWOPCF_W_ANG_IW, c, nullptr, h, 0); }
+void cross_pair_count_angular(const GalaxyPixels *a, const GalaxyPixels *b, JKHists *h) {
+	count_ang(TWOPCF_W_ANG_IW, a, b, h, 0);
+}
+void auto_pair_count_angular_invpweights(const GalaxyPixels *c, JKHists *h, Params p) {
+	count_ang(TWOPCF_W_ANG_INVP, c, nullptr, h, p.n_bitwise_runs);
+}
+void cross_pair_count_angular_invpweights(const GalaxyPixels *a, const GalaxyPixels *b, JKHists *h) {
+	count_ang(TWOPCF_W_ANG_W, a, b, h, 0);
+}
+
+// correlators_invpweight.cpp:985-1033
+void calc_angular_upweight(GalaxyCatalogue *data, GalaxyCatalogue *randoms, Params p) {
+	Hist t(0.0, 1.0 - std::cos(p.theta_max), p.theta_n_bins, p.theta_log_base);
+	if (p.calculate_angular_dd) {
+		std::cout << "Calculating angular DDs\n";
+		GalaxyPixels dp(data, p.healpix_order, p.theta_max, false);
+		JKHists DD(p.n_jk_regions, t);
+		auto_pair_count_angular(&dp, &DD);
+		store_angular_hist(p.angular_dd_filename, DD);
+	}
+	if (p.calculate_angular_dd_invpweights) {
+		std::cout << "Calculating angular DDs with inverse p-weights\n";
+		GalaxyPixels dp(data, p.healpix_order, p.theta_max, true);
+		JKHists DD(p.n_jk_regions, t);
+		auto_pair_count_angular_invpweights(&dp, &DD, p);
+		store_angular_hist(p.angular_dd_invpweights_filename, DD);
+	}
+	if (p.calculate_angular_dr) {
+		std::cout << "Calculating angular DRs\n";
+		GalaxyPixels dp(data, p.healpix_order, p.theta_max, false);
+		GalaxyPixels rp(randoms, p.healpix_order, p.theta_max, false);
+		JKHists DR(p.n_jk_regions, t);
+		cross_pair_count_angular(&dp, &rp, &DR);
+		store_angular_hist(p.angular_dr_filename, DR);
+	}
+	if (p.calculate_angular_dr_invpweights) {
+		std::cout << "Calculating angular DRs with inverse weights\n";
+		GalaxyPixels dp(data, p.healpix_order, p.theta_max, true);
+		GalaxyPixels rp(randoms, p.healpix_order, p.theta_max, true);
+		JKHists DR(p.n_jk_regions, t);
+		cross_pair_count_angular_invpweights(&dp, &rp, &DR);
+		store_angular_hist(p.angular_dr_invpweights_filename, DR);
+	}
+}
+
+// correlators_invpweight.cpp:14-146, 148-208 (and the sigma-pi, s-mu
+// counterparts at 339-443, 579-680)
+void auto_pair_count_monopole_invpweights(const GalaxyBoxes *c, JKHists *h, Params p) {
+	UpWeight u(p.angular_dd_filename, p.angular_dd_invpweights_filename);
+	std::vector<twopcf_job> j{job1(TWOPCF_MONOPOLE, c, nullptr, h)};
+	u.apply(j[0], TWOPCF_W_INVP_3D, p.n_bitwise_runs);
+	std::vector<Out> o{out1(h, true)};
+	run(j, o);
+}
+void cross_pair_count_monopole_invpweights(const GalaxyBoxes *a, const GalaxyBoxes *b, JKHists *h, Params p) {
+	UpWeight u(p.angular_dr_filename, p.angular_dr_invpweights_filename);
+	std::vector<twopcf_job> j{job1(TWOPCF_MONOPOLE, a, b, h)};
+	u.apply(j[0], TWOPCF_W_ANGUP_3D, 0);
+	std::vector<Out> o{out1(h, false)};
+	run(j, o);
+}
+void auto_pair_count_sigma_pi_invpweights(const GalaxyBoxes *c, JKHists2D *h, Params p) {
+	UpWeight u(p.angular_dd_filename, p.angular_dd_invpweights_filename);
+	std::vector<twopcf_job> j{job2(TWOPCF_SIGMA_PI, c, nullptr, h)};
+	u.apply(j[0], TWOPCF_W_INVP_3D, p.n_bitwise_runs);
+	std::vector<Out> o{out2(h, true)};
+	run(j, o);
+}
+void cross_pair_count_sigma_pi_invpweights(const GalaxyBoxes *a, const GalaxyBoxes *b, JKHists2D *h, Params p) {
+	UpWeight u(p.angular_dr_filename, p.angular_dr_invpweights_filename);
+	std::vector<twopcf_job> j{job2(TWOPCF_SIGMA_PI, a, b, h)};
+	u.apply(j[0], TWOPCF_W_ANGUP_3D, 0);
+	std::vector<Out> o{out2(h, false)};
+	run(j, o);
+}
+void auto_pair_count_s_mu_invpweights(const GalaxyBoxes *c, JKHists2D *h, Params p) {
+	UpWeight u(p.angular_dd_filename, p.angular_dd_invpweights_filename);
+	std::vector<twopcf_job> j{job2(TWOPCF_S_MU, c, nullptr, h)};
+	u.apply(j[0], TWOPCF_W_INVP_3D, p.n_bitwise_runs);
+	std::vector<Out> o{out2(h, true)};
+	run(j, o);
+}
+void cross_pair_count_s_mu_invpweights(const GalaxyBoxes *a, const GalaxyBoxes *b, JKHists2D *h, Params p) {
+	UpWeight u(p.angular_dr_filename, p.angular_dr_invpweights_filename);
+	std::vector<twopcf_job> j{job2(TWOPCF_S_MU, a, b, h)};
+	u.apply(j[0], TWOPCF_W_ANGUP_3D, 0);
+	std::vector<Out> o{out2(h, false)};
+	run(j, o);
+}
+
+// correlators_invpweight.cpp:210-237, 445-474, 682-711: DD with inverse-p
+// weights, DR with the angular up-weight, RR plain — here in one batched call
+void auto_correlate_monopole_invpweights(GalaxyCatalogue *data, GalaxyCatalogue *randoms, Params p) {
+	std::cout << "Calculating autocorrelation monopole with inverse p weights\n";
+	Hist t(p.monopole_min, p.monopole_max, p.monopole_n_bins, p.monopole_log_base);
+	JKHists DD(p.n_jk_regions, t), DR(p.n_jk_regions, t), RR(p.n_jk_regions, t);
+	GalaxyBoxes db(data, randoms, p.monopole_max);
+	GalaxyBoxes rb(randoms, data, p.monopole_max);
+	UpWeight udd(p.angular_dd_filename, p.angular_dd_invpweights_filename);
+	UpWeight udr(p.angular_dr_filename, p.angular_dr_invpweights_filename);
+	std::vector<twopcf_job> jobs{job1(TWOPCF_MONOPOLE, &db, nullptr, &DD), job1(TWOPCF_MONOPOLE, &db, &rb, &DR),
+	                             job1(TWOPCF_MONOPOLE, &rb, nullptr, &RR)};
+	udd.apply(jobs[0], TWOPCF_W_INVP_3D, p.n_bitwise_runs);
+	udr.apply(jobs[1], TWOPCF_W_ANGUP_3D, 0);
+	std::vector<Out> outs{out1(&DD, true), out1(&DR, false), out1(&RR, true)};
+	run(jobs, outs);
+	DD.normalise(data, db.n_gals);
+	DR.normalise(data, db.n_gals, randoms, rb.n_gals);
+	RR.normalise(randoms, rb.n_gals);
+	output_xi_1D(p.monopole_filename, p.monopole_output_type, &DD, &DR, &RR, data, randoms);
+}
+
+void auto_correlate_sigma_pi_invpweights(GalaxyCatalogue *data, GalaxyCatalogue *randoms, Params p) {
+	std::cout << "Calculating autocorrelation sigma-pi decomposition with inverse p weights\n";
+	Hist2D t(p.sigma_min, p.sigma_max, p.sigma_n_bins, p.sigma_log_base, p.pi_min, p.pi_max, p.pi_n_bins,
+	         p.pi_log_base);
+	JKHists2D DD(p.n_jk_regions, t), DR(p.n_jk_regions, t), RR(p.n_jk_regions, t);
+	const double rmax = std::sqrt(p.sigma_max * p.sigma_max + p.pi_max * p.pi_max);
+	GalaxyBoxes db(data, randoms, rmax);
+	GalaxyBoxes rb(randoms, data, rmax);
+	UpWeight udd(p.angular_dd_filename, p.angular_dd_invpweights_filename);
+	UpWeight udr(p.angular_dr_filename, p.angular_dr_invpweights_filename);
+	std::vector<twopcf_job> jobs{job2(TWOPCF_SIGMA_PI, &db, nullptr, &DD), job2(TWOPCF_SIGMA_PI, &db, &rb, &DR),
+	                             job2(TWOPCF_SIGMA_PI, &rb, nullptr, &RR)};
+	udd.apply(jobs[0], TWOPCF_W_INVP_3D, p.n_bitwise_runs);
+	udr.apply(jobs[1], TWOPCF_W_ANGUP_3D, 0);
+	std::vector<Out> outs{out2(&DD, true), out2(&DR, false), out2(&RR, true)};
+	run(jobs, outs);
+	DD.normalise(data, db.n_gals);
+	DR.normalise(data, db.n_gals, randoms, rb.n_gals);
+	RR.normalise(randoms, rb.n_gals);
+	output_xi_2D(p.sigma_pi_filename, p.sigma_pi_output_type, "sigma", "pi", &DD, &DR, &RR, data, randoms);
+}
+
+void auto_correlate_s_mu_invpweights(GalaxyCatalogue *data, GalaxyCatalogue *randoms, Params p) {
+	std::cout << "Calculating autocorrelation s-mu decomposition with inverse p weights\n";
+	Hist2D t(p.s_min, p.s_max, p.s_n_bins, p.s_log_base, 0.0, 1.0, p.mu_n_bins, 0);
+	JKHists2D DD(p.n_jk_regions, t), DR(p.n_jk_regions, t), RR(p.n_jk_regions, t);
+	GalaxyBoxes db(data, randoms, p.s_max);
+	GalaxyBoxes rb(randoms, data, p.s_max);
+	UpWeight udd(p.angular_dd_filename, p.angular_dd_invpweights_filename);
+	UpWeight udr(p.angular_dr_filename, p.angular_dr_invpweights_filename);
+	std::vector<twopcf_job> jobs{job2(TWOPCF_S_MU, &db, nullptr, &DD), job2(TWOPCF_S_MU, &db, &rb, &DR),
+	                             job2(TWOPCF_S_MU, &rb, nullptr, &RR)};
+	udd.apply(jobs[0], TWOPCF_W_INVP_3D, p.n_bitwise_runs);
+	udr.apply(jobs[1], TWOPCF_W_ANGUP_3D, 0);
+	std::vector<Out> outs{out2(&DD, true), out2(&DR, false), out2(&RR, true)};
+	run(jobs, outs);
+	DD.normalise(data, db.n_gals);
+	DR.normalise(data, db.n_gals, randoms, rb.n_gals);
+	RR.normalise(randoms, rb.n_gals);
+	output_xi_2D(p.s_mu_filename, p.s_mu_output_type, "s", "mu", &DD, &DR, &RR, data, randoms);
+}
+
+static bool g_inverse_p = false;
+void twopcf_enable_inverse_p(bool on) { g_inverse_p = on; }
+bool twopcf_inverse_p_enabled() { return g_inverse_p; }
+
+// io.cpp:521-588 (without dither weights)
+void apply_bitwise_weights(GalaxyCatalogue *cat, const Params &p) {
+	const int nw = cat->n_words;
+	for (int i = 0; i < cat->n_gals; ++i) {
+		int total = 0;
+		for (int k = 0; k < nw; ++k) total += __builtin_popcountll(cat->masks[(size_t)i * nw + k]);
+		Galaxy &g = cat->gals[i];
+		g.input_weight = g.weight;
+		if (total == 0) total = 1;
+		if (total != p.n_bitwise_runs) g.weight *= (double)p.n_bitwise_runs / (double)total;
+	}
+}
+
 // ----------------------------------------------------------------------- io --
 
 static std::string strip_spaces(std::string s) {
@@ -537,6 +817,14 @@ GalaxyCatalogue *read_data_ascii(std::string filename, const Params &p) {
 		return cat;
 	}
 	std::vector<double> c1(n), c2(n), c3(n);
+	// inverse-p build: bit masks as extra integer columns
+	if (twopcf_inverse_p_enabled() && p.n_bitwise_runs <= 0) {
+		std::fprintf(stderr, "twopcf: the inverse-p pipeline needs n_bitwise_runs > 0\n");
+		std::exit(1);
+	}
+	const int n_words = twopcf_inverse_p_enabled() ? (p.n_bitwise_runs + 63) / 64 : 0;
+	cat->n_words = n_words;
+	cat->masks.assign((size_t)n * n_words, 0);
 	for (int i = 0; i < n; ++i) {
 		const std::string &l = lines[i];
 		char sep = ' ';
@@ -555,7 +843,18 @@ GalaxyCatalogue *read_data_ascii(std::string filename, const Params &p) {
 			             g.jk_region, p.n_jk_regions);
 			std::exit(1);
 		}
+		for (int k = 0; k < n_words; ++k) {
+			const std::string f = next_field(ss, sep);
+			if (f.empty()) {
+				std::fprintf(stderr, "twopcf: %s line %d: %d bit-mask columns expected\n", filename.c_str(), i + 1,
+				             n_words);
+				std::exit(1);
+			}
+			cat->masks[(size_t)i * n_words + k] =
+			    f[0] == '-' ? (uint64_t)std::stoll(f) : (uint64_t)std::stoull(f);
+		}
 	}
+	if (n_words > 0) apply_bitwise_weights(cat, p);
 	if (eq) {
 		std::vector<double> d(n);
 		check(twopcf_comoving_distance(p.omega_m, p.h, p.z_min, p.z_max, n, c3.data(), d.data()),

@@ -11,7 +11,17 @@
 //   * Params members are zero-initialised instead of left undefined when a
 //     key is missing (io.cpp:16-237 leaves them uninitialised).
 //   * Only ascii catalogues and ascii output are supported (the _HAVE_HDF5
-//     parts need libhdf5).
+//     parts need libhdf5).  In the inverse-p build the ascii reader takes
+//     the bit masks as ceil(n_bitwise_runs / 64) extra integer columns (the
+//     reference reads them from the HDF5 datasets <name>0, <name>1, ...).
+//   * The inverse-p build (_USE_INV_WEIGHTS) is a run-time switch,
+//     twopcf_enable_inverse_p(), set by the twopcf_invp driver; the angular
+//     histograms
+//     calc_angular_upweight produces are kept in the process under their
+//     file names instead of HDF5 files (read_hdf5_hist, hist.cpp:115-178).
+//   * GalaxyPixels accepts healpix_order and ignores it: angular candidates
+//     come from a grid on the unit vectors, a superset of the pixel discs,
+//     so the counts are the same.
 #pragma once
 #include <cstdint>
 #include <string>
@@ -124,6 +134,20 @@ public:
 	GalaxyBoxes &operator=(const GalaxyBoxes &) = delete;
 };
 
+// boxes.h:51-67: the device-resident catalogue of an angular count
+class GalaxyPixels {
+public:
+	int n_gals;
+	double theta_max;
+	twopcf_cat *cat = nullptr;
+	GalaxyPixels(Galaxy input_galaxies[], int input_n_gals, int healpix_order, double input_theta_max,
+	             bool require_redshift);
+	GalaxyPixels(GalaxyCatalogue *input, int healpix_order, double input_theta_max, bool require_redshift);
+	~GalaxyPixels();
+	GalaxyPixels(const GalaxyPixels &) = delete;
+	GalaxyPixels &operator=(const GalaxyPixels &) = delete;
+};
+
 // io.h:10-86
 class Params {
 public:
@@ -181,6 +205,40 @@ void cross_pair_count_s_mu(const GalaxyBoxes *cat_1, const GalaxyBoxes *cat_2, J
 void auto_correlate_monopole(GalaxyCatalogue *data, GalaxyCatalogue *randoms, Params params);
 void auto_correlate_sigma_pi(GalaxyCatalogue *data, GalaxyCatalogue *randoms, Params params);
 void auto_correlate_s_mu(GalaxyCatalogue *data, GalaxyCatalogue *randoms, Params params);
+
+// correlators_invpweight.h:9-33 and correlators.h:29-31 — the inverse-p
+// pipeline, same names and post-conditions.
+void auto_pair_count_angular(const GalaxyPixels *cat, JKHists *hist);
+void cross_pair_count_angular(const GalaxyPixels *cat_1, const GalaxyPixels *cat_2, JKHists *hist);
+void auto_pair_count_angular_invpweights(const GalaxyPixels *cat, JKHists *hist, Params params);
+void cross_pair_count_angular_invpweights(const GalaxyPixels *cat_1, const GalaxyPixels *cat_2, JKHists *hist);
+void calc_angular_upweight(GalaxyCatalogue *data, GalaxyCatalogue *randoms, Params params);
+void auto_pair_count_monopole_invpweights(const GalaxyBoxes *cat, JKHists *hist, Params params);
+void cross_pair_count_monopole_invpweights(const GalaxyBoxes *cat_1, const GalaxyBoxes *cat_2, JKHists *hist,
+                                           Params params);
+void auto_pair_count_sigma_pi_invpweights(const GalaxyBoxes *cat, JKHists2D *hist, Params params);
+void cross_pair_count_sigma_pi_invpweights(const GalaxyBoxes *cat_1, const GalaxyBoxes *cat_2, JKHists2D *hist,
+                                           Params params);
+void auto_pair_count_s_mu_invpweights(const GalaxyBoxes *cat, JKHists2D *hist, Params params);
+void cross_pair_count_s_mu_invpweights(const GalaxyBoxes *cat_1, const GalaxyBoxes *cat_2, JKHists2D *hist,
+                                       Params params);
+void auto_correlate_monopole_invpweights(GalaxyCatalogue *data, GalaxyCatalogue *randoms, Params params);
+void auto_correlate_sigma_pi_invpweights(GalaxyCatalogue *data, GalaxyCatalogue *randoms, Params params);
+void auto_correlate_s_mu_invpweights(GalaxyCatalogue *data, GalaxyCatalogue *randoms, Params params);
+
+// The inverse-p build switch: the readers then take bit masks and apply the
+// IIP weights (io.cpp:521-588).
+void twopcf_enable_inverse_p(bool on);
+bool twopcf_inverse_p_enabled();
+
+// io.cpp:521-588: input_weight = weight; weight *= n_runs / popcount(masks)
+// when the popcount differs from n_runs (0 counts as 1).
+void apply_bitwise_weights(GalaxyCatalogue *cat, const Params &params);
+
+// The angular histograms of calc_angular_upweight, by file name:
+// JKHists::output_hdf5 / read_hdf5_hist (hist.cpp:115-222) within one process.
+void store_angular_hist(const std::string &filename, const JKHists &hist);
+JKHists *read_angular_hist(const std::string &filename);   // new object, caller deletes; exits if absent
 
 // The context every entry point uses (device TWOPCF_DEVICE, default 0;
 // counting method TWOPCF_METHOD = fp32 (default) or fp64).  Aborts with a
