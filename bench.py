@@ -182,6 +182,95 @@ def workload_multi(args, L, ctx):
     return dict(jobs=jobs, upload_ms=up_ms, config=config, traffic_key=None, cpu=None)
 
 
+def _eq_upload(L, ctx, a, c, masks=False):
+    d = L.comoving_distance(c["om"], c["h"], c["z_min"], c["z_max"], a["z"])
+    x, y, z, nx, ny, nz = L.eq_to_cart(a["ra"], a["dec"], d)
+    return ctx.upload(x, y, z, nx, ny, nz, a["w"], a["jk"], iw=a.get("iw"), masks=a.get("masks") if masks else None)
+
+
+SP_AXES = [(0.0, 50.0, 50, 1.0), (0.0, 50.0, 50, 1.0)]
+
+
+def workload_equatorial(args, L, ctx):
+    """1M clustered galaxies (20k parents x Poisson(50), 5 Mpc/h) + 5M randoms
+    over ra 150-250, dec 0-60, z 0.2-0.6, weights U[0.5, 1.5]: s-mu (40 log
+    bins x 50 mu) and sigma-pi (50 x 50 linear to 50 Mpc/h) DD, DR and RR in
+    one call."""
+    from two_pcf_amd import synth
+    c = synth.equatorial_smu_sigma_pi(scale=args.scale)
+    t_up = time.perf_counter()
+    dc = {t: _eq_upload(L, ctx, c[t], c) for t in ("data", "randoms")}
+    upload_ms = 1000.0 * (time.perf_counter() - t_up)
+    jobs = []
+    for kind, axes in ((L.S_MU, AXES), (L.SIGMA_PI, SP_AXES)):
+        jobs += [dict(kind=kind, a=dc["data"], b=None, axes=axes, n_jk=1),
+                 dict(kind=kind, a=dc["data"], b=dc["randoms"], axes=axes, n_jk=1),
+                 dict(kind=kind, a=dc["randoms"], b=None, axes=axes, n_jk=1)]
+    config = {"workload": "equatorial s-mu + sigma-pi: 1M clustered galaxies + 5M randoms, weights "
+                          "U[0.5,1.5]; s-mu 40 log bins x 50 mu, sigma-pi 50x50 linear to 50; DD+DR+RR in one call",
+              "n_data": int(c["data"]["ra"].size), "n_randoms": int(c["randoms"]["ra"].size), "scale": args.scale}
+    return dict(jobs=jobs, upload_ms=upload_ms, config=config, traffic_key=None, cpu=None)
+
+
+ANG_AXIS = (0.0, float(1.0 - np.cos(0.175)), 40, 1.6)
+
+
+def workload_invp(args, L, ctx):
+    """Bitwise inverse-p weights: 500k clustered galaxies with 2048-bit masks
+    (32 words, bit probability U[0.3, 0.95] per galaxy) + 2.5M randoms with
+    all-ones masks.  A step is the pipeline of the reference's inverse-p build:
+    angular DD and DR with parent and inverse-p weights (theta < 0.175 rad, 40
+    log bins base 1.6), the up-weights DD/DDinvp and DR/DRinvp, then s-mu DD
+    (inverse-p weights), DR (angular up-weight) and RR."""
+    from two_pcf_amd import synth
+    t0 = time.time()
+    c = synth.inverse_p_catalogs(scale=args.scale)
+    log(f"inverse-p catalogues ({time.time() - t0:.1f} s)")
+    n_runs = c["n_runs"]
+    t_up = time.perf_counter()
+    dc = {t: _eq_upload(L, ctx, c[t], c, masks=True) for t in ("data", "randoms")}
+    upload_ms = 1000.0 * (time.perf_counter() - t_up)
+    D, R = dc["data"], dc["randoms"]
+    ang = [dict(kind=L.ANGULAR, a=D, b=None, axes=[ANG_AXIS], weight_mode=L.W_ANG_IW, n_runs=n_runs),
+           dict(kind=L.ANGULAR, a=D, b=None, axes=[ANG_AXIS], weight_mode=L.W_ANG_INVP, n_runs=n_runs),
+           dict(kind=L.ANGULAR, a=D, b=R, axes=[ANG_AXIS], weight_mode=L.W_ANG_IW, n_runs=n_runs),
+           dict(kind=L.ANGULAR, a=D, b=R, axes=[ANG_AXIS], weight_mode=L.W_ANG_W, n_runs=n_runs)]
+
+    def threed(up_dd, up_dr):
+        return [dict(kind=L.S_MU, a=D, b=None, axes=AXES, weight_mode=L.W_INVP_3D, n_runs=n_runs,
+                     up_bins=up_dd, up_axis=ANG_AXIS),
+                dict(kind=L.S_MU, a=D, b=R, axes=AXES, weight_mode=L.W_ANGUP_3D, n_runs=n_runs,
+                     up_bins=up_dr, up_axis=ANG_AXIS),
+                dict(kind=L.S_MU, a=R, b=None, axes=AXES)]
+
+    def run(method, tu):
+        r1, t1 = ctx.count(ang, method=method, tuning=tu)
+        fin = [L.finish(j.get("b") is None, r["full"], r["jk"])[0] for r, j in zip(r1, ang)]
+        with np.errstate(invalid="ignore", divide="ignore"):
+            up_dd, up_dr = fin[0] / fin[1], fin[2] / fin[3]
+        r2, t2 = ctx.count(threed(up_dd, up_dr), method=method, tuning=tu)
+        t = {k: t1[k] + t2[k] for k in ("kernel_ms", "total_ms", "prep_ms", "n_work_items", "n_rechecked", "n_pairs")}
+        t["angular_kernel_ms"], t["threed_kernel_ms"] = t1["kernel_ms"], t2["kernel_ms"]
+        log(f"  angular {t1['kernel_ms']:.1f} ms ({t1['n_pairs']} pairs), 3-D {t2['kernel_ms']:.1f} ms "
+            f"({t2['n_pairs']} pairs)")
+        return r1 + r2, t
+
+    config = {"workload": "bitwise inverse-p: 500k clustered galaxies with 2048-bit masks + 2.5M randoms; "
+                          "angular DD/DR (parent and inverse-p weights, theta<0.175, 40 log bins) then s-mu "
+                          "DD (inverse-p) / DR (up-weight) / RR, as two calls per step",
+              "n_data": int(c["data"]["ra"].size), "n_randoms": int(c["randoms"]["ra"].size), "scale": args.scale,
+              "n_runs": n_runs}
+    all_jobs = ang + threed(None, None)
+    return dict(jobs=ang, all_jobs=all_jobs, upload_ms=upload_ms, config=config, traffic_key=None, cpu=None,
+                run=run, tunable=False)
+
+
+WORKLOADS = {"jackknife": lambda a, L, c: workload_jackknife(a, L, c),
+             "multi": lambda a, L, c: workload_multi(a, L, c),
+             "equatorial": lambda a, L, c: workload_equatorial(a, L, c),
+             "invp": lambda a, L, c: workload_invp(a, L, c)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -195,9 +284,11 @@ def main():
     ap.add_argument("--slots", type=int, default=0)
     ap.add_argument("--n-jk", type=int, default=0, help="override the number of jackknife regions (1: none)")
     ap.add_argument("--no-autotune", action="store_true", help="use the default tuning instead of the autotuner's")
-    ap.add_argument("--workload", choices=["jackknife", "multi"], default="jackknife",
+    ap.add_argument("--workload", choices=["jackknife", "multi", "equatorial", "invp"], default="jackknife",
                     help="jackknife: the 2M+10M s-mu jackknife batch (default); multi: 64 uneven "
-                         "cartesian mocks, monopole + s-mu of each in one call")
+                         "cartesian mocks, monopole + s-mu of each in one call; equatorial: 1M+5M weighted "
+                         "s-mu + sigma-pi; invp: 500k+2.5M with 2048-bit masks, angular DD/DR then s-mu "
+                         "with inverse-p weights")
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="only run a device copy of this size (counter calibration)")
     args = ap.parse_args()
@@ -212,24 +303,26 @@ def main():
     from two_pcf_amd import lib as L
     ctx = L.Context(0)
     log("device", ctx.info())
-    W = workload_multi(args, L, ctx) if args.workload == "multi" else workload_jackknife(args, L, ctx)
+    W = WORKLOADS[args.workload](args, L, ctx)
     jobs, upload_ms = W["jobs"], W["upload_ms"]
+    W.setdefault("all_jobs", jobs)
     tuning = {}
     if args.subdiv:
         tuning["cell_subdiv"] = args.subdiv
     if args.slots:
         tuning["max_slots"] = args.slots
     tuned = None
-    if not tuning and not args.no_autotune:
+    if not tuning and not args.no_autotune and W.get("tunable", True):
         t0 = time.perf_counter()
         tu, est = ctx.autotune(jobs)
         tuned = dict(choice=tu, estimated_kernel_ms=est, autotune_s=time.perf_counter() - t0)
         log(f"autotune: {tu} (estimated kernel {est:.1f} ms, {tuned['autotune_s']:.1f} s)")
 
+    run = W.get("run") or (lambda method, tu: ctx.count(jobs, method=method, tuning=tu))
     ref = None
     first_call_ms = None
     for i in range(args.warmup):
-        ref, t = ctx.count(jobs, method=L.FP32_RECHECK, tuning=tuning or None)
+        ref, t = run(L.FP32_RECHECK, tuning or None)
         if first_call_ms is None:
             first_call_ms = t["total_ms"]   # gridding + planning + graph capture + one run
         log(f"warmup {i}: kernel {t['kernel_ms']:.1f} ms, total {t['total_ms']:.1f} ms, "
@@ -237,7 +330,7 @@ def main():
     kms = []
     t0 = time.perf_counter()
     for i in range(args.steps):
-        res, t = ctx.count(jobs, method=L.FP32_RECHECK, tuning=tuning or None)
+        res, t = run(L.FP32_RECHECK, tuning or None)
         kms.append(t["kernel_ms"])
         log(f"step {i}: kernel {t['kernel_ms']:.1f} ms, total {t['total_ms']:.1f} ms")
     wall = time.perf_counter() - t0
@@ -245,7 +338,8 @@ def main():
     rechecked = t["n_rechecked"]
     if ref is not None:
         for a, b in zip(ref, res):
-            assert np.array_equal(a["full"], b["full"]) and np.array_equal(a["jk"], b["jk"]), "steps differ"
+            assert a["n_pairs"] == b["n_pairs"], "steps differ"
+            np.testing.assert_allclose(a["full"], b["full"], rtol=1e-12, atol=0, err_msg="steps differ")
     ms_step = 1000.0 * wall / args.steps
     value = pairs / (wall / args.steps)
     kernel_s = float(np.mean(kms)) / 1000.0
@@ -254,15 +348,26 @@ def main():
     if args.plain_steps > 0:
         pk = []
         for i in range(args.plain_steps):
-            pres, pt = ctx.count(jobs, method=L.PLAIN_FP64)
+            pres, pt = run(L.PLAIN_FP64, None)
             pk.append(pt["kernel_ms"])
             log(f"plain fp64 {i}: kernel {pt['kernel_ms']:.1f} ms, total {pt['total_ms']:.1f} ms")
-        same = all(np.array_equal(a["full"], b["full"]) and np.array_equal(a["jk"], b["jk"])
-                   for a, b in zip(pres, res))
+        # counts must be identical; weighted sums (added in another order)
+        # are compared by their largest relative difference
+        same = all(a["n_pairs"] == b["n_pairs"] for a, b in zip(pres, res))
+        rel = 0.0
+        for a, b in zip(pres, res):
+            for k in ("full", "jk"):
+                fin = np.isfinite(a[k]) & np.isfinite(b[k])
+                same = same and bool(np.array_equal(np.isfinite(a[k]), np.isfinite(b[k])))
+                d = np.abs(a[k] - b[k])
+                m = fin & (np.abs(a[k]) > 0)
+                if m.any():
+                    rel = max(rel, float(np.max(d[m] / np.abs(a[k][m]))))
+                same = same and bool(np.all(d[fin & ~m] == 0))
         plain = dict(kernel_ms=float(np.mean(pk)), pairs_per_s=pairs / (float(np.mean(pk)) / 1000.0),
-                     counts_identical=bool(same))
+                     counts_identical=bool(same), max_rel_diff_sums=rel)
 
-    flop = sum(r["n_pairs"] * FLOP_BY_KIND[j["kind"]] for r, j in zip(res, jobs))
+    flop = sum(r["n_pairs"] * FLOP_BY_KIND.get(j["kind"], 0) for r, j in zip(res, W["all_jobs"]))
     achieved = flop / kernel_s / 1e12
     tu_used = tuned["choice"] if tuned else dict(cell_subdiv=tuning.get("cell_subdiv", 2),
                                                    max_slots=tuning.get("max_slots", 8), i_chunk=256)

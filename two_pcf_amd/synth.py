@@ -147,3 +147,51 @@ def uneven_mocks(n_mocks=64, n_min=20_000, n_max=2_000_000, density=3e-4, seed=2
         r = uniform_cube(5 * int(n), side, np.random.Generator(np.random.PCG64(seed + 17 * (k + 1) + 1)))
         out.append(dict(data=d, randoms=r, side=side))
     return out
+
+
+def random_masks(n, n_words, p, seed):
+    """n x n_words uint64 masks, bit k of object i set with probability p[i]
+    (built one 64-bit word at a time to bound memory)."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    out = np.empty((n, n_words), dtype=np.uint64)
+    for k in range(n_words):
+        bits = rng.random((n, 64)) < p[:, None]
+        out[:, k] = np.packbits(bits.astype(np.uint8), axis=1, bitorder="little").view(np.uint64)[:, 0]
+    return out
+
+
+def popcount_rows(masks):
+    """Number of set bits per row of a uint64 mask array."""
+    m = masks.copy()
+    m = m - ((m >> np.uint64(1)) & np.uint64(0x5555555555555555))
+    m = (m & np.uint64(0x3333333333333333)) + ((m >> np.uint64(2)) & np.uint64(0x3333333333333333))
+    m = (m + (m >> np.uint64(4))) & np.uint64(0x0F0F0F0F0F0F0F0F)
+    m = (m * np.uint64(0x0101010101010101)) >> np.uint64(56)
+    return m.sum(axis=1).astype(np.int64)
+
+
+def inverse_p_catalogs(scale=1.0, n_words=32, seed=20260104):
+    """500k clustered equatorial galaxies (10k parents x Poisson(50), 5 Mpc/h,
+    ra 150-250, dec 0-60, z 0.2-0.6) with n_words x 64-bit masks, each bit set
+    with a per-galaxy probability p ~ U[0.3, 0.95]; 2.5M randoms from the same
+    footprint and n(z) with all-ones masks; weights U[0.5, 1.5].  Returns the
+    input weights `iw` and the weights after the inverse-probability rule at
+    read time (io.cpp:578-582) as `w`."""
+    n_runs = 64 * n_words
+    n_par = max(1, int(round(10000 * scale)))
+    ra, dec, z = neyman_scott_equatorial(n_par, 50.0, 5.0, FOOT_RA, FOOT_DEC, FOOT_Z, seed=seed)
+    n_r = max(1, int(round(2_500_000 * scale)))
+    rra, rdec, rz = randoms_equatorial(n_r, FOOT_RA, FOOT_DEC, z, seed=seed + 1)
+    rng = np.random.Generator(np.random.PCG64(seed + 2))
+    p = rng.uniform(0.3, 0.95, ra.size)
+    dmask = random_masks(ra.size, n_words, p, seed + 3)
+    rmask = np.full((rra.size, n_words), np.uint64(0xFFFFFFFFFFFFFFFF), dtype=np.uint64)
+    out = {}
+    for tag, (a, b, c, m) in (("data", (ra, dec, z, dmask)), ("randoms", (rra, rdec, rz, rmask))):
+        iw = rng.uniform(0.5, 1.5, a.size).astype(np.float32)
+        tot = popcount_rows(m)
+        tot[tot == 0] = 1
+        w = np.where(tot != n_runs, (iw.astype(np.float64) * (n_runs / tot)).astype(np.float32), iw)
+        out[tag] = dict(ra=a, dec=b, z=c, w=w, iw=iw, masks=m, jk=np.zeros(a.size, np.int32))
+    out.update(n_runs=n_runs, om=0.25, h=1.0, z_min=0.0, z_max=1.0)
+    return out
