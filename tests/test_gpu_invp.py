@@ -89,3 +89,34 @@ def test_invp_pipeline(oracle_lib):
                 assert r["n_pairs"] == acc
                 np.testing.assert_allclose(r["full"], full, rtol=1e-12, atol=0)
                 np.testing.assert_allclose(r["jk"], sub, rtol=1e-12, atol=0)
+
+
+@pytest.mark.parametrize("axis", [(0.0, None, 12, 1.6), (0.0, None, 10, 1.0)])
+def test_angular_counts_all_subdivisions(oracle_lib, axis):
+    """Unweighted angular DD/DR with 3 regions, duplicate objects included
+    (1 - cos = 0), on a log and a linear axis: counts equal the oracle's at
+    every cell subdivision of the batched FP32 method and in the plain method."""
+    from two_pcf_amd import lib as L
+    o = oracle_lib
+    D = make(L, 1500, 51, ra_l=(150.0, 156.0), dec_l=(0.0, 6.0))
+    R = make(L, 3000, 52, ones=True, ra_l=(150.0, 156.0), dec_l=(0.0, 6.0))
+    for cat in (D, R):
+        cat["jk"] = (np.arange(cat["x"].size) % 3).astype(np.int32)
+        cat["w"] = np.ones(cat["x"].size, np.float32)
+        for k in ("x", "y", "z", "nx", "ny", "nz"):
+            cat[k][:20] = cat[k][20]   # 21 identical objects
+    ax = (axis[0], float(1.0 - np.cos(0.02)), axis[2], axis[3])
+    oax = o.axis(*ax)
+    oc = {k: o.Catalog(v["x"], v["y"], v["z"], v["nx"], v["ny"], v["nz"], v["w"], v["jk"]) for k, v in (("d", D), ("r", R))}
+    want = [o.count(3, oc["d"], None, oax, None, 3), o.count(3, oc["d"], oc["r"], oax, None, 3)]
+    with L.Context(0) as ctx:
+        dc = {k: ctx.upload(v["x"], v["y"], v["z"], v["nx"], v["ny"], v["nz"], v["w"], v["jk"]) for k, v in (("d", D), ("r", R))}
+        jobs = [dict(kind=L.ANGULAR, a=dc["d"], b=None, axes=[ax], n_jk=3),
+                dict(kind=L.ANGULAR, a=dc["d"], b=dc["r"], axes=[ax], n_jk=3)]
+        for method, tu in ((0, None), (1, {"cell_subdiv": 1}), (1, {"cell_subdiv": 2}), (1, {"cell_subdiv": 3})):
+            print(f"[gpu] angular counts method={method} tuning={tu}", flush=True)
+            res, _ = ctx.count(jobs, method=method, tuning=tu)
+            for r, (full, sub, acc) in zip(res, want):
+                np.testing.assert_array_equal(r["full"], full)
+                np.testing.assert_array_equal(r["jk"], sub)
+                assert r["n_pairs"] == acc

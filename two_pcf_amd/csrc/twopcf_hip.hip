@@ -499,7 +499,11 @@ __device__ void process_item_fast(const JobDev &J, const WorkItem &W, const int4
 			for (int s = 0; s < n_slots; ++s)
 				if (sreg[s] == r) code = s;
 		}
-		cia[tid] = make_float4((float)(J.A.x[g] - c0), (float)(J.A.y[g] - c1), (float)(J.A.z[g] - c2), J.A.w[g]);
+		// angular counts work on the unit vectors (1 - n1.n2 = |n1 - n2|^2 / 2)
+		const double *px = KIND == K_ANGULAR ? J.A.nx : J.A.x;
+		const double *py = KIND == K_ANGULAR ? J.A.ny : J.A.y;
+		const double *pz = KIND == K_ANGULAR ? J.A.nz : J.A.z;
+		cia[tid] = make_float4((float)(px[g] - c0), (float)(py[g] - c1), (float)(pz[g] - c2), J.A.w[g]);
 		cib[tid] = make_float4((float)J.A.nx[g], (float)J.A.ny[g], (float)J.A.nz[g], __int_as_float(code));
 	}
 	__syncthreads();
@@ -560,8 +564,9 @@ __device__ void process_item_fast(const JobDev &J, const WorkItem &W, const int4
 				const double xj[3] = {J.B.x[g2], J.B.y[g2], J.B.z[g2]};
 				const double nj[3] = {J.B.nx[g2], J.B.ny[g2], J.B.nz[g2]};
 				const float *t_s2 = tab + J.t_s2_off;
+				const float *t1 = tab + J.t1_off;
 				int bin;
-				if (exact_bin<KIND>(xi, ni, xj, nj, J.min_r_sq, J.max_r_sq, t_s2, nullptr, nullptr, J.ax1, J.ax2, bin)) {
+				if (exact_bin<KIND>(xi, ni, xj, nj, J.min_r_sq, J.max_r_sq, t_s2, t1, nullptr, J.ax1, J.ax2, bin)) {
 					++n_acc;
 					int sj = 0;
 					if (jk) {
@@ -594,9 +599,10 @@ __device__ void process_item_fast(const JobDev &J, const WorkItem &W, const int4
 			const float4 ia = cia[l2], ib = cib[l2], a = tfa[q], b = tfb[q];
 			const float dx = ia.x - a.x, dy = ia.y - a.y, dz = ia.z - a.z;
 			const float s2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
-			// s bin: closed-form estimate, checked against the exact
-			// thresholds, proved by the error margins
-			const float sv = __builtin_amdgcn_sqrtf(s2) - s_min;
+			// bin: closed-form estimate, checked against the exact thresholds,
+			// proved by the error margins; the axis variable is s, or
+			// 1 - cos = chord^2 / 2 for angular counts
+			const float sv = (KIND == K_ANGULAR ? 0.5f * s2 : __builtin_amdgcn_sqrtf(s2)) - s_min;
 			int k = s_log ? (int)(s_ilb_ln2 * __builtin_amdgcn_logf(fmaf(s_ils, sv, 1.0f))) : (int)(sv * s_inv_step);
 			k = min(max(k, 0), n1 - 1);
 			const float4 q0 = tq[k], q1 = tq[k + 1];
@@ -669,7 +675,10 @@ __device__ void process_item_fast(const JobDev &J, const WorkItem &W, const int4
 					for (int s = 0; s < n_slots; ++s)
 						if (sreg[s] == rj) sl = s;
 				}
-				tfa[tid] = make_float4((float)(J.B.x[gj] - c0), (float)(J.B.y[gj] - c1), (float)(J.B.z[gj] - c2),
+				const double *px = KIND == K_ANGULAR ? J.B.nx : J.B.x;
+				const double *py = KIND == K_ANGULAR ? J.B.ny : J.B.y;
+				const double *pz = KIND == K_ANGULAR ? J.B.nz : J.B.z;
+				tfa[tid] = make_float4((float)(px[gj] - c0), (float)(py[gj] - c1), (float)(pz[gj] - c2),
 				                       __int_as_float(gj));
 				tfb[tid] = make_float4((float)J.B.nx[gj], (float)J.B.ny[gj], (float)J.B.nz[gj], J.B.w[gj]);
 				tslot[tid] = sl;
@@ -1357,7 +1366,7 @@ int build_plan(twopcf_ctx *ctx, const twopcf_job *jobs, int n_jobs, int method, 
 			k.append((const char *)H.j->ax, sizeof H.j->ax);
 			if (std::find(seen.begin(), seen.end(), k) != seen.end()) continue;
 			seen.push_back(k);
-			tab_floats += 7 * (H.ax1.n + 1) + (H.ax2.n + 1) + 32;
+			tab_floats += 9 * (H.ax1.n + 1) + (H.ax2.n + 1) + 32;
 		}
 	}
 	const int lds_fixed = (int)(((std::max(tile_bytes(true), tile_bytes(false)) + 15) & ~15) + ((tab_floats * 4 + 15) & ~15) +
@@ -1418,6 +1427,44 @@ int build_plan(twopcf_ctx *ctx, const twopcf_job *jobs, int n_jobs, int method, 
 		};
 		if (H.j->kind == TWOPCF_ANGULAR) {
 			D.t1_off = push(H.t1);
+			if (fast) {
+				// chord^2 = 2 (1 - cos): thresholds 2 t (exact), margins from
+				// the FP32 chord^2 bound plus the reference's own roundings of
+				// 1 - n1.n2 in double (about 1e-15 each for the products and
+				// subtractions and for the unit vectors' norm error; 1e-14 is
+				// allowed in the chord^2 domain) and to float (2^-24 relative,
+				// 2^-21 allowed)
+				const double R = 1.01 * (subdiv + 1) * H.g.h;
+				const size_t nt = H.t1.size();
+				std::vector<float> T2(nt), A_(nt), B_(nt);
+				for (size_t k = 0; k < nt; ++k) {
+					const double T = 2.0 * (double)H.t1[k];
+					T2[k] = (float)T;
+					if (!std::isfinite(T)) {
+						A_[k] = B_[k] = (float)T;
+						continue;
+					}
+					const double E = 1.01 * e_s2(std::max(T, 0.0) * 1.001, R) + 1e-14 + std::ldexp(std::fabs(T), -21) +
+					                 1e-30;
+					A_[k] = round_up(T + E);
+					B_[k] = round_dn(T - E);
+				}
+				D.t_s2_off = push(T2);
+				D.tA_off = push(A_);
+				D.tB_off = push(B_);
+				std::vector<float> Q(4 * nt, 0.0f);
+				for (size_t k = 0; k < nt; ++k) {
+					Q[4 * k] = T2[k];
+					Q[4 * k + 1] = A_[k];
+					Q[4 * k + 2] = B_[k];
+				}
+				D.tq_off = push(Q);
+				D.s_min = H.ax1.min;
+				D.s_log = H.ax1.log ? 1 : 0;
+				D.s_ilb_ln2 = (float)((double)H.ax1.ilb * std::log(2.0));
+				D.s_ils = H.ax1.ils;
+				D.s_inv_step = H.ax1.inv_step;
+			}
 		} else if (H.j->kind == TWOPCF_SIGMA_PI) {
 			D.t1_off = push(H.t1);
 			D.t2_off = push(H.t2);
@@ -1459,8 +1506,7 @@ int build_plan(twopcf_ctx *ctx, const twopcf_job *jobs, int n_jobs, int method, 
 		D.row_off = rr.first;
 		D.n_rows = rr.second;
 		// small histograms of the fp32 path get one copy per lane
-		D.f_shift = (fast && (H.j->kind == TWOPCF_MONOPOLE || H.j->kind == TWOPCF_S_MU) &&
-		             (size_t)H.nb * WAVE * (H.weighted ? 8 : 4) <= 16384)
+		D.f_shift = (fast && H.j->kind != TWOPCF_SIGMA_PI && (size_t)H.nb * WAVE * (H.weighted ? 8 : 4) <= 32768)
 		                ? 6 : 0;
 		P.f_elems = H.nb << D.f_shift;
 		D.wmode = H.j->weight_mode;
@@ -1700,7 +1746,7 @@ int build_plan(twopcf_ctx *ctx, const twopcf_job *jobs, int n_jobs, int method, 
 	};
 	auto launch = [&](int kind, bool weighted, const WorkItem *it, int n_it, int nb, int slots, unsigned *queue,
 	                  int grid, int f_elems) -> int {
-		const bool fast_kernel = fast && (kind == K_MONO || kind == K_S_MU);
+		const bool fast_kernel = fast && (kind == K_MONO || kind == K_S_MU || kind == K_ANGULAR);
 		Lds L = layout(fast_kernel, weighted, nb, slots, f_elems);
 		if ((size_t)L.total > ctx->lds_max)
 			return fail(ctx, TWOPCF_E_LIMIT, "histogram does not fit in LDS (" + std::to_string(L.total) + " B)");
@@ -1711,7 +1757,7 @@ int build_plan(twopcf_ctx *ctx, const twopcf_job *jobs, int n_jobs, int method, 
 			if (kind == K_MONO) { if (weighted) PICK(K_MONO, true, true); else PICK(K_MONO, true, false); }
 			else if (kind == K_S_MU) { if (weighted) PICK(K_S_MU, true, true); else PICK(K_S_MU, true, false); }
 			else if (kind == K_SIGMA_PI) { if (weighted) PICK(K_SIGMA_PI, false, true); else PICK(K_SIGMA_PI, false, false); }
-			else { if (weighted) PICK(K_ANGULAR, false, true); else PICK(K_ANGULAR, false, false); }
+			else { if (weighted) PICK(K_ANGULAR, true, true); else PICK(K_ANGULAR, true, false); }
 		} else {
 			if (kind == K_MONO) { if (weighted) PICK(K_MONO, false, true); else PICK(K_MONO, false, false); }
 			else if (kind == K_S_MU) { if (weighted) PICK(K_S_MU, false, true); else PICK(K_S_MU, false, false); }
@@ -1769,7 +1815,7 @@ int build_plan(twopcf_ctx *ctx, const twopcf_job *jobs, int n_jobs, int method, 
 			int slots = 0;
 			for (size_t k = std::get<2>(sp); k < std::get<2>(sp) + std::get<3>(sp); ++k)
 				slots = std::max(slots, flat[k].n_slots);
-			Lds L = layout(kind == K_MONO || kind == K_S_MU, weighted, nb, slots, f_elems);
+			Lds L = layout(kind == K_MONO || kind == K_S_MU || kind == K_ANGULAR, weighted, nb, slots, f_elems);
 			const int per_cu = std::max(1, std::min(8, (int)(ctx->lds_max / std::max(L.total, 1))));
 			const int grid = std::min<long long>((long long)ctx->n_cu * per_cu, (long long)std::get<3>(sp));
 			if ((rc = launch(kind, weighted, d_items + std::get<2>(sp), (int)std::get<3>(sp), nb, slots, d_queue + qi,
