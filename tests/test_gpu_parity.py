@@ -184,7 +184,7 @@ def test_grid_independence(ctx, subdiv):
     case = load_case("eq_clustered")
     print(f"[gpu] grid independence subdiv={subdiv}", flush=True)
     cats = lib_cats(L, ctx, case.inputs, case.coord)
-    for corr in ("monopole", "s_mu"):
+    for corr in ("monopole", "sigma_pi", "s_mu"):
         if corr not in case.corrs():
             continue
         axes = case.axes(corr)

@@ -136,6 +136,30 @@ bool interp_thresholds(const AxisF &a, std::vector<float> &t) {
 	return true;
 }
 
+bool sqrt_thresholds(const std::vector<float> &t, std::vector<float> &X) {
+	const uint32_t lo = fkey(0.0f);
+	const uint32_t hi = fkey(std::numeric_limits<float>::max());
+	X.assign(t.size(), 0.0f);
+	for (size_t k = 0; k < t.size(); ++k) {
+		const float tk = t[k];
+		if (!(tk > 0.0f)) {
+			X[k] = -0.0f;   // sqrtf(x) >= tk for every x >= -0, NaN below
+			continue;
+		}
+		auto pred = [&](float x) { return sqrtf(x) >= tk; };
+		uint32_t key = first_true(lo, hi, pred);
+		if (key > hi) {
+			X[k] = std::numeric_limits<float>::infinity();
+			continue;
+		}
+		if (!check_switch(key, lo, hi, pred)) return false;
+		X[k] = keyf(key);
+	}
+	for (size_t k = 1; k < X.size(); ++k)
+		if (X[k] < X[k - 1]) return false;
+	return true;
+}
+
 bool s2_thresholds(int kind, const AxisF &a1, const AxisF &a2, std::vector<float> &t) {
 	float mn, mx;
 	range_sq(kind, a1, a2, &mn, &mx);

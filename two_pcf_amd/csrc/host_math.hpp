@@ -37,6 +37,12 @@ bool value_thresholds(const AxisF &a, std::vector<float> &t);
 // with bn >= k and t[n] the first with bn > n - 1.
 bool interp_thresholds(const AxisF &a, std::vector<float> &t);
 
+// sqrtf-domain thresholds: X[k] = the smallest float x with sqrtf(x) >= t[k]
+// (x below every X[k] includes the negative floats, whose sqrtf is NaN);
+// -0 for t[k] <= 0, +inf for t[k] = +inf.  Returns false if sqrtf is not
+// monotone near a threshold.
+bool sqrt_thresholds(const std::vector<float> &t, std::vector<float> &X);
+
 // s^2-domain thresholds of a monopole / s-mu s axis, folding in the range
 // test and sqrtf (correlators.cpp:315-317, 338).  Same convention.
 bool s2_thresholds(int kind, const AxisF &a1, const AxisF &a2, std::vector<float> &t);

@@ -81,6 +81,11 @@ struct JobDev {
 	int s_log;
 	float mu_c1, mu_c2;                              // fast mu margin: delta = c1 + c2 * rsq(s2)
 	int f_shift;                                     // fp32 path: full histogram in 2^f_shift lane copies
+	int x1_off;                                      // sigma-pi fp32: sqrtf-domain thresholds of the sigma axis
+	float p_min, p_ilb_ln2, p_ils, p_inv_step;       // sigma-pi fp32: pi-axis index estimate
+	int p_log;
+	float e_c1, e_c2;                                // sigma-pi fp32: E(S) = e_c1 S + e_c2 sqrt(S)
+	float pi_c1, pi_c2;                              // sigma-pi fp32: |pi_fp32 - pi_ref| <= pi_c1 s + pi_c2
 	int row_off, n_rows;                             // stencil rows in the row pool
 	int wmode, n_runs, n_words;                      // pair-weight rule (TWOPCF_W_*)
 	int up_n;                                        // angular up-weight table
@@ -223,6 +228,28 @@ __device__ float pair_weight(const JobDev &J, int gi, int gj, const double *ni, 
 	case TWOPCF_W_PLAIN:
 	default:
 		return J.A.w[gi] * J.B.w[gj];
+	}
+}
+
+// The rules of pair_weight that need more than the two staged weights, given
+// their float product wp (w1 * w2, or iw1 * iw2 for the input-weight rules):
+// the same operations in the same order.
+__device__ float pair_weight_staged(const JobDev &J, int gi, int gj, const double *ni, const double *nj, float wp) {
+	switch (J.wmode) {
+	case TWOPCF_W_INVP_3D: {
+		const float pc = popc_weight(J, gi, gj);
+		double t = interp_up(J, ref_one_minus_cos(ni, nj)) * (double)J.n_runs;
+		t = t / (double)pc;
+		return wp * (float)t;
+	}
+	case TWOPCF_W_ANGUP_3D:
+		return (float)((double)wp * interp_up(J, ref_one_minus_cos(ni, nj)));
+	case TWOPCF_W_ANG_INVP: {
+		const float pc = popc_weight(J, gi, gj);
+		return wp * ((float)J.n_runs / pc);
+	}
+	default:
+		return wp;
 	}
 }
 
@@ -468,6 +495,11 @@ __device__ void process_item_fast(const JobDev &J, const WorkItem &W, const int4
 	const bool is_auto = J.is_auto != 0;
 	const bool jk = J.n_jk > 1;
 	const int n_slots = W.n_slots;
+	// weights staged with the objects: the input weights for the rules built
+	// on them, else the weights
+	const int wmode = J.wmode;
+	const bool stage_iw = wmode == TWOPCF_W_ANG_IW || wmode == TWOPCF_W_ANG_INVP || wmode == TWOPCF_W_INVP_3D;
+	const bool needs_more = wmode == TWOPCF_W_ANG_INVP || wmode == TWOPCF_W_INVP_3D || wmode == TWOPCF_W_ANGUP_3D;
 
 	float4 *tfa = (float4 *)(smem + L.tile);   // (o0, o1, o2, gj)
 	float4 *tfb = tfa + TILE;                  // (n0, n1, n2, w)
@@ -503,7 +535,8 @@ __device__ void process_item_fast(const JobDev &J, const WorkItem &W, const int4
 		const double *px = KIND == K_ANGULAR ? J.A.nx : J.A.x;
 		const double *py = KIND == K_ANGULAR ? J.A.ny : J.A.y;
 		const double *pz = KIND == K_ANGULAR ? J.A.nz : J.A.z;
-		cia[tid] = make_float4((float)(px[g] - c0), (float)(py[g] - c1), (float)(pz[g] - c2), J.A.w[g]);
+		cia[tid] = make_float4((float)(px[g] - c0), (float)(py[g] - c1), (float)(pz[g] - c2),
+		                       stage_iw ? J.A.iw[g] : J.A.w[g]);
 		cib[tid] = make_float4((float)J.A.nx[g], (float)J.A.ny[g], (float)J.A.nz[g], __int_as_float(code));
 	}
 	__syncthreads();
@@ -540,14 +573,16 @@ __device__ void process_item_fast(const JobDev &J, const WorkItem &W, const int4
 			if (sj != si) add_region(sj, bin, w);
 		}
 	};
-	auto weight = [&](int l2, int g2, float w) {
-		if (WEIGHTED && J.wmode != TWOPCF_W_PLAIN) {
+	// wp: the float product of the two staged weights (w, or iw for the
+	// rules built on the input weights); rules that need more read it here
+	auto weight = [&](int l2, int g2, float wp) {
+		if (WEIGHTED && needs_more) {
 			const int g1 = W.i_begin + l2;
 			const double ni[3] = {J.A.nx[g1], J.A.ny[g1], J.A.nz[g1]};
 			const double nj[3] = {J.B.nx[g2], J.B.ny[g2], J.B.nz[g2]};
-			w = pair_weight(J, g1, g2, ni, nj);
+			wp = pair_weight_staged(J, g1, g2, ni, nj, wp);
 		}
-		return w;
+		return wp;
 	};
 
 	// recheck queue: the exact pipeline on up to RQ pairs, 64 lanes at a time
@@ -565,8 +600,9 @@ __device__ void process_item_fast(const JobDev &J, const WorkItem &W, const int4
 				const double nj[3] = {J.B.nx[g2], J.B.ny[g2], J.B.nz[g2]};
 				const float *t_s2 = tab + J.t_s2_off;
 				const float *t1 = tab + J.t1_off;
+				const float *t2 = tab + J.t2_off;
 				int bin;
-				if (exact_bin<KIND>(xi, ni, xj, nj, J.min_r_sq, J.max_r_sq, t_s2, t1, nullptr, J.ax1, J.ax2, bin)) {
+				if (exact_bin<KIND>(xi, ni, xj, nj, J.min_r_sq, J.max_r_sq, t_s2, t1, t2, J.ax1, J.ax2, bin)) {
 					++n_acc;
 					int sj = 0;
 					if (jk) {
@@ -575,7 +611,7 @@ __device__ void process_item_fast(const JobDev &J, const WorkItem &W, const int4
 						for (int s = 0; s < n_slots; ++s)
 							if (sreg[s] == rj) sj = s;
 					}
-					const float w = weight(e.x, g2, cia[e.x].w * J.B.w[g2]);
+					const float w = weight(e.x, g2, cia[e.x].w * (stage_iw ? J.B.iw[g2] : J.B.w[g2]));
 					add(bin, w, __float_as_int(cib[e.x].w), sj);
 				}
 			}
@@ -587,6 +623,8 @@ __device__ void process_item_fast(const JobDev &J, const WorkItem &W, const int4
 	const bool s_log = J.s_log != 0;
 	const float s_min = J.s_min, s_ilb_ln2 = J.s_ilb_ln2, s_ils = J.s_ils, s_inv_step = J.s_inv_step;
 	const float mu_c1 = J.mu_c1, mu_c2 = J.mu_c2;
+	const float *x1t = tab + J.x1_off, *t2t = tab + J.t2_off;
+	const float rA0 = tab[J.tA_off], rBn = tab[J.tB_off + n1];
 	// bin n queued pairs (lanes >= n idle)
 	auto consume = [&](unsigned head, unsigned n) {
 		__builtin_amdgcn_wave_barrier();
@@ -599,6 +637,37 @@ __device__ void process_item_fast(const JobDev &J, const WorkItem &W, const int4
 			const float4 ia = cia[l2], ib = cib[l2], a = tfa[q], b = tfb[q];
 			const float dx = ia.x - a.x, dy = ia.y - a.y, dz = ia.z - a.z;
 			const float s2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
+			bool in_range = true;
+			if constexpr (KIND == K_SIGMA_PI) {
+				// range test on s^2, sigma in the x = s^2 - pi^2 domain
+				// against sqrtf-domain thresholds, |pi| against its value
+				// thresholds; margins per pair from the bounds on s^2 and pi
+				const float p = 0.5f * fmaf(dz, ib.z + b.z, fmaf(dy, ib.y + b.y, dx * (ib.x + b.x)));
+				const float ap = fabsf(p);
+				const float s = __builtin_amdgcn_sqrtf(s2);
+				const float dpi = fmaf(J.pi_c1, s, J.pi_c2);
+				const float x = fmaf(-p, p, s2);
+				const float ex = fmaf(J.e_c1, s2, J.e_c2 * s) + dpi * fmaf(2.0f, ap, dpi) + 3.6e-7f * fmaf(p, p, s2);
+				const bool r_in = s2 >= rA0 && s2 < rBn;   // min_r_sq < s2_ref < max_r_sq
+				const float sg = __builtin_amdgcn_sqrtf(fmaxf(x, 0.0f)) - s_min;
+				int k1 = s_log ? (int)(s_ilb_ln2 * __builtin_amdgcn_logf(fmaf(s_ils, sg, 1.0f))) : (int)(sg * s_inv_step);
+				k1 = min(max(k1, 0), n1 - 1);
+				const bool sure1 = x >= x1t[k1] + ex && x < x1t[k1 + 1] - ex;
+				const bool out1 = x < x1t[0] - ex || x >= x1t[n1] + ex;
+				const float pv = ap - J.p_min;
+				int k2 = J.p_log ? (int)(J.p_ilb_ln2 * __builtin_amdgcn_logf(fmaf(J.p_ils, pv, 1.0f)))
+				                 : (int)(pv * J.p_inv_step);
+				k2 = min(max(k2, 0), n2 - 1);
+				const bool sure2 = ap >= t2t[k2] + dpi && ap < t2t[k2 + 1] - dpi;
+				const bool out2 = ap < t2t[0] - dpi || ap >= t2t[n2] + dpi;
+				if (r_in && (out1 || out2)) {
+					sure = true;   // in range but certainly outside the histogram
+					in_range = false;
+				} else {
+					sure = r_in && sure1 && sure2;
+				}
+				bin = k1 * n2 + k2;
+			} else {
 			// bin: closed-form estimate, checked against the exact thresholds,
 			// proved by the error margins; the axis variable is s, or
 			// 1 - cos = chord^2 / 2 for angular counts
@@ -620,7 +689,8 @@ __device__ void process_item_fast(const JobDev &J, const WorkItem &W, const int4
 			} else {
 				bin = k;
 			}
-			if (sure) {
+			}
+			if (sure && in_range) {
 				++n_acc;
 				add(bin, weight(l2, __float_as_int(a.w), ia.w * b.w), __float_as_int(ib.w), tslot[q]);
 			}
@@ -680,7 +750,8 @@ __device__ void process_item_fast(const JobDev &J, const WorkItem &W, const int4
 				const double *pz = KIND == K_ANGULAR ? J.B.nz : J.B.z;
 				tfa[tid] = make_float4((float)(px[gj] - c0), (float)(py[gj] - c1), (float)(pz[gj] - c2),
 				                       __int_as_float(gj));
-				tfb[tid] = make_float4((float)J.B.nx[gj], (float)J.B.ny[gj], (float)J.B.nz[gj], J.B.w[gj]);
+				tfb[tid] = make_float4((float)J.B.nx[gj], (float)J.B.ny[gj], (float)J.B.nz[gj],
+				                       stage_iw ? J.B.iw[gj] : J.B.w[gj]);
 				tslot[tid] = sl;
 			}
 			__syncthreads();
@@ -1468,6 +1539,42 @@ int build_plan(twopcf_ctx *ctx, const twopcf_job *jobs, int n_jobs, int method, 
 		} else if (H.j->kind == TWOPCF_SIGMA_PI) {
 			D.t1_off = push(H.t1);
 			D.t2_off = push(H.t2);
+			if (fast) {
+				// sigma decided on x = s^2 - pi^2 against the smallest x whose
+				// sqrtf reaches each sigma threshold; range limits with the
+				// s^2 margins (A/B at [0] and [n1]); per-pair margins from
+				// E(S) and |dpi| <= 2^-23 (20 s + 8 R) (generous for the
+				// offsets', the normals' and both pipelines' roundings)
+				std::vector<float> X1;
+				if (!sqrt_thresholds(H.t1, X1)) return fail(ctx, TWOPCF_E_AXIS, "sigma axis not monotone under sqrtf");
+				D.x1_off = push(X1);
+				const double R = 1.01 * (subdiv + 1) * H.g.h;
+				const int n1 = H.ax1.n;
+				std::vector<float> A_(n1 + 1, 0.0f), B_(n1 + 1, 0.0f);
+				const double mn = H.min_r_sq, mx = H.max_r_sq;
+				const double Emn = 1.01 * e_s2(std::max(mn, 0.0) * 1.001, R) + 1e-30;
+				const double Emx = 1.01 * e_s2(mx * 1.001, R) + 1e-30;
+				B_[0] = round_dn(mn - Emn);
+				A_[0] = round_up(mn + Emn);
+				A_[n1] = round_up(mx + Emx);
+				B_[n1] = round_dn(mx - Emx);
+				D.tA_off = push(A_);
+				D.tB_off = push(B_);
+				D.s_min = H.ax1.min;
+				D.s_log = H.ax1.log ? 1 : 0;
+				D.s_ilb_ln2 = (float)((double)H.ax1.ilb * std::log(2.0));
+				D.s_ils = H.ax1.ils;
+				D.s_inv_step = H.ax1.inv_step;
+				D.p_min = H.ax2.min;
+				D.p_log = H.ax2.log ? 1 : 0;
+				D.p_ilb_ln2 = (float)((double)H.ax2.ilb * std::log(2.0));
+				D.p_ils = H.ax2.ils;
+				D.p_inv_step = H.ax2.inv_step;
+				D.e_c1 = (float)(1.02 * std::ldexp(11.0, -23));
+				D.e_c2 = (float)(1.02 * std::ldexp(4.0 * R, -23));
+				D.pi_c1 = (float)(1.02 * std::ldexp(20.0, -23));
+				D.pi_c2 = (float)(1.02 * std::ldexp(8.0 * R, -23));
+			}
 		} else {
 			D.t_s2_off = push(H.t_s2);
 			// fast-path margins around each s^2 threshold
@@ -1506,7 +1613,7 @@ int build_plan(twopcf_ctx *ctx, const twopcf_job *jobs, int n_jobs, int method, 
 		D.row_off = rr.first;
 		D.n_rows = rr.second;
 		// small histograms of the fp32 path get one copy per lane
-		D.f_shift = (fast && H.j->kind != TWOPCF_SIGMA_PI && (size_t)H.nb * WAVE * (H.weighted ? 8 : 4) <= 32768)
+		D.f_shift = (fast && (size_t)H.nb * WAVE * (H.weighted ? 8 : 4) <= 32768)
 		                ? 6 : 0;
 		P.f_elems = H.nb << D.f_shift;
 		D.wmode = H.j->weight_mode;
@@ -1746,7 +1853,7 @@ int build_plan(twopcf_ctx *ctx, const twopcf_job *jobs, int n_jobs, int method, 
 	};
 	auto launch = [&](int kind, bool weighted, const WorkItem *it, int n_it, int nb, int slots, unsigned *queue,
 	                  int grid, int f_elems) -> int {
-		const bool fast_kernel = fast && (kind == K_MONO || kind == K_S_MU || kind == K_ANGULAR);
+		const bool fast_kernel = fast;
 		Lds L = layout(fast_kernel, weighted, nb, slots, f_elems);
 		if ((size_t)L.total > ctx->lds_max)
 			return fail(ctx, TWOPCF_E_LIMIT, "histogram does not fit in LDS (" + std::to_string(L.total) + " B)");
@@ -1756,7 +1863,7 @@ int build_plan(twopcf_ctx *ctx, const twopcf_job *jobs, int n_jobs, int method, 
 		if (fast) {
 			if (kind == K_MONO) { if (weighted) PICK(K_MONO, true, true); else PICK(K_MONO, true, false); }
 			else if (kind == K_S_MU) { if (weighted) PICK(K_S_MU, true, true); else PICK(K_S_MU, true, false); }
-			else if (kind == K_SIGMA_PI) { if (weighted) PICK(K_SIGMA_PI, false, true); else PICK(K_SIGMA_PI, false, false); }
+			else if (kind == K_SIGMA_PI) { if (weighted) PICK(K_SIGMA_PI, true, true); else PICK(K_SIGMA_PI, true, false); }
 			else { if (weighted) PICK(K_ANGULAR, true, true); else PICK(K_ANGULAR, true, false); }
 		} else {
 			if (kind == K_MONO) { if (weighted) PICK(K_MONO, false, true); else PICK(K_MONO, false, false); }
@@ -1815,7 +1922,7 @@ int build_plan(twopcf_ctx *ctx, const twopcf_job *jobs, int n_jobs, int method, 
 			int slots = 0;
 			for (size_t k = std::get<2>(sp); k < std::get<2>(sp) + std::get<3>(sp); ++k)
 				slots = std::max(slots, flat[k].n_slots);
-			Lds L = layout(kind == K_MONO || kind == K_S_MU || kind == K_ANGULAR, weighted, nb, slots, f_elems);
+			Lds L = layout(true, weighted, nb, slots, f_elems);
 			const int per_cu = std::max(1, std::min(8, (int)(ctx->lds_max / std::max(L.total, 1))));
 			const int grid = std::min<long long>((long long)ctx->n_cu * per_cu, (long long)std::get<3>(sp));
 			if ((rc = launch(kind, weighted, d_items + std::get<2>(sp), (int)std::get<3>(sp), nb, slots, d_queue + qi,
