@@ -529,7 +529,7 @@ __device__ void process_item_fast(const JobDev &J, const WorkItem &W, const int4
 		if (jk) {
 			code = -(r + 1);
 			for (int s = 0; s < n_slots; ++s)
-				if (sreg[s] == r) code = s;
+				if (sreg[s] == r) code = s * nb;
 		}
 		// angular counts work on the unit vectors (1 - n1.n2 = |n1 - n2|^2 / 2)
 		const double *px = KIND == K_ANGULAR ? J.A.nx : J.A.x;
@@ -554,10 +554,12 @@ __device__ void process_item_fast(const JobDev &J, const WorkItem &W, const int4
 
 	unsigned int n_acc = 0, n_rc = 0;
 
+	// Region codes: s * nb for LDS slot s, -(region + 1) for a region without
+	// a slot in this item (its sums go straight to global memory).
 	auto add_region = [&](int code, int bin, float w) {
 		if (code >= 0) {
-			if (WEIGHTED) atomicAdd((double *)&H[code * nb + bin], (double)w);
-			else atomicAdd((unsigned int *)&H[code * nb + bin], 1u);
+			if (WEIGHTED) atomicAdd((double *)&H[code + bin], (double)w);
+			else atomicAdd((unsigned int *)&H[code + bin], 1u);
 		} else {
 			const size_t o = (size_t)(-code - 1) * nb + bin;
 			if (WEIGHTED) atomicAdd(&J.jk_d[o], (double)w);
@@ -569,8 +571,18 @@ __device__ void process_item_fast(const JobDev &J, const WorkItem &W, const int4
 		if (WEIGHTED) atomicAdd((double *)&F[fb], (double)w);
 		else atomicAdd((unsigned int *)&F[fb], 1u);
 		if (jk) {
-			add_region(si, bin, w);
-			if (sj != si) add_region(sj, bin, w);
+			if (__builtin_amdgcn_ballot_w64(si < 0 || sj < 0) == 0ull) {
+				// every active lane's regions have LDS slots
+				if (WEIGHTED) atomicAdd((double *)&H[si + bin], (double)w);
+				else atomicAdd((unsigned int *)&H[si + bin], 1u);
+				if (sj != si) {
+					if (WEIGHTED) atomicAdd((double *)&H[sj + bin], (double)w);
+					else atomicAdd((unsigned int *)&H[sj + bin], 1u);
+				}
+			} else {
+				add_region(si, bin, w);
+				if (sj != si) add_region(sj, bin, w);
+			}
 		}
 	};
 	// wp: the float product of the two staged weights (w, or iw for the
@@ -609,7 +621,7 @@ __device__ void process_item_fast(const JobDev &J, const WorkItem &W, const int4
 						const int rj = J.B.jk[g2];
 						sj = -(rj + 1);
 						for (int s = 0; s < n_slots; ++s)
-							if (sreg[s] == rj) sj = s;
+							if (sreg[s] == rj) sj = s * nb;
 					}
 					const float w = weight(e.x, g2, cia[e.x].w * (stage_iw ? J.B.iw[g2] : J.B.w[g2]));
 					add(bin, w, __float_as_int(cib[e.x].w), sj);
@@ -743,7 +755,7 @@ __device__ void process_item_fast(const JobDev &J, const WorkItem &W, const int4
 					const int rj = J.B.jk[gj];
 					sl = -(rj + 1);
 					for (int s = 0; s < n_slots; ++s)
-						if (sreg[s] == rj) sl = s;
+						if (sreg[s] == rj) sl = s * nb;
 				}
 				const double *px = KIND == K_ANGULAR ? J.B.nx : J.B.x;
 				const double *py = KIND == K_ANGULAR ? J.B.ny : J.B.y;
