@@ -54,7 +54,8 @@ def build(verbose_isa=False):
     # twopcf: the plain build; twopcf_invp: the inverse-p build
     for name, inv in (("twopcf", 0), ("twopcf_invp", 1)):
         drv = os.path.join(HERE, name)
-        run(["g++", "-O2", "-std=c++17", "-ffp-contract=off", "-fno-fast-math", "-Wall", f"-DTWOPCF_INVERSE_P={inv}",
+        run(["g++", "-O2", "-std=c++17", "-pthread", "-ffp-contract=off", "-fno-fast-math", "-Wall",
+             f"-DTWOPCF_INVERSE_P={inv}",
              os.path.join(host, "twopcf_ref_api.cpp"), os.path.join(host, "main.cpp"),
              "-L", HERE, "-ltwopcf_hip", f"-Wl,-rpath,{HERE}", "-Wl,-rpath,$ORIGIN", "-o", drv + ".tmp"])
         os.replace(drv + ".tmp", drv)

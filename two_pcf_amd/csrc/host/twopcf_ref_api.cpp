@@ -12,6 +12,7 @@
 #include <iostream>
 #include <map>
 #include <sstream>
+#include <thread>
 
 // ------------------------------------------------------------------ context --
 
@@ -789,26 +790,108 @@ Params::Params(std::string fn) : param_filename(fn) {
 	if (n_jk_regions <= 1) n_jk_regions = 1;
 }
 
-// Next non-empty field of an ascii line, spaces removed (io.cpp:287-303).
-static std::string next_field(std::stringstream &ss, char sep) {
-	std::string part;
-	do {
-		if (!std::getline(ss, part, sep)) return std::string();
-	} while (part.empty());
-	return strip_spaces(part);
+// One field of an ascii line as the reference's reader takes it
+// (io.cpp:287-303, std::getline on the separator): the next non-empty part
+// between separators, spaces removed, copied into buf.  Returns false when
+// the line has no further part.
+static bool next_field(const char *&p, const char *e, char sep, char *buf, size_t cap) {
+	while (p < e) {
+		const char *q = (const char *)std::memchr(p, sep, (size_t)(e - p));
+		if (!q) q = e;
+		const char *b = p;
+		p = q < e ? q + 1 : e;
+		if (q == b) continue;   // empty part
+		size_t n = 0;
+		for (const char *c = b; c < q; ++c)
+			if (*c != ' ' && n + 1 < cap) buf[n++] = *c;
+		buf[n] = '\0';
+		return true;
+	}
+	return false;
 }
 
-// io.cpp:240-415
+namespace {
+
+// Parse failures of a line, reported by the caller with its line number.
+enum { PARSE_OK = 0, PARSE_MISSING = 1, PARSE_BAD = 2 };
+
+struct Columns {
+	std::vector<double> c1, c2, c3;
+	std::vector<float> w;
+	std::vector<int> jk;
+	std::vector<uint64_t> masks;
+};
+
+// The numeric conversions of std::stod / std::stof / std::stoi / std::stoll /
+// std::stoull (strtod, strtof, strtol, strtoll, strtoull on the field).
+int parse_line(const char *b, const char *e, const Params &p, int n_words, size_t i, Columns &c) {
+	char sep = ' ';
+	if (std::memchr(b, '\t', (size_t)(e - b))) sep = '\t';
+	if (std::memchr(b, ',', (size_t)(e - b))) sep = ',';
+	char buf[128];
+	char *end;
+	double *xyz[3] = {&c.c1[i], &c.c2[i], &c.c3[i]};
+	for (double *v : xyz) {
+		if (!next_field(b, e, sep, buf, sizeof buf)) return PARSE_MISSING;
+		*v = std::strtod(buf, &end);
+		if (end == buf) return PARSE_BAD;
+	}
+	c.w[i] = 1.0f;
+	if (p.use_weights) {
+		if (!next_field(b, e, sep, buf, sizeof buf)) return PARSE_MISSING;
+		c.w[i] = std::strtof(buf, &end);
+		if (end == buf) return PARSE_BAD;
+	}
+	c.jk[i] = 0;
+	if (p.n_jk_regions > 1) {
+		if (!next_field(b, e, sep, buf, sizeof buf)) return PARSE_MISSING;
+		const long v = std::strtol(buf, &end, 10);
+		if (end == buf) return PARSE_BAD;
+		c.jk[i] = (int)v;
+	}
+	for (int k = 0; k < n_words; ++k) {
+		if (!next_field(b, e, sep, buf, sizeof buf)) return PARSE_MISSING;
+		uint64_t m = buf[0] == '-' ? (uint64_t)std::strtoll(buf, &end, 10) : (uint64_t)std::strtoull(buf, &end, 10);
+		if (end == buf) return PARSE_BAD;
+		c.masks[i * (size_t)n_words + k] = m;
+	}
+	return PARSE_OK;
+}
+
+}  // namespace
+
+// Run work(t) for t = 0 .. n_threads - 1 on the calling thread and
+// n_threads - 1 others.
+static const int n_threads = std::max(1, std::min(16, (int)std::thread::hardware_concurrency()));
+template <class F>
+static void parallel_chunks(F work) {
+	std::vector<std::thread> pool;
+	for (int t = 1; t < n_threads; ++t) pool.emplace_back(work, t);
+	work(0);
+	for (auto &th : pool) th.join();
+}
+
+// io.cpp:240-415 on a whole-file buffer: lines (not empty, not starting with
+// '#') are found in one pass, then parsed by up to 16 threads into
+// per-object columns; values are the reference's conversions of the same
+// fields.
 GalaxyCatalogue *read_data_ascii(std::string filename, const Params &p) {
-	std::ifstream file(filename);
+	std::ifstream file(filename, std::ios::binary);
 	if (!file.is_open()) {
 		std::cout << "File not open\n";
 		return new GalaxyCatalogue(0, p.n_jk_regions);
 	}
-	std::vector<std::string> lines;
-	std::string line;
-	while (std::getline(file, line))
-		if (!line.empty() && line[0] != '#') lines.push_back(line);
+	file.seekg(0, std::ios::end);
+	std::string data((size_t)std::max<std::streamoff>(file.tellg(), 0), '\0');
+	file.seekg(0, std::ios::beg);
+	file.read(&data[0], (std::streamsize)data.size());
+	std::vector<std::pair<size_t, size_t>> lines;
+	for (size_t pos = 0; pos < data.size();) {
+		const char *nl = (const char *)std::memchr(data.data() + pos, '\n', data.size() - pos);
+		const size_t end = nl ? (size_t)(nl - data.data()) : data.size();
+		if (end > pos && data[pos] != '#') lines.emplace_back(pos, end);
+		pos = end + 1;
+	}
 	const int n = (int)lines.size();
 	auto *cat = new GalaxyCatalogue(n, p.n_jk_regions);
 	const bool eq = p.coord_system == "equatorial";
@@ -816,59 +899,84 @@ GalaxyCatalogue *read_data_ascii(std::string filename, const Params &p) {
 		std::cout << "Coord system not found\n";
 		return cat;
 	}
-	std::vector<double> c1(n), c2(n), c3(n);
 	// inverse-p build: bit masks as extra integer columns
 	if (twopcf_inverse_p_enabled() && p.n_bitwise_runs <= 0) {
 		std::fprintf(stderr, "twopcf: the inverse-p pipeline needs n_bitwise_runs > 0\n");
 		std::exit(1);
 	}
 	const int n_words = twopcf_inverse_p_enabled() ? (p.n_bitwise_runs + 63) / 64 : 0;
-	cat->n_words = n_words;
-	cat->masks.assign((size_t)n * n_words, 0);
+	Columns c;
+	c.c1.resize(n);
+	c.c2.resize(n);
+	c.c3.resize(n);
+	c.w.resize(n);
+	c.jk.resize(n);
+	c.masks.assign((size_t)n * n_words, 0);
+	std::vector<int> fail_at(n_threads, -1), fail_code(n_threads, PARSE_OK);
+	auto work = [&](int t) {
+		const size_t b = (size_t)n * t / n_threads, e = (size_t)n * (t + 1) / n_threads;
+		for (size_t i = b; i < e; ++i) {
+			const int rc = parse_line(data.data() + lines[i].first, data.data() + lines[i].second, p, n_words, i, c);
+			if (rc) {
+				fail_at[t] = (int)i;
+				fail_code[t] = rc;
+				return;
+			}
+		}
+	};
+	parallel_chunks(work);
+	for (int t = 0; t < n_threads; ++t)
+		if (fail_at[t] >= 0) {
+			std::fprintf(stderr, "twopcf: %s catalogue line %d: %s\n", filename.c_str(), fail_at[t] + 1,
+			             fail_code[t] == PARSE_MISSING ? "too few columns" : "not a number");
+			std::exit(1);
+		}
 	for (int i = 0; i < n; ++i) {
-		const std::string &l = lines[i];
-		char sep = ' ';
-		if (l.find('\t') != std::string::npos) sep = '\t';
-		if (l.find(',') != std::string::npos) sep = ',';
-		std::stringstream ss(l);
-		c1[i] = std::stod(next_field(ss, sep));
-		c2[i] = std::stod(next_field(ss, sep));
-		c3[i] = std::stod(next_field(ss, sep));
 		Galaxy &g = cat->gals[i];
-		g.weight = p.use_weights ? std::stof(next_field(ss, sep)) : 1.0f;
-		g.jk_region = p.n_jk_regions > 1 ? std::stoi(next_field(ss, sep)) : 0;
+		g.weight = c.w[i];
+		g.jk_region = c.jk[i];
 		g.input_weight = g.weight;
 		if (g.jk_region < 0 || g.jk_region >= p.n_jk_regions) {
 			std::fprintf(stderr, "twopcf: %s line %d: jackknife region %d outside [0, %d)\n", filename.c_str(), i + 1,
 			             g.jk_region, p.n_jk_regions);
 			std::exit(1);
 		}
-		for (int k = 0; k < n_words; ++k) {
-			const std::string f = next_field(ss, sep);
-			if (f.empty()) {
-				std::fprintf(stderr, "twopcf: %s line %d: %d bit-mask columns expected\n", filename.c_str(), i + 1,
-				             n_words);
-				std::exit(1);
-			}
-			cat->masks[(size_t)i * n_words + k] =
-			    f[0] == '-' ? (uint64_t)std::stoll(f) : (uint64_t)std::stoull(f);
-		}
 	}
+	cat->n_words = n_words;
+	cat->masks.swap(c.masks);
 	if (n_words > 0) apply_bitwise_weights(cat, p);
 	if (eq) {
+		// comoving distances and Galaxy::set, in chunks over the threads
+		// (both are per-object functions of the library)
 		std::vector<double> d(n);
-		check(twopcf_comoving_distance(p.omega_m, p.h, p.z_min, p.z_max, n, c3.data(), d.data()),
-		      "twopcf_comoving_distance");
+		std::vector<double> pos[6];
+		for (auto &v : pos) v.resize(n);
+		std::vector<int> rc(n_threads, 0);
+		parallel_chunks([&](int t) {
+			const size_t b = (size_t)n * t / n_threads, e = (size_t)n * (t + 1) / n_threads;
+			const int64_t m = (int64_t)(e - b);
+			if (m <= 0) return;
+			rc[t] = twopcf_comoving_distance(p.omega_m, p.h, p.z_min, p.z_max, m, c.c3.data() + b, d.data() + b);
+			if (!rc[t])
+				rc[t] = twopcf_eq_to_cart(m, c.c1.data() + b, c.c2.data() + b, d.data() + b, pos[0].data() + b,
+				                          pos[1].data() + b, pos[2].data() + b, pos[3].data() + b, pos[4].data() + b,
+				                          pos[5].data() + b);
+		});
+		for (int r : rc) check(r, "twopcf_comoving_distance / twopcf_eq_to_cart");
+		for (int i = 0; i < n; ++i) cat->gals[i].d_comov = d[i];
 		for (int i = 0; i < n; ++i) {
-			cat->gals[i].d_comov = d[i];
-			cat->gals[i].set(c1[i], c2[i]);
+			Galaxy &g = cat->gals[i];
+			for (int k = 0; k < 3; ++k) {
+				g.x[k] = pos[k][i];
+				g.x_norm[k] = pos[3 + k][i];
+			}
 		}
 	} else {
 		for (int i = 0; i < n; ++i) {
 			Galaxy &g = cat->gals[i];
-			g.x[0] = c1[i];
-			g.x[1] = c2[i];
-			g.x[2] = c3[i];
+			g.x[0] = c.c1[i];
+			g.x[1] = c.c2[i];
+			g.x[2] = c.c3[i];
 			g.d_comov = std::sqrt(g.x[0] * g.x[0] + g.x[1] * g.x[1] + g.x[2] * g.x[2]);
 			for (int k = 0; k < 3; ++k) g.x_norm[k] = g.x[k] / g.d_comov;
 		}
