@@ -32,7 +32,7 @@
 extern "C" {
 #endif
 
-#define TWOPCF_ABI_VERSION 1
+#define TWOPCF_ABI_VERSION 2
 
 enum {
 	TWOPCF_OK = 0,
@@ -63,6 +63,12 @@ enum {
 	 * of a range limit or bin edge are re-evaluated with the exact pipeline,
 	 * so counts are identical to TWOPCF_PLAIN_FP64. */
 	TWOPCF_FP32_RECHECK = 1
+};
+
+/* Line-of-sight definition of pi (sigma-pi and s-mu). */
+enum {
+	TWOPCF_PI_ANGULAR = 0,   /* (d . (n1 + n2)) / 2, _ANGULAR_PI_DEFN (makefile:4, the default build) correlators.cpp:319-323 */
+	TWOPCF_PI_MIDPOINT = 1   /* d . (x1 + x2) / |x1 + x2|, builds without it          correlators.cpp:325-336 */
 };
 
 /* Pair-weight rules (correlators.cpp:340, correlators_invpweight.cpp). */
@@ -112,6 +118,7 @@ typedef struct {
 	int32_t n_runs;                 /* Params::n_bitwise_runs (inverse-p modes)         */
 	const double *up_bins;          /* angular up-weight DD/DDinvp per bin (modes 1,2)  */
 	twopcf_axis up_axis;            /* its axis                                         */
+	int32_t pi_def;                 /* TWOPCF_PI_* (0: the reference's default build)   */
 	/* outputs, caller-allocated host memory */
 	double *full;                   /* [nb]        raw sum per bin                      */
 	double *jk;                     /* [n_jk * nb] raw sum over pairs touching region r (may be NULL) */

@@ -39,7 +39,10 @@ class Cat(C.Structure):
 
 class WOpt(C.Structure):
     _fields_ = [("mode", C.c_int), ("n_runs", C.c_int), ("up_bins", C.c_void_p),
-                ("up_axis", Axis)]
+                ("up_axis", Axis), ("pi_def", C.c_int)]
+
+
+PI_ANGULAR, PI_MIDPOINT = 0, 1
 
 
 def build():

@@ -15,6 +15,10 @@
  *       (exact integer counts and exactly-representable weighted sums) on
  *       seeded catalogues including duplicates, lattice points on bin edges,
  *       radial (mu = 1) pairs and log axes with min > 0; plus brute force.
+ *   or_pair_bin_pi, pair-midpoint line of sight       correlators.cpp:172-183,
+ *       325-336 (builds without _ANGULAR_PI_DEFN) — pinned: golden vectors
+ *       of the reference compiled without that macro (oracle/_ref/
+ *       ref_counts_mid), sigma-pi and s-mu.
  *   or_count (cells vs brute force)                   correlators.cpp:11-125,
  *       190-278, 345-430 — pinned: both methods agree bit-for-bit and agree
  *       with the golden vectors; sum over bins == accepted pair count.
@@ -135,9 +139,27 @@ static inline float los_pi(float d0, float d1, float d2, const double *n1, const
 	return p;
 }
 
+/* Pair-midpoint line-of-sight separation, correlators.cpp:172-183 / 325-336
+ * (builds without _ANGULAR_PI_DEFN): sum_vec = float(x1 + x2), its squared
+ * norm summed in float, the inverse norm as 1.0 / sqrtf(...) in double
+ * rounded to float, then pi = (d0 sum0) inv + (d1 sum1) inv + (d2 sum2) inv
+ * in float, left to right. */
+static inline float mid_pi(float d0, float d1, float d2, const double *x1, const double *x2) {
+	float sv[3], inv = 0.0f;
+	for (int k = 0; k < 3; ++k) {
+		sv[k] = (float)(x1[k] + x2[k]);
+		inv += sv[k] * sv[k];
+	}
+	inv = (float)(1.0 / (double)sqrtf(inv));
+	float p = d0 * sv[0] * inv;
+	p += d1 * sv[1] * inv;
+	p += d2 * sv[2] * inv;
+	return p;
+}
+
 static inline __attribute__((always_inline)) int pair_bin_k(int kind,
 		const double *x1, const double *n1, const double *x2, const double *n2,
-		float min_r_sq, float max_r_sq, const or_axis *ax1, const or_axis *ax2, int *bin) {
+		float min_r_sq, float max_r_sq, const or_axis *ax1, const or_axis *ax2, int pi_def, int *bin) {
 	if (kind == OR_ANGULAR) {
 		double v = 1.0;
 		v -= n1[0] * n2[0];
@@ -151,7 +173,7 @@ static inline __attribute__((always_inline)) int pair_bin_k(int kind,
 	float s = d0 * d0 + d1 * d1 + d2 * d2;
 	if (!(s < max_r_sq && s > min_r_sq)) return 0;
 	if (kind == OR_MONO) return accept_idx(ax1, or_axis_bin_num(ax1, sqrtf(s)), bin);
-	float p = los_pi(d0, d1, d2, n1, n2);
+	float p = pi_def == OR_PI_MIDPOINT ? mid_pi(d0, d1, d2, x1, x2) : los_pi(d0, d1, d2, n1, n2);
 	int b1, b2;
 	if (kind == OR_SIGMA_PI) {
 		float sig = sqrtf(s - p * p);
@@ -168,7 +190,12 @@ static inline __attribute__((always_inline)) int pair_bin_k(int kind,
 
 int or_pair_bin(int kind, const double *x1, const double *n1, const double *x2, const double *n2,
                 float min_r_sq, float max_r_sq, const or_axis *ax1, const or_axis *ax2, int *bin) {
-	return pair_bin_k(kind, x1, n1, x2, n2, min_r_sq, max_r_sq, ax1, ax2, bin);
+	return pair_bin_k(kind, x1, n1, x2, n2, min_r_sq, max_r_sq, ax1, ax2, OR_PI_ANGULAR, bin);
+}
+
+int or_pair_bin_pi(int kind, const double *x1, const double *n1, const double *x2, const double *n2,
+                   float min_r_sq, float max_r_sq, const or_axis *ax1, const or_axis *ax2, int pi_def, int *bin) {
+	return pair_bin_k(kind, x1, n1, x2, n2, min_r_sq, max_r_sq, ax1, ax2, pi_def, bin);
 }
 
 /* ------------------------------------------------------------ weights -- */
@@ -337,7 +364,7 @@ static inline __attribute__((always_inline)) void loop_block(int kind, const or_
 			double x2[3], n2[3];
 			load_pos(b, j, x2, n2);
 			int bin;
-			if (pair_bin_k(kind, x1, n1, x2, n2, mn, mx, ax1, ax2, &bin))
+			if (pair_bin_k(kind, x1, n1, x2, n2, mn, mx, ax1, ax2, wo ? wo->pi_def : OR_PI_ANGULAR, &bin))
 				add_pair(h, nbins, bin, a->jk[i], b->jk[j], pair_weight(wo, a, i, b, j, n1, n2));
 		}
 	}

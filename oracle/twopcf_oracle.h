@@ -48,11 +48,16 @@ enum {
 	OR_W_ANG_W = 5         /* w1*w2 (angular DR with inverse-p)       correlators_invpweight.cpp:853 */
 };
 
+/* Line-of-sight definition of pi: the angular one (_ANGULAR_PI_DEFN,
+ * makefile:4, the reference's default build) or the pair midpoint. */
+enum { OR_PI_ANGULAR = 0, OR_PI_MIDPOINT = 1 };
+
 typedef struct {
 	int mode;
 	int n_runs;
 	const double *up_bins;   /* angular up-weight table (JKHists::full_hist.bins of DD/DDinvp) */
 	or_axis up_axis;
+	int pi_def;              /* OR_PI_* */
 } or_wopt;
 
 enum { OR_MONO = 0, OR_SIGMA_PI = 1, OR_S_MU = 2, OR_ANGULAR = 3 };
@@ -68,6 +73,9 @@ double or_interp(const or_axis *a, const double *bins, double value);
  * flattened bin if the pair lands in the histogram, else 0. */
 int or_pair_bin(int kind, const double *x1, const double *n1, const double *x2, const double *n2,
                 float min_r_sq, float max_r_sq, const or_axis *ax1, const or_axis *ax2, int *bin);
+/* The same with the line-of-sight definition chosen (OR_PI_*). */
+int or_pair_bin_pi(int kind, const double *x1, const double *n1, const double *x2, const double *n2,
+                   float min_r_sq, float max_r_sq, const or_axis *ax1, const or_axis *ax2, int pi_def, int *bin);
 void or_range_sq(int kind, const or_axis *ax1, const or_axis *ax2, float *min_r_sq, float *max_r_sq);
 
 /* Raw pair sums: full[nb] and sub[n_jk*nb] = sum over pairs touching each

@@ -16,6 +16,8 @@ class Case:
         self.p = parse_params(self.g["params"])
         self.coord = self.p["coord_system"]
         self.n_jk = max(1, int(self.p["n_jk_regions"]))
+        # line of sight: 0 angular (the reference's default build), 1 pair midpoint
+        self.pi_def = int(self.g["pi_def"]) if "pi_def" in self.g else 0
         self.inputs = {}
         cols = ("x", "y", "z") if self.coord == "cartesian" else ("ra", "dec", "z")
         if "in_data_w" in self.g:

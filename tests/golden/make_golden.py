@@ -35,6 +35,8 @@ sys.path.insert(0, ROOT)
 from two_pcf_amd import synth  # noqa: E402
 
 REF = os.path.join(ROOT, "oracle", "_ref", "ref_counts")
+# the reference built without _ANGULAR_PI_DEFN (pair-midpoint line of sight)
+REF_MID = os.path.join(ROOT, "oracle", "_ref", "ref_counts_mid")
 
 BASE_PARAMS = dict(
     data_file_type="ascii", random_file_type="ascii", coord_system="cartesian",
@@ -69,7 +71,8 @@ def write_cat(path, cols, w, jk, use_w, n_jk):
             f.write(" ".join(parts) + "\n")
 
 
-def run_case(name, params, data, rand, coord, dump_cat=False, store_inputs=True, norm_sub=("monopole",)):
+def run_case(name, params, data, rand, coord, dump_cat=False, store_inputs=True, norm_sub=("monopole",),
+             midpoint=False):
     """data/rand: dict with three position columns (x,y,z or ra,dec,z), w, jk.
     store_inputs=False keeps only a SHA-256 of the inputs (for catalogues
     regenerated from a seed); norm_sub lists the correlators whose
@@ -89,7 +92,7 @@ def run_case(name, params, data, rand, coord, dump_cat=False, store_inputs=True,
         with open(pf, "w") as f:
             f.write(text)
         js = os.path.join(td, "out.json")
-        subprocess.run([REF, pf, js, "1" if dump_cat else "0", "1"], check=True,
+        subprocess.run([REF_MID if midpoint else REF, pf, js, "1" if dump_cat else "0", "1"], check=True,
                        stdout=subprocess.DEVNULL, cwd=td)
         with open(js) as f:
             res = json.load(f)
@@ -108,6 +111,7 @@ def run_case(name, params, data, rand, coord, dump_cat=False, store_inputs=True,
             for fld, v in res[f"{tag}_cat"].items():
                 out[f"cat_{tag}_{fld}"] = np.array([float.fromhex(s) for s in v])
     out["in_sha256"] = np.array(h.hexdigest())
+    out["pi_def"] = np.array(1 if midpoint else 0)
     n_jk = max(1, int(p["n_jk_regions"]))
     for corr in ("monopole", "sigma_pi", "s_mu"):
         if corr not in res:
@@ -221,10 +225,48 @@ def case_uniform_monopole():
              "cartesian", store_inputs=False)
 
 
+def case_cart_midpoint():
+    """Pair-midpoint line of sight (the reference built without
+    _ANGULAR_PI_DEFN), cartesian points around the origin: pairs on opposite
+    sides have a tiny or zero x1 + x2."""
+    rng = np.random.Generator(np.random.PCG64(16))
+    nd, nr = 1200, 2000
+    d = rng.uniform(-70, 70, (nd, 3))
+    d[:10] = -d[10:20]                    # x1 + x2 == 0 exactly
+    r = rng.uniform(-70, 70, (nr, 3))
+    wd = rng.uniform(0.5, 1.5, nd).astype(np.float32)
+    wr = rng.uniform(0.5, 1.5, nr).astype(np.float32)
+    jd = rng.integers(0, 3, nd).astype(np.int32)
+    jr = rng.integers(0, 3, nr).astype(np.int32)
+    run_case("cart_midpoint",
+             dict(use_weights=1, n_jk_regions=3, plot_sigma_pi=1, plot_s_mu=1, sigma_log_base=1.25,
+                  sigma_min=1.0, sigma_max=40.0, sigma_n_bins=20, pi_max=40.0, pi_n_bins=20),
+             cat_dict(d.T, "xyz", wd, jd), cat_dict(r.T, "xyz", wr, jr), "cartesian", midpoint=True)
+
+
+def case_eq_midpoint():
+    ra_l, dec_l, z_l = (180.0, 195.0), (5.0, 20.0), (0.10, 0.16)
+    ra, dec, z = synth.neyman_scott_equatorial(70, 25.0, 5.0, ra_l, dec_l, z_l, seed=12)
+    rra, rdec, rz = synth.randoms_equatorial(3000, ra_l, dec_l, z, seed=13)
+    rng = np.random.Generator(np.random.PCG64(14))
+    wd = rng.uniform(0.5, 1.5, ra.size).astype(np.float32)
+    wr = rng.uniform(0.5, 1.5, rra.size).astype(np.float32)
+    jd = synth.jk_grid_labels(ra, dec, ra_l, dec_l, 3, 3)
+    jr = synth.jk_grid_labels(rra, rdec, ra_l, dec_l, 3, 3)
+    run_case("eq_midpoint",
+             dict(use_weights=1, n_jk_regions=9, plot_sigma_pi=1, plot_s_mu=1),
+             cat_dict((ra, dec, z), ("ra", "dec", "z"), wd, jd),
+             cat_dict((rra, rdec, rz), ("ra", "dec", "z"), wr, jr), "equatorial", midpoint=True)
+
+
+CASES = dict(cart_uniform=case_cart_uniform, eq_clustered=case_eq_clustered, edge=case_edge,
+             uniform_monopole=case_uniform_monopole, cart_midpoint=case_cart_midpoint,
+             eq_midpoint=case_eq_midpoint)
+
 if __name__ == "__main__":
-    if not os.path.exists(REF):
-        sys.exit(f"{REF} missing: run make -C oracle ref")
-    case_cart_uniform()
-    case_eq_clustered()
-    case_edge()
-    case_uniform_monopole()
+    # python make_golden.py [case ...]   (all cases by default)
+    for ref in (REF, REF_MID):
+        if not os.path.exists(ref):
+            sys.exit(f"{ref} missing: run make -C oracle ref")
+    for name in sys.argv[1:] or list(CASES):
+        CASES[name]()

@@ -11,7 +11,7 @@ from cases import load_case
 pytestmark = pytest.mark.gpu
 
 KIND = {"monopole": 0, "sigma_pi": 1, "s_mu": 2}
-GOLDEN = ["cart_uniform", "eq_clustered", "edge", "uniform_monopole_50k"]
+GOLDEN = ["cart_uniform", "eq_clustered", "edge", "uniform_monopole_50k", "cart_midpoint", "eq_midpoint"]
 
 
 @pytest.fixture(scope="module")
@@ -44,9 +44,9 @@ def test_golden(ctx, name, method):
     cats = lib_cats(L, ctx, case.inputs, case.coord)
     for corr in case.corrs():
         axes = case.axes(corr)
-        jobs = [dict(kind=KIND[corr], a=cats["data"], b=None, axes=axes, n_jk=case.n_jk),
-                dict(kind=KIND[corr], a=cats["data"], b=cats["randoms"], axes=axes, n_jk=case.n_jk),
-                dict(kind=KIND[corr], a=cats["randoms"], b=None, axes=axes, n_jk=case.n_jk)]
+        jobs = [dict(kind=KIND[corr], a=cats["data"], b=None, axes=axes, n_jk=case.n_jk, pi_def=case.pi_def),
+                dict(kind=KIND[corr], a=cats["data"], b=cats["randoms"], axes=axes, n_jk=case.n_jk, pi_def=case.pi_def),
+                dict(kind=KIND[corr], a=cats["randoms"], b=None, axes=axes, n_jk=case.n_jk, pi_def=case.pi_def)]
         res, _ = ctx.count(jobs, method=method)
         for h, r, j in zip(("DD", "DR", "RR"), res, jobs):
             full, sub = L.finish(j["b"] is None, r["full"], r["jk"])

@@ -1,6 +1,8 @@
 // twopcf / twopcf_invp — the parameter-file driver (main.cpp:12-75 of the
 // reference) on the GPU counting library.  Usage: twopcf <param-file>
+#include <cstdlib>
 #include <iostream>
+#include <string>
 
 #include "twopcf_ref_api.hpp"
 
@@ -20,6 +22,8 @@ int main(int argc, char *argv[]) {
 	std::cout << "Correct number of arguments passed\n";
 	Params params(argv[1]);
 	twopcf_enable_inverse_p(TWOPCF_INVERSE_P);
+	const char *pi = std::getenv("TWOPCF_PI");
+	if (pi && std::string(pi) == "midpoint") twopcf_set_pi_definition(TWOPCF_PI_MIDPOINT);
 	GalaxyCatalogue *data = read_file(params.data_filename, params.data_file_type, params);
 	GalaxyCatalogue *randoms = read_file(params.random_filename, params.random_file_type, params);
 	if (TWOPCF_INVERSE_P) {

@@ -359,6 +359,7 @@ twopcf_job job1(int kind, const GalaxyBoxes *a, const GalaxyBoxes *b, JKHists *h
 
 twopcf_job job2(int kind, const GalaxyBoxes *a, const GalaxyBoxes *b, JKHists2D *h) {
 	twopcf_job j{};
+	j.pi_def = twopcf_pi_definition();
 	j.kind = kind;
 	j.a = a->cat;
 	j.b = b ? b->cat : nullptr;
@@ -685,6 +686,10 @@ void auto_correlate_s_mu_invpweights(GalaxyCatalogue *data, GalaxyCatalogue *ran
 	RR.normalise(randoms, rb.n_gals);
 	output_xi_2D(p.s_mu_filename, p.s_mu_output_type, "s", "mu", &DD, &DR, &RR, data, randoms);
 }
+
+static int g_pi_def = TWOPCF_PI_ANGULAR;
+void twopcf_set_pi_definition(int pi_def) { g_pi_def = pi_def; }
+int twopcf_pi_definition() { return g_pi_def; }
 
 static bool g_inverse_p = false;
 void twopcf_enable_inverse_p(bool on) { g_inverse_p = on; }

@@ -226,6 +226,13 @@ void auto_correlate_monopole_invpweights(GalaxyCatalogue *data, GalaxyCatalogue 
 void auto_correlate_sigma_pi_invpweights(GalaxyCatalogue *data, GalaxyCatalogue *randoms, Params params);
 void auto_correlate_s_mu_invpweights(GalaxyCatalogue *data, GalaxyCatalogue *randoms, Params params);
 
+// The line-of-sight definition of sigma-pi and s-mu (TWOPCF_PI_*).  The
+// reference fixes it when compiled (_ANGULAR_PI_DEFN, makefile:4, set in its
+// default build); here the drivers take it from TWOPCF_PI (angular by
+// default, "midpoint" for builds without that macro).
+void twopcf_set_pi_definition(int pi_def);
+int twopcf_pi_definition();
+
 // The inverse-p build switch: the readers then take bit masks and apply the
 // IIP weights (io.cpp:521-588).
 void twopcf_enable_inverse_p(bool on);

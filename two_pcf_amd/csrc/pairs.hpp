@@ -64,6 +64,23 @@ __device__ __forceinline__ float ref_pi(float d0, float d1, float d2, const doub
 	return p;
 }
 
+// Line-of-sight separation, pair-midpoint definition (builds without
+// _ANGULAR_PI_DEFN, correlators.cpp:325-336): sum = float(x1 + x2), its norm
+// squared summed in float, 1.0 / sqrtf(.) divided in double and rounded to
+// float, then the three float products added left to right.
+__device__ __forceinline__ float ref_pi_mid(float d0, float d1, float d2, const double *x1, const double *x2) {
+	const float s0 = (float)(x1[0] + x2[0]), s1 = (float)(x1[1] + x2[1]), s2 = (float)(x1[2] + x2[2]);
+	float q = 0.0f;
+	q += s0 * s0;
+	q += s1 * s1;
+	q += s2 * s2;
+	const float inv = (float)(1.0 / (double)__builtin_sqrtf(q));
+	float p = d0 * s0 * inv;
+	p += d1 * s1 * inv;
+	p += d2 * s2 * inv;
+	return p;
+}
+
 // 1 - cos(theta) as the angular correlators form it: double, running
 // subtraction of the three products (correlators_invpweight.cpp:738-741).
 __device__ __forceinline__ double ref_one_minus_cos(const double *n1, const double *n2) {
@@ -81,7 +98,8 @@ __device__ __forceinline__ double ref_one_minus_cos(const double *n1, const doub
 template <int KIND>
 __device__ __forceinline__ bool exact_bin(const double *x1, const double *n1, const double *x2, const double *n2,
                                           float min_r_sq, float max_r_sq, const float *t_s2, const float *t1,
-                                          const float *t2, const AxisDev &a1, const AxisDev &a2, int &bin) {
+                                          const float *t2, const AxisDev &a1, const AxisDev &a2, int &bin,
+                                          int pi_mid = 0) {
 	if constexpr (KIND == K_ANGULAR) {
 		// value rounded to float for Axis::bin_num (hist.cpp:246-250); no range test
 		return tab_bin(t1, a1.n, (float)ref_one_minus_cos(n1, n2), bin);
@@ -97,7 +115,7 @@ __device__ __forceinline__ bool exact_bin(const double *x1, const double *n1, co
 			bin = b1;
 			return true;
 		}
-		float p = ref_pi(d0, d1, d2, n1, n2);
+		float p = pi_mid ? ref_pi_mid(d0, d1, d2, x1, x2) : ref_pi(d0, d1, d2, n1, n2);
 		float r = __builtin_sqrtf(s2);
 		float mu = fabsf(p) / r;
 		int b2;
@@ -106,7 +124,7 @@ __device__ __forceinline__ bool exact_bin(const double *x1, const double *n1, co
 		return true;
 	} else {
 		if (!(s2 < max_r_sq && s2 > min_r_sq)) return false;
-		float p = ref_pi(d0, d1, d2, n1, n2);
+		float p = pi_mid ? ref_pi_mid(d0, d1, d2, x1, x2) : ref_pi(d0, d1, d2, n1, n2);
 		float sig = __builtin_sqrtf(s2 - p * p);
 		int b1, b2;
 		if (a1.log ? !tab_bin(t1, a1.n, sig, b1) : !lin_bin(sig, a1.minus_min, a1.inv_step, a1.n, b1)) return false;

@@ -86,6 +86,8 @@ struct JobDev {
 	int p_log;
 	float e_c1, e_c2;                                // sigma-pi fp32: E(S) = e_c1 S + e_c2 sqrt(S)
 	float pi_c1, pi_c2;                              // sigma-pi fp32: |pi_fp32 - pi_ref| <= pi_c1 s + pi_c2
+	int pi_def;                                      // TWOPCF_PI_* (line of sight of sigma-pi and s-mu)
+	float mid_c;                                     // pair-midpoint pi: extra |dpi| / s per 1 / |x1 + x2|
 	int row_off, n_rows;                             // stencil rows in the row pool
 	int wmode, n_runs, n_words;                      // pair-weight rule (TWOPCF_W_*)
 	int up_n;                                        // angular up-weight table
@@ -415,7 +417,7 @@ __device__ void process_item(const JobDev &J, const WorkItem &W, const int4 *__r
 							++n_rc;
 							double xj[3] = {J.B.x[gj], J.B.y[gj], J.B.z[gj]};
 							double nj[3] = {J.B.nx[gj], J.B.ny[gj], J.B.nz[gj]};
-							if (!exact_bin<KIND>(xi, ni, xj, nj, J.min_r_sq, J.max_r_sq, t_s2, t1, t2, J.ax1, J.ax2, bin))
+							if (!exact_bin<KIND>(xi, ni, xj, nj, J.min_r_sq, J.max_r_sq, t_s2, t1, t2, J.ax1, J.ax2, bin, J.pi_def))
 								continue;
 						}
 						++n_acc;
@@ -424,7 +426,7 @@ __device__ void process_item(const JobDev &J, const WorkItem &W, const int4 *__r
 						const int gj = tgj[q];
 						if (J.is_auto && gj <= gi) continue;
 						const double *e = tx + 6 * q;
-						if (!exact_bin<KIND>(xi, ni, e, e + 3, J.min_r_sq, J.max_r_sq, t_s2, t1, t2, J.ax1, J.ax2, bin))
+						if (!exact_bin<KIND>(xi, ni, e, e + 3, J.min_r_sq, J.max_r_sq, t_s2, t1, t2, J.ax1, J.ax2, bin, J.pi_def))
 							continue;
 						++n_acc;
 						add(bin, J.wmode == TWOPCF_W_PLAIN ? wi * tw[q] : pair_weight(J, gi, gj, ni, e + 3), tslot[q]);
@@ -614,7 +616,7 @@ __device__ void process_item_fast(const JobDev &J, const WorkItem &W, const int4
 				const float *t1 = tab + J.t1_off;
 				const float *t2 = tab + J.t2_off;
 				int bin;
-				if (exact_bin<KIND>(xi, ni, xj, nj, J.min_r_sq, J.max_r_sq, t_s2, t1, t2, J.ax1, J.ax2, bin)) {
+				if (exact_bin<KIND>(xi, ni, xj, nj, J.min_r_sq, J.max_r_sq, t_s2, t1, t2, J.ax1, J.ax2, bin, J.pi_def)) {
 					++n_acc;
 					int sj = 0;
 					if (jk) {
@@ -637,6 +639,22 @@ __device__ void process_item_fast(const JobDev &J, const WorkItem &W, const int4
 	const float mu_c1 = J.mu_c1, mu_c2 = J.mu_c2;
 	const float *x1t = tab + J.x1_off, *t2t = tab + J.t2_off;
 	const float rA0 = tab[J.tA_off], rBn = tab[J.tB_off + n1];
+	// line of sight in FP32: the angular definition from the float unit
+	// vectors, or the pair midpoint from 2c + the two cell-local offsets;
+	// inv_mid = 1 / |x1 + x2| for the midpoint (it widens the margins), else 0
+	const bool pi_mid = J.pi_def == TWOPCF_PI_MIDPOINT;
+	const float mid_c = J.mid_c;
+	const float c2x = (float)(2.0 * c0), c2y = (float)(2.0 * c1), c2z = (float)(2.0 * c2);
+	auto fp32_pi = [&](float dx, float dy, float dz, const float4 &ia, const float4 &ib, const float4 &a,
+	                   const float4 &b, float &inv_mid) {
+		if (pi_mid) {
+			const float sx = c2x + (ia.x + a.x), sy = c2y + (ia.y + a.y), sz = c2z + (ia.z + a.z);
+			inv_mid = __builtin_amdgcn_rsqf(fmaf(sz, sz, fmaf(sy, sy, sx * sx)));
+			return fmaf(dz, sz, fmaf(dy, sy, dx * sx)) * inv_mid;
+		}
+		inv_mid = 0.0f;
+		return 0.5f * fmaf(dz, ib.z + b.z, fmaf(dy, ib.y + b.y, dx * (ib.x + b.x)));
+	};
 	// bin n queued pairs (lanes >= n idle)
 	auto consume = [&](unsigned head, unsigned n) {
 		__builtin_amdgcn_wave_barrier();
@@ -654,10 +672,11 @@ __device__ void process_item_fast(const JobDev &J, const WorkItem &W, const int4
 				// range test on s^2, sigma in the x = s^2 - pi^2 domain
 				// against sqrtf-domain thresholds, |pi| against its value
 				// thresholds; margins per pair from the bounds on s^2 and pi
-				const float p = 0.5f * fmaf(dz, ib.z + b.z, fmaf(dy, ib.y + b.y, dx * (ib.x + b.x)));
+				float inv_mid;
+				const float p = fp32_pi(dx, dy, dz, ia, ib, a, b, inv_mid);
 				const float ap = fabsf(p);
 				const float s = __builtin_amdgcn_sqrtf(s2);
-				const float dpi = fmaf(J.pi_c1, s, J.pi_c2);
+				const float dpi = fmaf(J.pi_c1, s, J.pi_c2) + mid_c * s * inv_mid;
 				const float x = fmaf(-p, p, s2);
 				const float ex = fmaf(J.e_c1, s2, J.e_c2 * s) + dpi * fmaf(2.0f, ap, dpi) + 3.6e-7f * fmaf(p, p, s2);
 				const bool r_in = s2 >= rA0 && s2 < rBn;   // min_r_sq < s2_ref < max_r_sq
@@ -689,11 +708,12 @@ __device__ void process_item_fast(const JobDev &J, const WorkItem &W, const int4
 			const float4 q0 = tq[k], q1 = tq[k + 1];
 			sure = (s2 >= q0.y) && (s2 < q1.z);   // T_k + E <= s2 < T_k+1 - E
 			if (KIND == K_S_MU) {
-				const float p = 0.5f * fmaf(dz, ib.z + b.z, fmaf(dy, ib.y + b.y, dx * (ib.x + b.x)));
+				float inv_mid;
+				const float p = fp32_pi(dx, dy, dz, ia, ib, a, b, inv_mid);
 				const float rs = __builtin_amdgcn_rsqf(s2);
 				const float nmu = (float)n2;
 				const float xm = fabsf(p) * rs * nmu;
-				const float del = fmaf(mu_c2, rs, mu_c1);
+				const float del = fmaf(mu_c2, rs, mu_c1) + nmu * mid_c * inv_mid;
 				const int km = (int)xm;
 				const float fr = xm - (float)km;
 				sure = sure && (xm < nmu - del) && (fr <= 1.0f - del) && (km == 0 || fr >= del);
@@ -1325,6 +1345,8 @@ int build_plan(twopcf_ctx *ctx, const twopcf_job *jobs, int n_jobs, int method, 
 		if (J.kind < TWOPCF_MONOPOLE || J.kind > TWOPCF_ANGULAR) return fail(ctx, TWOPCF_E_ARG, "bad correlator kind");
 		if (J.weight_mode < TWOPCF_W_PLAIN || J.weight_mode > TWOPCF_W_ANG_W)
 			return fail(ctx, TWOPCF_E_ARG, "bad weight mode");
+		if (J.pi_def != TWOPCF_PI_ANGULAR && J.pi_def != TWOPCF_PI_MIDPOINT)
+			return fail(ctx, TWOPCF_E_ARG, "bad line-of-sight definition");
 		const bool needs_masks = J.weight_mode == TWOPCF_W_INVP_3D || J.weight_mode == TWOPCF_W_ANG_INVP;
 		const bool needs_up = J.weight_mode == TWOPCF_W_INVP_3D || J.weight_mode == TWOPCF_W_ANGUP_3D;
 		if (needs_masks) {
@@ -1584,8 +1606,10 @@ int build_plan(twopcf_ctx *ctx, const twopcf_job *jobs, int n_jobs, int method, 
 				D.p_inv_step = H.ax2.inv_step;
 				D.e_c1 = (float)(1.02 * std::ldexp(11.0, -23));
 				D.e_c2 = (float)(1.02 * std::ldexp(4.0 * R, -23));
-				D.pi_c1 = (float)(1.02 * std::ldexp(20.0, -23));
+				const double mid = H.j->pi_def == TWOPCF_PI_MIDPOINT ? 2.0 : 1.0;
+				D.pi_c1 = (float)(1.02 * std::ldexp(20.0 * mid, -23));
 				D.pi_c2 = (float)(1.02 * std::ldexp(8.0 * R, -23));
+				D.mid_c = H.j->pi_def == TWOPCF_PI_MIDPOINT ? (float)(1.02 * std::ldexp(8.0 * R, -23)) : 0.0f;
 			}
 		} else {
 			D.t_s2_off = push(H.t_s2);
@@ -1617,8 +1641,12 @@ int build_plan(twopcf_ctx *ctx, const twopcf_job *jobs, int n_jobs, int method, 
 			D.s_ils = H.ax1.ils;
 			D.s_inv_step = H.ax1.inv_step;
 			const double n2 = H.ax2.n;
-			D.mu_c1 = (float)(n2 * std::ldexp(1.0, -23) * 41.0);
+			// the pair-midpoint line of sight doubles the relative part and
+			// adds 1.02 2^-23 8 R s / |x1 + x2| to the pi bound
+			const double mid = H.j->pi_def == TWOPCF_PI_MIDPOINT ? 2.0 : 1.0;
+			D.mu_c1 = (float)(n2 * std::ldexp(1.0, -23) * 41.0 * mid);
 			D.mu_c2 = (float)(n2 * std::ldexp(1.0, -23) * 4.75 * R * 1.01);
+			D.mid_c = H.j->pi_def == TWOPCF_PI_MIDPOINT ? (float)(1.02 * std::ldexp(8.0 * R, -23)) : 0.0f;
 			max_tab_n1 = std::max(max_tab_n1, H.ax1.n);
 		}
 		auto rr = rows_of[H.rmax];
@@ -1628,6 +1656,7 @@ int build_plan(twopcf_ctx *ctx, const twopcf_job *jobs, int n_jobs, int method, 
 		D.f_shift = (fast && (size_t)H.nb * WAVE * (H.weighted ? 8 : 4) <= 32768)
 		                ? 6 : 0;
 		P.f_elems = H.nb << D.f_shift;
+		D.pi_def = H.j->pi_def;
 		D.wmode = H.j->weight_mode;
 		D.n_runs = H.j->n_runs;
 		D.n_words = H.a->n_words;
@@ -1968,6 +1997,7 @@ std::string job_set_key(const twopcf_job *jobs, int n_jobs, int method, std::vec
 		put(&J.n_jk, sizeof J.n_jk);
 		put(&J.weight_mode, sizeof J.weight_mode);
 		put(&J.n_runs, sizeof J.n_runs);
+		put(&J.pi_def, sizeof J.pi_def);
 		put(&J.up_axis, sizeof J.up_axis);
 		if (J.up_bins && J.up_axis.n_bins > 0) put(J.up_bins, sizeof(double) * J.up_axis.n_bins);
 	}

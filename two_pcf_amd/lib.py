@@ -13,6 +13,8 @@ LIB_PATH = os.path.join(HERE, "libtwopcf_hip.so")
 MONOPOLE, SIGMA_PI, S_MU, ANGULAR = 0, 1, 2, 3
 PLAIN_FP64, FP32_RECHECK = 0, 1
 W_PLAIN, W_INVP_3D, W_ANGUP_3D, W_ANG_IW, W_ANG_INVP, W_ANG_W = range(6)
+PI_ANGULAR, PI_MIDPOINT = 0, 1
+ABI_VERSION = 2   # TWOPCF_ABI_VERSION of the structures below
 
 EXPORTS = [
     "twopcf_abi_version", "twopcf_ctx_create", "twopcf_ctx_destroy", "twopcf_last_error",
@@ -37,7 +39,7 @@ class AxisDesc(C.Structure):
 class JobDesc(C.Structure):
     _fields_ = [("kind", C.c_int32), ("a", C.c_void_p), ("b", C.c_void_p), ("ax", AxisDesc * 2),
                 ("n_jk", C.c_int32), ("weight_mode", C.c_int32), ("n_runs", C.c_int32),
-                ("up_bins", C.c_void_p), ("up_axis", AxisDesc),
+                ("up_bins", C.c_void_p), ("up_axis", AxisDesc), ("pi_def", C.c_int32),
                 ("full", C.c_void_p), ("jk", C.c_void_p), ("n_pairs", C.c_void_p),
                 ("n_rechecked", C.c_void_p)]
 
@@ -66,6 +68,9 @@ def lib():
             raise RuntimeError(f"{LIB_PATH} is not built (run `python -m two_pcf_amd.build`); "
                                "there is no CPU fallback")
         L = C.CDLL(LIB_PATH)
+        if L.twopcf_abi_version() != ABI_VERSION:
+            raise RuntimeError(f"{LIB_PATH} has ABI version {L.twopcf_abi_version()}, this binding {ABI_VERSION}; "
+                               "rebuild it (python -m two_pcf_amd.build)")
         L.twopcf_ctx_create.argtypes = [C.c_int, C.POINTER(C.c_void_p)]
         L.twopcf_ctx_destroy.argtypes = [C.c_void_p]
         L.twopcf_last_error.argtypes = [C.c_void_p]
@@ -249,6 +254,7 @@ class Context:
             D.n_jk = n_jk
             D.weight_mode = int(j.get("weight_mode", W_PLAIN))
             D.n_runs = int(j.get("n_runs", 0))
+            D.pi_def = int(j.get("pi_def", PI_ANGULAR))
             if j.get("up_bins") is not None:
                 ub = np.ascontiguousarray(j["up_bins"], dtype=np.float64)
                 o["_up"] = ub
