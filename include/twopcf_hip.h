@@ -32,7 +32,7 @@
 extern "C" {
 #endif
 
-#define TWOPCF_ABI_VERSION 2
+#define TWOPCF_ABI_VERSION 3
 
 enum {
 	TWOPCF_OK = 0,
@@ -96,6 +96,8 @@ typedef struct {
 	const float *iw;                /* Galaxy::input_weight  (NULL: same as w)    */
 	const uint64_t *masks;          /* Galaxy::bitmask, n_words per object, object-major (NULL: none) */
 	int32_t n_words;
+	const uint64_t *dither;         /* Galaxy::dither_mask (_USE_DITHER_WEIGHTS, galaxy.h:28-30),
+	                                   n_words per object, object-major (NULL: none) */
 } twopcf_cat_desc;
 
 /* Axis(min, max, n_bins, log_base) — hist.cpp:15-36.  Values are given as
@@ -119,6 +121,9 @@ typedef struct {
 	const double *up_bins;          /* angular up-weight DD/DDinvp per bin (modes 1,2)  */
 	twopcf_axis up_axis;            /* its axis                                         */
 	int32_t pi_def;                 /* TWOPCF_PI_* (0: the reference's default build)   */
+	int32_t dither;                 /* modes 1 and 4: numerator popc(dither1 & dither2), 0 -> 1,
+	                                   instead of n_runs (_USE_DITHER_WEIGHTS,
+	                                   correlators_invpweight.cpp:62-70, 749-757)       */
 	/* outputs, caller-allocated host memory */
 	double *full;                   /* [nb]        raw sum per bin                      */
 	double *jk;                     /* [n_jk * nb] raw sum over pairs touching region r (may be NULL) */

@@ -34,12 +34,12 @@ class Cat(C.Structure):
                 ("x", C.c_void_p), ("y", C.c_void_p), ("z", C.c_void_p),
                 ("nx", C.c_void_p), ("ny", C.c_void_p), ("nz", C.c_void_p),
                 ("w", C.c_void_p), ("jk", C.c_void_p), ("iw", C.c_void_p),
-                ("masks", C.c_void_p), ("n_words", C.c_int)]
+                ("masks", C.c_void_p), ("n_words", C.c_int), ("dither", C.c_void_p)]
 
 
 class WOpt(C.Structure):
     _fields_ = [("mode", C.c_int), ("n_runs", C.c_int), ("up_bins", C.c_void_p),
-                ("up_axis", Axis), ("pi_def", C.c_int)]
+                ("up_axis", Axis), ("pi_def", C.c_int), ("dither", C.c_int)]
 
 
 PI_ANGULAR, PI_MIDPOINT = 0, 1
@@ -119,7 +119,7 @@ def _p(a):
 class Catalog:
     """SoA catalogue as the oracle wants it (keeps the numpy arrays alive)."""
 
-    def __init__(self, x, y, z, nx, ny, nz, w=None, jk=None, iw=None, masks=None):
+    def __init__(self, x, y, z, nx, ny, nz, w=None, jk=None, iw=None, masks=None, dither=None):
         n = len(x)
         f8 = lambda a: np.ascontiguousarray(a, dtype=np.float64)
         self.x, self.y, self.z = f8(x), f8(y), f8(z)
@@ -129,8 +129,11 @@ class Catalog:
         self.iw = None if iw is None else np.ascontiguousarray(iw, dtype=np.float32)
         self.masks = None if masks is None else np.ascontiguousarray(masks, dtype=np.uint64)
         self.n_words = 0 if masks is None else self.masks.shape[1]
+        self.dither = None if dither is None else np.ascontiguousarray(dither, dtype=np.uint64)
+        if self.dither is not None and self.dither.shape != self.masks.shape:
+            raise ValueError("dither masks must have the shape of the bit masks")
         self.c = Cat(n, _p(self.x), _p(self.y), _p(self.z), _p(self.nx), _p(self.ny), _p(self.nz),
-                     _p(self.w), _p(self.jk), _p(self.iw), _p(self.masks), self.n_words)
+                     _p(self.w), _p(self.jk), _p(self.iw), _p(self.masks), self.n_words, _p(self.dither))
 
     def __len__(self):
         return int(self.c.n)

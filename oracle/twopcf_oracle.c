@@ -32,6 +32,10 @@
  *       reference binary (its inverse-p build needs libhdf5); pinned instead
  *       by brute-force popcount (Python int.bit_count) and closed forms in
  *       tests/test_oracle_invp.py.
+ *   dither-weight numerator (_USE_DITHER_WEIGHTS)     correlators_invpweight.cpp
+ *       :62-70, 749-757 — parity unpinned against the reference binary (same
+ *       libhdf5 reason); pinned by the same Python brute force, with dither
+ *       masks whose pair popcounts include 0 (the 0 -> 1 rule).
  *
  * Compiled with -ffp-contract=off and no fast-math so that every float
  * operation rounds exactly as in the reference's x86-64 g++ build.
@@ -206,6 +210,16 @@ static inline int popc_and(const uint64_t *m1, const uint64_t *m2, int nw) {
 	return c;
 }
 
+/* The numerator of the inverse-p rules: n_runs, or with dither weights the
+ * float popcount of the AND of the dither masks, 0 counting as 1
+ * (correlators_invpweight.cpp:62-68, 749-755). */
+static inline float invp_numerator(const or_wopt *wo, const or_cat *a, long i, const or_cat *b, long j) {
+	if (!wo->dither) return (float)wo->n_runs;
+	float dw = (float)popc_and(a->dither + i * a->n_words, b->dither + j * b->n_words, a->n_words);
+	if (dw < 0.5f) dw += 1.0f;
+	return dw;
+}
+
 /* Pair weight under each rule; see the enum in twopcf_oracle.h. */
 static inline float pair_weight(const or_wopt *wo, const or_cat *a, long i, const or_cat *b, long j,
                                 const double *n1, const double *n2) {
@@ -220,7 +234,10 @@ static inline float pair_weight(const or_wopt *wo, const or_cat *a, long i, cons
 		omc -= n1[2] * n2[2];
 		pc = (float)popc_and(a->masks + i * a->n_words, b->masks + j * b->n_words, a->n_words);
 		if (pc < 0.5f) pc += 1.0f;
-		pc = (float)(or_interp(&wo->up_axis, wo->up_bins, omc) * wo->n_runs / (double)pc);
+		/* interp(omc) * numerator / pc in double, rounded to float once; the
+		 * numerator is an integer (n_runs, or a float holding a popcount) */
+		pc = (float)(or_interp(&wo->up_axis, wo->up_bins, omc) * (double)invp_numerator(wo, a, i, b, j) /
+		             (double)pc);
 		return a->iw[i] * b->iw[j] * pc;
 	case OR_W_ANGUP_3D:
 		omc = 1.0;
@@ -233,7 +250,7 @@ static inline float pair_weight(const or_wopt *wo, const or_cat *a, long i, cons
 	case OR_W_ANG_INVP:
 		pc = (float)popc_and(a->masks + i * a->n_words, b->masks + j * b->n_words, a->n_words);
 		if (pc < 0.5f) pc += 1.0f;
-		pc = (float)wo->n_runs / pc;
+		pc = invp_numerator(wo, a, i, b, j) / pc;
 		return a->iw[i] * b->iw[j] * pc;
 	case OR_W_ANG_W:
 	default:
@@ -379,6 +396,8 @@ static int count_impl(int kind, const or_cat *a, const or_cat *b, const or_axis 
 	if (wo && wo->mode != OR_W_PLAIN) {
 		if ((wo->mode == OR_W_INVP_3D || wo->mode == OR_W_ANG_INVP) &&
 		    (!a->masks || !(b ? b : a)->masks || a->n_words != (b ? b : a)->n_words)) return -3;
+		if (wo->dither && (wo->mode == OR_W_INVP_3D || wo->mode == OR_W_ANG_INVP) &&
+		    (!a->dither || !(b ? b : a)->dither)) return -3;
 		if ((wo->mode == OR_W_INVP_3D || wo->mode == OR_W_ANG_IW || wo->mode == OR_W_ANG_INVP) &&
 		    (!a->iw || !(b ? b : a)->iw)) return -3;
 		if ((wo->mode == OR_W_INVP_3D || wo->mode == OR_W_ANGUP_3D) && !wo->up_bins) return -3;

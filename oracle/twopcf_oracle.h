@@ -36,6 +36,7 @@ typedef struct {
 	const float *iw;             /* Galaxy::input_weight (inverse-p builds), may be NULL */
 	const uint64_t *masks;       /* Galaxy::bitmask, n_words per object, may be NULL */
 	int n_words;
+	const uint64_t *dither;      /* Galaxy::dither_mask (_USE_DITHER_WEIGHTS), n_words per object, may be NULL */
 } or_cat;
 
 /* Pair-weight rules of the inverse-p build (correlators_invpweight.cpp). */
@@ -58,6 +59,9 @@ typedef struct {
 	const double *up_bins;   /* angular up-weight table (JKHists::full_hist.bins of DD/DDinvp) */
 	or_axis up_axis;
 	int pi_def;              /* OR_PI_* */
+	int dither;              /* _USE_DITHER_WEIGHTS: the numerator of OR_W_INVP_3D / OR_W_ANG_INVP is the
+	                            popcount of dither1 & dither2 (0 counts as 1) instead of n_runs,
+	                            correlators_invpweight.cpp:62-70, 749-757 */
 } or_wopt;
 
 enum { OR_MONO = 0, OR_SIGMA_PI = 1, OR_S_MU = 2, OR_ANGULAR = 3 };

@@ -52,9 +52,11 @@ DRIVER_INVP = os.path.join(ROOT, "two_pcf_amd", "twopcf_invp")
 N_RUNS = 128
 
 
-def _invp_inputs(seed=41):
+def _invp_inputs(seed=41, dither=False):
     """Equatorial clustered data with 2-word bit masks and weights, randoms
-    with all-ones masks (their IIP weight is unchanged), 2 regions."""
+    with all-ones masks (their IIP weight is unchanged), 2 regions; with
+    dither, sparse 2-word dither masks for both (many data pairs share no
+    dither bit)."""
     import numpy as np
     from two_pcf_amd import synth
     rng = np.random.Generator(np.random.PCG64(seed))
@@ -73,6 +75,11 @@ def _invp_inputs(seed=41):
         w = rng.uniform(0.5, 1.5, n).astype(np.float32)
         jk = (a > 185.0).astype(np.int32)
         out[tag] = dict(ra=a, dec=b, z=c, w=w, jk=jk, masks=masks)
+        if dither:
+            p = rng.uniform(0.0, 0.4, n)
+            bits = rng.random((n, N_RUNS)) < p[:, None]
+            out[tag]["dither"] = np.packbits(bits.astype(np.uint8), axis=1,
+                                             bitorder="little").view(np.uint64).reshape(n, 2)
     return out
 
 
@@ -95,7 +102,8 @@ INVP_PARAMS = dict(
 
 def _expected_invp(o, inp):
     """The inverse-p pipeline from the oracle: positions (cosmology table,
-    Galaxy::set), IIP weights (io.cpp:578-582), angular DD/DR with parent and
+    Galaxy::set), IIP weights (io.cpp:578-582, with dither masks
+    io.cpp:566-572), angular DD/DR with parent and
     inverse-p weights, up-weights DD/DDinvp and DR/DRinvp (divide_by_hist),
     3-D DD (inverse-p), DR (up-weight), RR (plain), invert/x2, normalise and
     the LS estimator, formatted as the reference's ascii writer does."""
@@ -108,9 +116,13 @@ def _expected_invp(o, inp):
         x, y, z, nx, ny, nz = o.eq_to_cart(a["ra"], a["dec"], d)
         tot = np.array([sum(int(int(v)).bit_count() for v in row) for row in a["masks"]])
         tot[tot == 0] = 1
-        w = np.where(tot != N_RUNS, (a["w"].astype(np.float64) * (N_RUNS / tot)).astype(np.float32), a["w"])
-        cats[tag] = o.Catalog(x, y, z, nx, ny, nz, w, a["jk"], iw=a["w"], masks=a["masks"])
+        num = N_RUNS
+        if "dither" in a:
+            num = np.array([sum(int(int(v)).bit_count() for v in row) for row in a["dither"]])
+        w = np.where(tot != N_RUNS, (a["w"].astype(np.float64) * (num / tot)).astype(np.float32), a["w"])
+        cats[tag] = o.Catalog(x, y, z, nx, ny, nz, w, a["jk"], iw=a["w"], masks=a["masks"], dither=a.get("dither"))
     D, R = cats["data"], cats["rand"]
+    dith = int("dither" in inp["data"])
     n_jk = p["n_jk_regions"]
     ang = o.axis(0.0, float(1.0 - np.cos(p["theta_max"])), p["theta_n_bins"], p["theta_log_base"])
     none = o.Axis()
@@ -120,7 +132,7 @@ def _expected_invp(o, inp):
         return o.finish(b is None, full, sub)
 
     add = fin(3, D, None, ang, None, o.WOpt(3, N_RUNS, None, none))[0]
-    addi = fin(3, D, None, ang, None, o.WOpt(4, N_RUNS, None, none))[0]
+    addi = fin(3, D, None, ang, None, o.WOpt(4, N_RUNS, None, none, 0, dith))[0]
     adr = fin(3, D, R, ang, None, o.WOpt(3, N_RUNS, None, none))[0]
     adri = fin(3, D, R, ang, None, o.WOpt(5, N_RUNS, None, none))[0]
     up_dd = np.ascontiguousarray(add / addi)
@@ -130,7 +142,7 @@ def _expected_invp(o, inp):
                              ("smu.txt", 2, [(0.0, 40.0, 10, 1.3), (0.0, 1.0, 5, 0.0)])):
         ax1 = o.axis(*axes[0])
         ax2 = o.axis(*axes[1]) if len(axes) > 1 else None
-        dd = fin(kind, D, None, ax1, ax2, o.WOpt(1, N_RUNS, up_dd.ctypes.data, ang))
+        dd = fin(kind, D, None, ax1, ax2, o.WOpt(1, N_RUNS, up_dd.ctypes.data, ang, 0, dith))
         dr = fin(kind, D, R, ax1, ax2, o.WOpt(2, N_RUNS, up_dr.ctypes.data, ang))
         rr = fin(kind, R, None, ax1, ax2, None)
         dd = o.normalise_auto(*dd, D)
@@ -169,6 +181,8 @@ def _write_invp_inputs(tmp_path, inp):
         with open(os.path.join(tmp_path, tag + ".txt"), "w") as f:
             for i in range(a["ra"].size):
                 m = " ".join(str(int(v)) for v in a["masks"][i])
+                if "dither" in a:
+                    m += " " + " ".join(str(int(v)) for v in a["dither"][i])
                 f.write("%.17g %.17g %.17g %.9g %d %s\n" % (a["ra"][i], a["dec"][i], a["z"][i], a["w"][i],
                                                           a["jk"][i], m))
     with open(os.path.join(tmp_path, "params.txt"), "w") as f:
@@ -176,17 +190,19 @@ def _write_invp_inputs(tmp_path, inp):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("dither", [False, True], ids=["bitwise", "dither"])
 @pytest.mark.parametrize("method", ["fp64", "fp32"])
-def test_driver_invp_pipeline(method, tmp_path, oracle_lib):
+def test_driver_invp_pipeline(method, dither, tmp_path, oracle_lib):
     """twopcf_invp (the inverse-p build of the driver) on ascii catalogues with
-    bit-mask columns: every output file equals the oracle's pipeline to the
-    6 significant digits of the ascii writer (a last-digit difference is
-    allowed: weighted sums are added in another order)."""
+    bit-mask columns (and dither-mask columns under TWOPCF_DITHER=1): every
+    output file equals the oracle's pipeline to the 6 significant digits of
+    the ascii writer (a last-digit difference is allowed: weighted sums are
+    added in another order)."""
     import numpy as np
-    print(f"[gpu] driver inverse-p {method}", flush=True)
-    inp = _invp_inputs()
+    print(f"[gpu] driver inverse-p {method} dither={dither}", flush=True)
+    inp = _invp_inputs(dither=dither)
     _write_invp_inputs(tmp_path, inp)
-    env = dict(os.environ, TWOPCF_METHOD=method)
+    env = dict(os.environ, TWOPCF_METHOD=method, TWOPCF_DITHER="1" if dither else "0")
     r = subprocess.run([DRIVER_INVP, "params.txt"], cwd=tmp_path, env=env, capture_output=True, text=True,
                        timeout=300)
     assert r.returncode == 0, r.stderr

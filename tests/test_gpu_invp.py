@@ -120,3 +120,48 @@ def test_angular_counts_all_subdivisions(oracle_lib, axis):
                 np.testing.assert_array_equal(r["full"], full)
                 np.testing.assert_array_equal(r["jk"], sub)
                 assert r["n_pairs"] == acc
+
+
+def test_dither_weights(oracle_lib):
+    """Dither weights (_USE_DITHER_WEIGHTS, correlators_invpweight.cpp:62-70,
+    749-757): the inverse-p DD numerator is popc(dither1 & dither2), 0
+    counting as 1.  Angular and 3-D s-mu DD with sparse 32-word dither masks
+    equal the oracle in both methods; the numerator really changes the sums;
+    a dither job on catalogues without dither masks is refused."""
+    from two_pcf_amd import lib as L
+    o = oracle_lib
+    D = make(L, 2000, 61)
+    rng = np.random.Generator(np.random.PCG64(62))
+    p = rng.uniform(0.0, 0.05, D["x"].size)
+    bits = rng.random((D["x"].size, WORDS * 64)) < p[:, None]
+    dm = np.packbits(bits.astype(np.uint8), axis=1, bitorder="little").view(np.uint64).reshape(-1, WORDS)
+    oc = o.Catalog(D["x"], D["y"], D["z"], D["nx"], D["ny"], D["nz"], D["w"], D["jk"], iw=D["iw"],
+                   masks=D["masks"], dither=dm)
+    ang = (0.0, float(1.0 - np.cos(0.03)), 16, 1.6)
+    oang = o.axis(*ang)
+    up = np.linspace(1.4, 1.0, 16)
+    axes = [(0.0, 40.0, 20, 1.3), (0.0, 1.0, 10, 0.0)]
+    ax1, ax2 = o.axis(*axes[0]), o.axis(*axes[1])
+    with L.Context(0) as ctx:
+        dc = ctx.upload(D["x"], D["y"], D["z"], D["nx"], D["ny"], D["nz"], D["w"], D["jk"], iw=D["iw"],
+                        masks=D["masks"], dither=dm)
+        plain = ctx.upload(D["x"], D["y"], D["z"], D["nx"], D["ny"], D["nz"], D["w"], D["jk"], iw=D["iw"],
+                           masks=D["masks"])
+        jobs = [dict(kind=L.ANGULAR, a=dc, axes=[ang], n_jk=2, weight_mode=L.W_ANG_INVP, n_runs=N_RUNS, dither=True),
+                dict(kind=L.S_MU, a=dc, axes=axes, n_jk=2, weight_mode=L.W_INVP_3D, n_runs=N_RUNS, up_bins=up,
+                     up_axis=ang, dither=True)]
+        want = [o.count(3, oc, None, oang, None, 2, wopt=o.WOpt(L.W_ANG_INVP, N_RUNS, None, o.Axis(), 0, 1)),
+                o.count(2, oc, None, ax1, ax2, 2, wopt=o.WOpt(L.W_INVP_3D, N_RUNS, up.ctypes.data, oang, 0, 1))]
+        for method in (0, 1):
+            print(f"[gpu] dither weights method={method}", flush=True)
+            res, _ = ctx.count(jobs, method=method)
+            for r, (full, sub, acc) in zip(res, want):
+                assert r["n_pairs"] == acc
+                np.testing.assert_allclose(r["full"], full, rtol=1e-12, atol=0)
+                np.testing.assert_allclose(r["jk"], sub, rtol=1e-12, atol=0)
+        # without the dither flag the same catalogue gives the n_runs numerator
+        res, _ = ctx.count([dict(j, dither=False) for j in jobs], method=1)
+        for r, (full, _, _) in zip(res, want):
+            assert np.abs(r["full"] - full).max() > 1e-3 * np.abs(full).max()
+        with pytest.raises(L.TwopcfError, match="dither"):
+            ctx.count([dict(jobs[0], a=plain)], method=1)

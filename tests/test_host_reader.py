@@ -120,6 +120,31 @@ def test_bit_masks_and_iip_weights(harness, tmp_path):
         assert float.fromhex(row[7]) == w0
 
 
+def test_dither_masks_and_iip_weights(harness, tmp_path):
+    """Dither weights: n_words dither-mask columns follow the bit masks; the
+    read weight is scaled by dither_total / total unless total is n_runs
+    (io.cpp:549-574); input_weight keeps the read weight."""
+    rows_in = [((3, 0), (1, 1)), ((2 ** 64 - 1, 2 ** 36 - 1), (7, 0)), ((0, 0), (0, 0)), ((5, 6), (0, 2 ** 63))]
+    text = "".join(f"{k + 1}.0 2.0 3.0 0.75 {m[0]} {m[1]} {d[0]} {d[1]}\n" for k, (m, d) in enumerate(rows_in))
+    p = dict(CART, n_jk_regions=0, n_bitwise_runs=100, bitwise_weight_dataset_name="BW",
+             dither_weight_dataset_name="DW")
+    r = run(harness, tmp_path, p, text, invp=2)
+    assert r.returncode == 0, r.stderr
+    n, nw, rows = parse(r.stdout)
+    assert n == 4 and nw == 2
+    w0 = strtof("0.75")
+    for row, (m, d) in zip(rows, rows_in):
+        assert [int(v) for v in row[10:14]] == [*m, *d]
+        tot = sum(bin(v).count("1") for v in m) or 1
+        dtot = sum(bin(v).count("1") for v in d)
+        want = w0 if tot == 100 else float(np.float32(w0 * (dtot / tot)))
+        assert float.fromhex(row[6]) == want
+        assert float.fromhex(row[7]) == w0
+    # the dither columns are required
+    r = run(harness, tmp_path, p, "1.0 2.0 3.0 0.75 3 0\n", invp=2)
+    assert r.returncode != 0 and "too few columns" in r.stderr
+
+
 def test_errors_are_reported(harness, tmp_path):
     r = run(harness, tmp_path, CART, "1 2 3 0.5\n")
     assert r.returncode != 0 and "too few columns" in r.stderr

@@ -24,6 +24,9 @@ int main(int argc, char *argv[]) {
 	twopcf_enable_inverse_p(TWOPCF_INVERSE_P);
 	const char *pi = std::getenv("TWOPCF_PI");
 	if (pi && std::string(pi) == "midpoint") twopcf_set_pi_definition(TWOPCF_PI_MIDPOINT);
+	// dither weights (_USE_DITHER_WEIGHTS, makefile:6) in the inverse-p pipeline
+	const char *dither = std::getenv("TWOPCF_DITHER");
+	if (TWOPCF_INVERSE_P && dither && std::string(dither) == "1") twopcf_enable_dither_weights(true);
 	GalaxyCatalogue *data = read_file(params.data_filename, params.data_file_type, params);
 	GalaxyCatalogue *randoms = read_file(params.random_filename, params.random_file_type, params);
 	if (TWOPCF_INVERSE_P) {

@@ -226,7 +226,8 @@ static int partition_valid(Galaxy *g, int n) {
 	return (int)(mid - g);
 }
 
-static twopcf_cat *upload(Galaxy *g, int n, const std::vector<uint64_t> *masks, int n_words) {
+static twopcf_cat *upload(Galaxy *g, int n, const std::vector<uint64_t> *masks, int n_words,
+                          const std::vector<uint64_t> *dither = nullptr) {
 	std::vector<double> c[6];
 	for (auto &v : c) v.resize(n);
 	std::vector<float> w(n), iw(n);
@@ -254,6 +255,7 @@ static twopcf_cat *upload(Galaxy *g, int n, const std::vector<uint64_t> *masks, 
 	if (masks && n_words > 0) {
 		d.masks = masks->data();
 		d.n_words = n_words;
+		if (dither && !dither->empty()) d.dither = dither->data();
 	}
 	twopcf_cat *cat = nullptr;
 	check(twopcf_catalog_upload(twopcf_default_ctx(), &d, &cat), "twopcf_catalog_upload");
@@ -274,19 +276,23 @@ static int partition_catalogue(GalaxyCatalogue *in) {
 	auto mid = std::stable_partition(order.begin(), order.end(), [&](int i) { return !(in->gals[i].d_comov < 0.0); });
 	const int n_valid = (int)(mid - order.begin());
 	std::vector<Galaxy> g(n);
-	std::vector<uint64_t> m((size_t)n * nw);
+	const bool dith = !in->dither.empty();
+	std::vector<uint64_t> m((size_t)n * nw), dm(dith ? (size_t)n * nw : 0);
 	for (int i = 0; i < n; ++i) {
 		g[i] = in->gals[order[i]];
 		for (int k = 0; k < nw; ++k) m[(size_t)i * nw + k] = in->masks[(size_t)order[i] * nw + k];
+		if (dith)
+			for (int k = 0; k < nw; ++k) dm[(size_t)i * nw + k] = in->dither[(size_t)order[i] * nw + k];
 	}
 	std::copy(g.begin(), g.end(), in->gals);
 	in->masks.swap(m);
+	in->dither.swap(dm);
 	return n_valid;
 }
 
 GalaxyBoxes::GalaxyBoxes(GalaxyCatalogue *in, GalaxyCatalogue *, double r) : rmax(r), source(in) {
 	n_gals = partition_catalogue(in);
-	cat = upload(in->gals, n_gals, &in->masks, in->n_words);
+	cat = upload(in->gals, n_gals, &in->masks, in->n_words, &in->dither);
 }
 
 // boxes.cpp:215-240: with require_redshift the objects without a distance go
@@ -299,7 +305,7 @@ GalaxyPixels::GalaxyPixels(Galaxy in[], int n, int, double t, bool require_redsh
 
 GalaxyPixels::GalaxyPixels(GalaxyCatalogue *in, int, double t, bool require_redshift) : theta_max(t) {
 	n_gals = require_redshift ? partition_catalogue(in) : in->n_gals;
-	cat = upload(in->gals, n_gals, &in->masks, in->n_words);
+	cat = upload(in->gals, n_gals, &in->masks, in->n_words, &in->dither);
 }
 
 GalaxyPixels::~GalaxyPixels() {
@@ -473,6 +479,7 @@ twopcf_job job_ang(int mode, const GalaxyPixels *a, const GalaxyPixels *b, JKHis
 	j.n_jk = h->n_jk_regions;
 	j.weight_mode = mode;
 	j.n_runs = n_runs;
+	j.dither = mode == TWOPCF_W_ANG_INVP && twopcf_dither_weights_enabled();
 	return j;
 }
 
@@ -497,6 +504,7 @@ struct UpWeight {
 	void apply(twopcf_job &j, int mode, int n_runs) const {
 		j.weight_mode = mode;
 		j.n_runs = n_runs;
+		j.dither = mode == TWOPCF_W_INVP_3D && twopcf_dither_weights_enabled();
 		j.up_bins = h->full_hist.bins;
 		j.up_axis = h->full_hist.ax.desc();
 	}
@@ -695,16 +703,26 @@ static bool g_inverse_p = false;
 void twopcf_enable_inverse_p(bool on) { g_inverse_p = on; }
 bool twopcf_inverse_p_enabled() { return g_inverse_p; }
 
-// io.cpp:521-588 (without dither weights)
+static bool g_dither = false;
+void twopcf_enable_dither_weights(bool on) { g_dither = on; }
+bool twopcf_dither_weights_enabled() { return g_dither; }
+
+// io.cpp:521-588: the input weight keeps the parent weight; the weight is
+// scaled by n_bitwise_runs / total, or with dither weights by
+// dither_total / total (io.cpp:549-574), when total != n_bitwise_runs
 void apply_bitwise_weights(GalaxyCatalogue *cat, const Params &p) {
 	const int nw = cat->n_words;
+	const bool dith = !cat->dither.empty();
 	for (int i = 0; i < cat->n_gals; ++i) {
-		int total = 0;
+		int total = 0, dither_total = 0;
 		for (int k = 0; k < nw; ++k) total += __builtin_popcountll(cat->masks[(size_t)i * nw + k]);
+		if (dith)
+			for (int k = 0; k < nw; ++k) dither_total += __builtin_popcountll(cat->dither[(size_t)i * nw + k]);
 		Galaxy &g = cat->gals[i];
 		g.input_weight = g.weight;
 		if (total == 0) total = 1;
-		if (total != p.n_bitwise_runs) g.weight *= (double)p.n_bitwise_runs / (double)total;
+		if (total != p.n_bitwise_runs)
+			g.weight *= (dith ? (double)dither_total : (double)p.n_bitwise_runs) / (double)total;
 	}
 }
 
@@ -910,18 +928,20 @@ GalaxyCatalogue *read_data_ascii(std::string filename, const Params &p) {
 		std::exit(1);
 	}
 	const int n_words = twopcf_inverse_p_enabled() ? (p.n_bitwise_runs + 63) / 64 : 0;
+	// with dither weights, n_words more columns follow: the dither mask
+	const int n_cols = n_words * (twopcf_dither_weights_enabled() ? 2 : 1);
 	Columns c;
 	c.c1.resize(n);
 	c.c2.resize(n);
 	c.c3.resize(n);
 	c.w.resize(n);
 	c.jk.resize(n);
-	c.masks.assign((size_t)n * n_words, 0);
+	c.masks.assign((size_t)n * n_cols, 0);
 	std::vector<int> fail_at(n_threads, -1), fail_code(n_threads, PARSE_OK);
 	auto work = [&](int t) {
 		const size_t b = (size_t)n * t / n_threads, e = (size_t)n * (t + 1) / n_threads;
 		for (size_t i = b; i < e; ++i) {
-			const int rc = parse_line(data.data() + lines[i].first, data.data() + lines[i].second, p, n_words, i, c);
+			const int rc = parse_line(data.data() + lines[i].first, data.data() + lines[i].second, p, n_cols, i, c);
 			if (rc) {
 				fail_at[t] = (int)i;
 				fail_code[t] = rc;
@@ -948,7 +968,17 @@ GalaxyCatalogue *read_data_ascii(std::string filename, const Params &p) {
 		}
 	}
 	cat->n_words = n_words;
-	cat->masks.swap(c.masks);
+	if (n_cols > n_words) {
+		cat->masks.resize((size_t)n * n_words);
+		cat->dither.resize((size_t)n * n_words);
+		for (size_t i = 0; i < (size_t)n; ++i)
+			for (int k = 0; k < n_words; ++k) {
+				cat->masks[i * n_words + k] = c.masks[i * n_cols + k];
+				cat->dither[i * n_words + k] = c.masks[i * n_cols + n_words + k];
+			}
+	} else {
+		cat->masks.swap(c.masks);
+	}
 	if (n_words > 0) apply_bitwise_weights(cat, p);
 	if (eq) {
 		// comoving distances and Galaxy::set, in chunks over the threads

@@ -15,7 +15,9 @@
 //     the bit masks as ceil(n_bitwise_runs / 64) extra integer columns (the
 //     reference reads them from the HDF5 datasets <name>0, <name>1, ...).
 //   * The inverse-p build (_USE_INV_WEIGHTS) is a run-time switch,
-//     twopcf_enable_inverse_p(), set by the twopcf_invp driver; the angular
+//     twopcf_enable_inverse_p(), set by the twopcf_invp driver, and so are
+//     dither weights (_USE_DITHER_WEIGHTS), twopcf_enable_dither_weights(),
+//     which the driver sets when TWOPCF_DITHER=1; the angular
 //     histograms
 //     calc_angular_upweight produces are kept in the process under their
 //     file names instead of HDF5 files (read_hdf5_hist, hist.cpp:115-178).
@@ -48,6 +50,7 @@ public:
 	Galaxy *gals;
 	int n_words = 0;
 	std::vector<uint64_t> masks;   // n_gals * n_words
+	std::vector<uint64_t> dither;  // Galaxy::dither_mask, n_gals * n_words (empty: none)
 	GalaxyCatalogue(int n_gals, int n_jk_regions);
 	~GalaxyCatalogue();
 	GalaxyCatalogue(const GalaxyCatalogue &) = delete;
@@ -241,6 +244,12 @@ bool twopcf_inverse_p_enabled();
 // io.cpp:521-588: input_weight = weight; weight *= n_runs / popcount(masks)
 // when the popcount differs from n_runs (0 counts as 1).
 void apply_bitwise_weights(GalaxyCatalogue *cat, const Params &params);
+// _USE_DITHER_WEIGHTS (makefile:6) as a run-time switch of the inverse-p
+// pipeline: catalogues carry dither masks (ascii: n_words more integer
+// columns after the bit masks), the read-time weight uses dither_total /
+// total and the inverse-p pair weights the dither popcount as numerator.
+void twopcf_enable_dither_weights(bool on);
+bool twopcf_dither_weights_enabled();
 
 // The angular histograms of calc_angular_upweight, by file name:
 // JKHists::output_hdf5 / read_hdf5_hist (hist.cpp:115-222) within one process.

@@ -64,6 +64,7 @@ struct CatDev {
 	const float *w, *iw;
 	const int *jk;
 	const uint64_t *masks;
+	int m_stride;       // mask words per object: the bit mask, then the dither mask if any
 	const int *start;   // cell starts [n_cell + 1]
 };
 
@@ -90,6 +91,7 @@ struct JobDev {
 	float mid_c;                                     // pair-midpoint pi: extra |dpi| / s per 1 / |x1 + x2|
 	int row_off, n_rows;                             // stencil rows in the row pool
 	int wmode, n_runs, n_words;                      // pair-weight rule (TWOPCF_W_*)
+	int dither;                                      // inverse-p numerator from the dither masks
 	int up_n;                                        // angular up-weight table
 	double up_min, up_max;
 	const double *up_bins, *up_centre;
@@ -196,13 +198,26 @@ __device__ double interp_up(const JobDev &J, double v) {
 	return b[seg] + (b[seg + 1] - b[seg]) * (v - c[seg]) / (c[seg + 1] - c[seg]);
 }
 
-__device__ __forceinline__ float popc_weight(const JobDev &J, int gi, int gj) {
-	const uint64_t *m1 = J.A.masks + (size_t)gi * J.n_words, *m2 = J.B.masks + (size_t)gj * J.n_words;
+// float(popcount(m1 & m2)) over n_words words at word offset off of each
+// object's masks (the bit mask at 0, the dither mask after it), 0 counting
+// as 1 (correlators_invpweight.cpp:55-59, 62-68).
+__device__ __forceinline__ float popc_and(const JobDev &J, int gi, int gj, int off) {
+	const uint64_t *m1 = J.A.masks + (size_t)gi * J.A.m_stride + off;
+	const uint64_t *m2 = J.B.masks + (size_t)gj * J.B.m_stride + off;
 	int c = 0;
 	for (int k = 0; k < J.n_words; ++k) c += __popcll(m1[k] & m2[k]);
 	float pc = (float)c;
 	if (pc < 0.5f) pc += 1.0f;
 	return pc;
+}
+
+__device__ __forceinline__ float popc_weight(const JobDev &J, int gi, int gj) { return popc_and(J, gi, gj, 0); }
+
+// The numerator of the inverse-p rules: n_runs, or under dither weights the
+// dither popcount (correlators_invpweight.cpp:62-70, 749-757).  Both are
+// integers, exact in float.
+__device__ __forceinline__ float invp_numerator(const JobDev &J, int gi, int gj) {
+	return J.dither ? popc_and(J, gi, gj, J.n_words) : (float)J.n_runs;
 }
 
 // Pair weight under the job's rule, in the reference's arithmetic
@@ -212,7 +227,7 @@ __device__ float pair_weight(const JobDev &J, int gi, int gj, const double *ni, 
 	switch (J.wmode) {
 	case TWOPCF_W_INVP_3D: {
 		const float pc = popc_weight(J, gi, gj);
-		double t = interp_up(J, ref_one_minus_cos(ni, nj)) * (double)J.n_runs;
+		double t = interp_up(J, ref_one_minus_cos(ni, nj)) * (double)invp_numerator(J, gi, gj);
 		t = t / (double)pc;
 		return J.A.iw[gi] * J.B.iw[gj] * (float)t;
 	}
@@ -224,7 +239,7 @@ __device__ float pair_weight(const JobDev &J, int gi, int gj, const double *ni, 
 		return J.A.iw[gi] * J.B.iw[gj];
 	case TWOPCF_W_ANG_INVP: {
 		const float pc = popc_weight(J, gi, gj);
-		return J.A.iw[gi] * J.B.iw[gj] * ((float)J.n_runs / pc);
+		return J.A.iw[gi] * J.B.iw[gj] * (invp_numerator(J, gi, gj) / pc);
 	}
 	case TWOPCF_W_ANG_W:
 	case TWOPCF_W_PLAIN:
@@ -240,7 +255,7 @@ __device__ float pair_weight_staged(const JobDev &J, int gi, int gj, const doubl
 	switch (J.wmode) {
 	case TWOPCF_W_INVP_3D: {
 		const float pc = popc_weight(J, gi, gj);
-		double t = interp_up(J, ref_one_minus_cos(ni, nj)) * (double)J.n_runs;
+		double t = interp_up(J, ref_one_minus_cos(ni, nj)) * (double)invp_numerator(J, gi, gj);
 		t = t / (double)pc;
 		return wp * (float)t;
 	}
@@ -248,7 +263,7 @@ __device__ float pair_weight_staged(const JobDev &J, int gi, int gj, const doubl
 		return (float)((double)wp * interp_up(J, ref_one_minus_cos(ni, nj)));
 	case TWOPCF_W_ANG_INVP: {
 		const float pc = popc_weight(J, gi, gj);
-		return wp * ((float)J.n_runs / pc);
+		return wp * (invp_numerator(J, gi, gj) / pc);
 	}
 	default:
 		return wp;
@@ -970,6 +985,7 @@ struct SortedCat {
 	float *w = nullptr, *iw = nullptr;
 	int *jk = nullptr;
 	uint64_t *masks = nullptr;
+	int m_stride = 0;
 	int *start = nullptr;
 	std::vector<int> start_h;
 	// regions present in each cell (ascending), CSR over cells (n_region > 1 only)
@@ -999,12 +1015,13 @@ struct twopcf_cat {
 	uint64_t id = 0;
 	int64_t n = 0;
 	int n_words = 0;
+	int m_stride = 0;                 // mask words per object: bit mask, then the dither mask if any
 	double *x = nullptr, *y = nullptr, *z = nullptr, *nx = nullptr, *ny = nullptr, *nz = nullptr;
 	float *w = nullptr, *iw = nullptr;
 	int *jk = nullptr;
 	uint64_t *masks = nullptr;
 	double lo[3], hi[3], nlo[3], nhi[3];
-	bool unit_w = true, has_iw = false;
+	bool unit_w = true, has_iw = false, has_dither = false;
 	int jk_min = 0, jk_max = 0;
 	std::map<GridKey, SortedCat> grids;
 };
@@ -1071,15 +1088,16 @@ int grid_catalog(twopcf_ctx *ctx, twopcf_cat *c, const GridKey &g, SortedCat **o
 	if ((rc = dalloc(ctx, &s.x, n)) || (rc = dalloc(ctx, &s.y, n)) || (rc = dalloc(ctx, &s.z, n)) ||
 	    (rc = dalloc(ctx, &s.nx, n)) || (rc = dalloc(ctx, &s.ny, n)) || (rc = dalloc(ctx, &s.nz, n)) ||
 	    (rc = dalloc(ctx, &s.w, n)) || (rc = dalloc(ctx, &s.iw, n)) || (rc = dalloc(ctx, &s.jk, n)) ||
-	    (rc = dalloc(ctx, &s.masks, (size_t)n * std::max(c->n_words, 0))) ||
+	    (rc = dalloc(ctx, &s.masks, (size_t)n * c->m_stride)) ||
 	    (rc = dalloc(ctx, &s.start, (size_t)n_cell + 1))) {
 		cleanup();
 		s.release();
 		return rc;
 	}
+	s.m_stride = c->m_stride;
 	if (n > 0) {
 		k_gather<<<nblk, 256, 0, ctx->stream>>>(perm, n, c->x, c->y, c->z, c->nx, c->ny, c->nz, c->w, c->iw, c->jk,
-		                                        c->masks, c->n_words, s.x, s.y, s.z, s.nx, s.ny, s.nz, s.w, s.iw,
+		                                        c->masks, c->m_stride, s.x, s.y, s.z, s.nx, s.ny, s.nz, s.w, s.iw,
 		                                        s.jk, s.masks);
 		HIPCK(ctx, hipGetLastError());
 	}
@@ -1218,7 +1236,7 @@ int twopcf_catalog_upload(twopcf_ctx *ctx, const twopcf_cat_desc *d, twopcf_cat 
 	if (d->n < 0 || d->n > INT32_MAX - 1) return fail(ctx, TWOPCF_E_LIMIT, "catalogue size out of range");
 	if (d->n > 0 && (!d->x || !d->y || !d->z || !d->nx || !d->ny || !d->nz))
 		return fail(ctx, TWOPCF_E_ARG, "positions and unit vectors are required");
-	if (d->n_words < 0 || d->n_words > 64 || (d->n_words > 0 && !d->masks))
+	if (d->n_words < 0 || d->n_words > 64 || (d->n_words > 0 && !d->masks) || (d->dither && d->n_words < 1))
 		return fail(ctx, TWOPCF_E_ARG, "bad mask description");
 	HIPCK(ctx, hipSetDevice(ctx->device));
 	auto *c = new twopcf_cat();
@@ -1226,7 +1244,19 @@ int twopcf_catalog_upload(twopcf_ctx *ctx, const twopcf_cat_desc *d, twopcf_cat 
 	c->id = ctx->next_cat_id++;
 	c->n = d->n;
 	c->n_words = d->n_words;
+	c->has_dither = d->dither != nullptr;
+	c->m_stride = d->n_words * (c->has_dither ? 2 : 1);
 	const int64_t n = d->n;
+	// masks on the device: per object the bit mask words, then the dither mask words
+	std::vector<uint64_t> mk;
+	if (c->has_dither) {
+		mk.resize((size_t)n * c->m_stride);
+		for (int64_t i = 0; i < n; ++i) {
+			std::copy(d->masks + i * d->n_words, d->masks + (i + 1) * d->n_words, mk.begin() + i * c->m_stride);
+			std::copy(d->dither + i * d->n_words, d->dither + (i + 1) * d->n_words,
+			          mk.begin() + i * c->m_stride + d->n_words);
+		}
+	}
 	std::vector<float> w(n, 1.0f), iw;
 	std::vector<int> jk(n, 0);
 	for (int k = 0; k < 3; ++k) {
@@ -1258,7 +1288,7 @@ int twopcf_catalog_upload(twopcf_ctx *ctx, const twopcf_cat_desc *d, twopcf_cat 
 	if ((rc = dalloc(ctx, &c->x, n)) || (rc = dalloc(ctx, &c->y, n)) || (rc = dalloc(ctx, &c->z, n)) ||
 	    (rc = dalloc(ctx, &c->nx, n)) || (rc = dalloc(ctx, &c->ny, n)) || (rc = dalloc(ctx, &c->nz, n)) ||
 	    (rc = dalloc(ctx, &c->w, n)) || (rc = dalloc(ctx, &c->iw, n)) || (rc = dalloc(ctx, &c->jk, n)) ||
-	    (rc = dalloc(ctx, &c->masks, (size_t)n * std::max(d->n_words, 0)))) {
+	    (rc = dalloc(ctx, &c->masks, (size_t)n * c->m_stride))) {
 		ctx->cats.push_back(c);
 		twopcf_catalog_free(c);
 		return rc;
@@ -1271,7 +1301,8 @@ int twopcf_catalog_upload(twopcf_ctx *ctx, const twopcf_cat_desc *d, twopcf_cat 
 	if ((rc = up(c->x, d->x, n * 8)) || (rc = up(c->y, d->y, n * 8)) || (rc = up(c->z, d->z, n * 8)) ||
 	    (rc = up(c->nx, d->nx, n * 8)) || (rc = up(c->ny, d->ny, n * 8)) || (rc = up(c->nz, d->nz, n * 8)) ||
 	    (rc = up(c->w, w.data(), n * 4)) || (rc = up(c->iw, iw.data(), n * 4)) || (rc = up(c->jk, jk.data(), n * 4)) ||
-	    (rc = up(c->masks, d->masks, (size_t)n * d->n_words * 8))) {
+	    (rc = up(c->masks, c->has_dither ? (const void *)mk.data() : (const void *)d->masks,
+	             (size_t)n * c->m_stride * 8))) {
 		twopcf_catalog_free(c);
 		return rc;
 	}
@@ -1354,6 +1385,10 @@ int build_plan(twopcf_ctx *ctx, const twopcf_job *jobs, int n_jobs, int method, 
 			if (J.a->n_words < 1 || J.a->n_words != bb->n_words)
 				return fail(ctx, TWOPCF_E_ARG, "inverse-p weights need bit masks of equal width in both catalogues");
 			if (J.n_runs <= 0) return fail(ctx, TWOPCF_E_ARG, "n_runs must be positive");
+			if (J.dither && (!J.a->has_dither || !bb->has_dither))
+				return fail(ctx, TWOPCF_E_ARG, "dither weights need dither masks in both catalogues");
+		} else if (J.dither) {
+			return fail(ctx, TWOPCF_E_ARG, "dither weights apply to the inverse-p rules only");
 		}
 		if (needs_up) {
 			if (!J.up_bins || J.up_axis.n_bins < 2 || J.up_axis.n_bins > MAX_AXIS_BINS)
@@ -1498,7 +1533,8 @@ int build_plan(twopcf_ctx *ctx, const twopcf_job *jobs, int n_jobs, int method, 
 		auto fill_cat = [](CatDev &cd, SortedCat *s) {
 			cd.x = s->x; cd.y = s->y; cd.z = s->z;
 			cd.nx = s->nx; cd.ny = s->ny; cd.nz = s->nz;
-			cd.w = s->w; cd.iw = s->iw; cd.jk = s->jk; cd.masks = s->masks; cd.start = s->start;
+			cd.w = s->w; cd.iw = s->iw; cd.jk = s->jk; cd.masks = s->masks; cd.m_stride = s->m_stride;
+			cd.start = s->start;
 		};
 		fill_cat(D.A, A);
 		fill_cat(D.B, B);
@@ -1660,6 +1696,7 @@ int build_plan(twopcf_ctx *ctx, const twopcf_job *jobs, int n_jobs, int method, 
 		D.wmode = H.j->weight_mode;
 		D.n_runs = H.j->n_runs;
 		D.n_words = H.a->n_words;
+		D.dither = H.j->dither ? 1 : 0;
 		if (!H.up_bins.empty()) {
 			double *ub = nullptr, *uc = nullptr;
 			float *ut = nullptr;
@@ -1998,6 +2035,7 @@ std::string job_set_key(const twopcf_job *jobs, int n_jobs, int method, std::vec
 		put(&J.weight_mode, sizeof J.weight_mode);
 		put(&J.n_runs, sizeof J.n_runs);
 		put(&J.pi_def, sizeof J.pi_def);
+		put(&J.dither, sizeof J.dither);
 		put(&J.up_axis, sizeof J.up_axis);
 		if (J.up_bins && J.up_axis.n_bins > 0) put(J.up_bins, sizeof(double) * J.up_axis.n_bins);
 	}

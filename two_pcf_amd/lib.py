@@ -14,7 +14,7 @@ MONOPOLE, SIGMA_PI, S_MU, ANGULAR = 0, 1, 2, 3
 PLAIN_FP64, FP32_RECHECK = 0, 1
 W_PLAIN, W_INVP_3D, W_ANGUP_3D, W_ANG_IW, W_ANG_INVP, W_ANG_W = range(6)
 PI_ANGULAR, PI_MIDPOINT = 0, 1
-ABI_VERSION = 2   # TWOPCF_ABI_VERSION of the structures below
+ABI_VERSION = 3   # TWOPCF_ABI_VERSION of the structures below
 
 EXPORTS = [
     "twopcf_abi_version", "twopcf_ctx_create", "twopcf_ctx_destroy", "twopcf_last_error",
@@ -29,7 +29,7 @@ class CatDesc(C.Structure):
                 ("x", C.c_void_p), ("y", C.c_void_p), ("z", C.c_void_p),
                 ("nx", C.c_void_p), ("ny", C.c_void_p), ("nz", C.c_void_p),
                 ("w", C.c_void_p), ("jk", C.c_void_p), ("iw", C.c_void_p),
-                ("masks", C.c_void_p), ("n_words", C.c_int32)]
+                ("masks", C.c_void_p), ("n_words", C.c_int32), ("dither", C.c_void_p)]
 
 
 class AxisDesc(C.Structure):
@@ -40,7 +40,7 @@ class JobDesc(C.Structure):
     _fields_ = [("kind", C.c_int32), ("a", C.c_void_p), ("b", C.c_void_p), ("ax", AxisDesc * 2),
                 ("n_jk", C.c_int32), ("weight_mode", C.c_int32), ("n_runs", C.c_int32),
                 ("up_bins", C.c_void_p), ("up_axis", AxisDesc), ("pi_def", C.c_int32),
-                ("full", C.c_void_p), ("jk", C.c_void_p), ("n_pairs", C.c_void_p),
+                ("dither", C.c_int32), ("full", C.c_void_p), ("jk", C.c_void_p), ("n_pairs", C.c_void_p),
                 ("n_rechecked", C.c_void_p)]
 
 
@@ -182,7 +182,7 @@ class Context:
         self.check(lib().twopcf_device_info(self.h, name, 64, C.byref(ncu), C.byref(lds)), "device_info")
         return dict(arch=name.value.decode(), n_cu=ncu.value, lds_per_block=lds.value)
 
-    def upload(self, x, y, z, nx, ny, nz, w=None, jk=None, iw=None, masks=None):
+    def upload(self, x, y, z, nx, ny, nz, w=None, jk=None, iw=None, masks=None, dither=None):
         f8 = lambda a: np.ascontiguousarray(a, dtype=np.float64)
         keep = [f8(x), f8(y), f8(z), f8(nx), f8(ny), f8(nz)]
         n = keep[0].size
@@ -190,10 +190,13 @@ class Context:
         ja = None if jk is None else np.ascontiguousarray(jk, dtype=np.int32)
         ia = None if iw is None else np.ascontiguousarray(iw, dtype=np.float32)
         ma = None if masks is None else np.ascontiguousarray(masks, dtype=np.uint64)
-        keep += [wa, ja, ia, ma]
+        da = None if dither is None else np.ascontiguousarray(dither, dtype=np.uint64)
+        if da is not None and (ma is None or da.shape != ma.shape):
+            raise ValueError("dither masks must have the shape of the bit masks")
+        keep += [wa, ja, ia, ma, da]
         p = lambda a: a.ctypes.data if a is not None else None
         d = CatDesc(n, *[p(a) for a in keep[:6]], p(wa), p(ja), p(ia), p(ma),
-                    0 if ma is None else ma.shape[1])
+                    0 if ma is None else ma.shape[1], p(da))
         h = C.c_void_p()
         self.check(lib().twopcf_catalog_upload(self.h, C.byref(d), C.byref(h)), "catalog_upload")
         cat = Catalog(self, h, n, None)
@@ -255,6 +258,7 @@ class Context:
             D.weight_mode = int(j.get("weight_mode", W_PLAIN))
             D.n_runs = int(j.get("n_runs", 0))
             D.pi_def = int(j.get("pi_def", PI_ANGULAR))
+            D.dither = int(bool(j.get("dither", False)))
             if j.get("up_bins") is not None:
                 ub = np.ascontiguousarray(j["up_bins"], dtype=np.float64)
                 o["_up"] = ub
