@@ -44,12 +44,13 @@ constexpr int BLOCK = WAVE * IG * JG;       // 1024 threads, 16 waves
 constexpr int ICHUNK = WAVE * IG;           // first-side objects per work item
 constexpr int TILE_FAST = 512;              // staged second-side objects (fp32 path)
 constexpr int TILE_EXACT = 512;             // staged second-side objects (fp64 path)
-constexpr int RING = 256;                   // per-wave queue of in-range pairs (fp32 path)
-constexpr int RQ = 128;                     // per-wave queue of pairs to re-evaluate exactly (fp32 path)
+constexpr int RING = 512;                   // per-wave queue of in-range pairs, u16 entries (fp32 path)
+constexpr int RQ = 192;                     // per-wave queue of pairs to re-evaluate exactly (fp32 path)
 __host__ __device__ constexpr int tile_of(bool fast) { return fast ? TILE_FAST : TILE_EXACT; }
-// tile + (fp32 path) first-side chunk and per-wave pair queues
+// tile + (fp32 path) first-side chunk with its two screening thresholds and
+// per-wave pair queues
 __host__ __device__ constexpr int tile_bytes(bool fast) {
-	return fast ? TILE_FAST * (2 * 16 + 4) + ICHUNK * 32 + (BLOCK / WAVE) * (RING * 4 + RQ * 8)
+	return fast ? TILE_FAST * (2 * 16 + 4) + ICHUNK * (32 + 8) + (BLOCK / WAVE) * (RING * 2 + RQ * 8)
 	            : TILE_EXACT * (6 * 8 + 4 + 4 + 4);
 }
 constexpr int MAX_SLOTS = 16;
@@ -92,6 +93,7 @@ struct JobDev {
 	int row_off, n_rows;                             // stencil rows in the row pool
 	int wmode, n_runs, n_words;                      // pair-weight rule (TWOPCF_W_*)
 	int dither;                                      // inverse-p numerator from the dither masks
+	float scr_eps;                                   // fp32 screening: bound on |s^2_mfma - s^2_fp32|
 	int up_n;                                        // angular up-weight table
 	double up_min, up_max;
 	const double *up_bins, *up_centre;
@@ -168,6 +170,8 @@ __global__ void k_fill_u64(unsigned long long *p, int64_t n, unsigned long long 
 struct Lds {
 	int tile, tables, slots, hist, total;
 };
+
+typedef float f32x4 __attribute__((ext_vector_type(4)));
 
 template <bool WEIGHTED>
 struct HistT;
@@ -518,13 +522,16 @@ __device__ void process_item_fast(const JobDev &J, const WorkItem &W, const int4
 	const bool stage_iw = wmode == TWOPCF_W_ANG_IW || wmode == TWOPCF_W_ANG_INVP || wmode == TWOPCF_W_INVP_3D;
 	const bool needs_more = wmode == TWOPCF_W_ANG_INVP || wmode == TWOPCF_W_INVP_3D || wmode == TWOPCF_W_ANGUP_3D;
 
-	float4 *tfa = (float4 *)(smem + L.tile);   // (o0, o1, o2, gj)
+	float4 *tfa = (float4 *)(smem + L.tile);   // (o0, o1, o2, |o|^2)
 	float4 *tfb = tfa + TILE;                  // (n0, n1, n2, w)
 	int *tslot = (int *)(tfb + TILE);
 	float4 *cia = (float4 *)(tslot + TILE);    // first side: (o0, o1, o2, w)
 	float4 *cib = cia + ICHUNK;                // (n0, n1, n2, region code)
-	unsigned *ring = (unsigned *)(cib + ICHUNK) + wave * RING;
-	int2 *rq = (int2 *)((unsigned *)(cib + ICHUNK) + (BLOCK / WAVE) * RING) + wave * RQ;   // (l2, gj)
+	float *cth = (float *)(cib + ICHUNK);      // screening thresholds per first-side object: upper
+	float *ctl = cth + ICHUNK;                 //   and lower
+	// ring entries: (tile entry << 6) | first-side object within the wave's 64
+	unsigned short *ring = (unsigned short *)(ctl + ICHUNK) + wave * RING;
+	int2 *rq = (int2 *)((unsigned short *)(ctl + ICHUNK) + (BLOCK / WAVE) * RING) + wave * RQ;   // (l2, gj)
 	float *tab = (float *)(smem + L.tables);
 	int *sreg = (int *)(smem + L.slots);
 	// the full histogram: 2^f_shift copies, bin-major, lane l adding to copy
@@ -535,6 +542,10 @@ __device__ void process_item_fast(const JobDev &J, const WorkItem &W, const int4
 	HT *H = F + (nb << f_shift);
 
 	for (int k = tid; k < n_slots; k += BLOCK) sreg[k] = slot_regions[W.slot_base + k];
+	// below s2_lo: certainly out of range; at or above s2_hi: certainly out
+	const float s2_lo = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(tab[J.tB_off])));
+	const float s2_hi = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(tab[J.tA_off + n1])));
+	const bool check_lo = s2_lo > 0.0f;       // with s_min = 0 every s^2 >= 0 passes
 	const int d1 = J.dim[1], d2 = J.dim[2];
 	const int cx = W.i_cell / (d1 * d2), cy = (W.i_cell / d2) % d1, cz = W.i_cell % d2;
 	const double c0 = J.lo[0] + (cx + 0.5) * J.h, c1 = J.lo[1] + (cy + 0.5) * J.h, c2 = J.lo[2] + (cz + 0.5) * J.h;
@@ -552,22 +563,22 @@ __device__ void process_item_fast(const JobDev &J, const WorkItem &W, const int4
 		const double *px = KIND == K_ANGULAR ? J.A.nx : J.A.x;
 		const double *py = KIND == K_ANGULAR ? J.A.ny : J.A.y;
 		const double *pz = KIND == K_ANGULAR ? J.A.nz : J.A.z;
-		cia[tid] = make_float4((float)(px[g] - c0), (float)(py[g] - c1), (float)(pz[g] - c2),
-		                       stage_iw ? J.A.iw[g] : J.A.w[g]);
+		const float ox = (float)(px[g] - c0), oy = (float)(py[g] - c1), oz = (float)(pz[g] - c2);
+		cia[tid] = make_float4(ox, oy, oz, stage_iw ? J.A.iw[g] : J.A.w[g]);
 		cib[tid] = make_float4((float)J.A.nx[g], (float)J.A.ny[g], (float)J.A.nz[g], __int_as_float(code));
+		// s^2 = |a|^2 + (|b|^2 - 2 a.b) may be in range iff the bracket lies
+		// within [s2_lo - eps - |a|^2, s2_hi + eps - |a|^2); objects past the
+		// end of the item never pass
+		const float aa = fmaf(oz, oz, fmaf(oy, oy, ox * ox));
+		const bool ok = tid < W.i_count;
+		cth[tid] = ok ? (s2_hi + J.scr_eps) - aa : -1e30f;
+		ctl[tid] = ok ? (s2_lo - J.scr_eps) - aa : 1e30f;
 	}
 	__syncthreads();
-	const int li = ig * WAVE + lane;
-	const unsigned long long valid_mask = __builtin_amdgcn_ballot_w64(li < W.i_count);
-	const int gi = W.i_begin + min(li, W.i_count - 1);
-	const float4 me = cia[li];
+	// row blocks of 16 first-side objects this wave screens (uniform)
+	const int rb_n = min(4, max(0, W.i_count - ig * WAVE + 15) >> 4);
 
 	const float4 *tq = (const float4 *)(tab + J.tq_off);
-	// below s2_lo: certainly out of range; at or above s2_hi: certainly out
-	const float s2_lo = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(tab[J.tB_off])));
-	const float s2_hi = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(tab[J.tA_off + n1])));
-	const bool check_lo = s2_lo > 0.0f;       // with s_min = 0 every s^2 >= 0 passes
-	const unsigned packed_li = (unsigned)li;
 
 	unsigned int n_acc = 0, n_rc = 0;
 
@@ -660,9 +671,11 @@ __device__ void process_item_fast(const JobDev &J, const WorkItem &W, const int4
 	const bool pi_mid = J.pi_def == TWOPCF_PI_MIDPOINT;
 	const float mid_c = J.mid_c;
 	const float c2x = (float)(2.0 * c0), c2y = (float)(2.0 * c1), c2z = (float)(2.0 * c2);
-	auto fp32_pi = [&](float dx, float dy, float dz, const float4 &ia, const float4 &ib, const float4 &a,
+	// (mid: the pi definition as a compile-time constant, std::true_type for
+	// the midpoint, so that the per-pair code has no branch on it)
+	auto fp32_pi = [&](auto mid, float dx, float dy, float dz, const float4 &ia, const float4 &ib, const float4 &a,
 	                   const float4 &b, float &inv_mid) {
-		if (pi_mid) {
+		if constexpr (decltype(mid)::value) {
 			const float sx = c2x + (ia.x + a.x), sy = c2y + (ia.y + a.y), sz = c2z + (ia.z + a.z);
 			inv_mid = __builtin_amdgcn_rsqf(fmaf(sz, sz, fmaf(sy, sy, sx * sx)));
 			return fmaf(dz, sz, fmaf(dy, sy, dx * sx)) * inv_mid;
@@ -670,53 +683,67 @@ __device__ void process_item_fast(const JobDev &J, const WorkItem &W, const int4
 		inv_mid = 0.0f;
 		return 0.5f * fmaf(dz, ib.z + b.z, fmaf(dy, ib.y + b.y, dx * (ib.x + b.x)));
 	};
-	// bin n queued pairs (lanes >= n idle)
-	auto consume = [&](unsigned head, unsigned n) {
-		__builtin_amdgcn_wave_barrier();
-		bool sure = false;
-		int bin = 0, l2 = 0, q = 0;
-		if (lane < (int)n) {
-			const unsigned e = ring[(head + lane) & (RING - 1)];
-			l2 = e & 255;
-			q = e >> 8;
-			const float4 ia = cia[l2], ib = cib[l2], a = tfa[q], b = tfb[q];
-			const float dx = ia.x - a.x, dy = ia.y - a.y, dz = ia.z - a.z;
-			const float s2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
-			bool in_range = true;
-			if constexpr (KIND == K_SIGMA_PI) {
-				// range test on s^2, sigma in the x = s^2 - pi^2 domain
-				// against sqrtf-domain thresholds, |pi| against its value
-				// thresholds; margins per pair from the bounds on s^2 and pi
-				float inv_mid;
-				const float p = fp32_pi(dx, dy, dz, ia, ib, a, b, inv_mid);
-				const float ap = fabsf(p);
-				const float s = __builtin_amdgcn_sqrtf(s2);
-				const float dpi = fmaf(J.pi_c1, s, J.pi_c2) + mid_c * s * inv_mid;
-				const float x = fmaf(-p, p, s2);
-				const float ex = fmaf(J.e_c1, s2, J.e_c2 * s) + dpi * fmaf(2.0f, ap, dpi) + 3.6e-7f * fmaf(p, p, s2);
-				const bool r_in = s2 >= rA0 && s2 < rBn;   // min_r_sq < s2_ref < max_r_sq
-				const float sg = __builtin_amdgcn_sqrtf(fmaxf(x, 0.0f)) - s_min;
-				int k1 = s_log ? (int)(s_ilb_ln2 * __builtin_amdgcn_logf(fmaf(s_ils, sg, 1.0f))) : (int)(sg * s_inv_step);
-				k1 = min(max(k1, 0), n1 - 1);
-				const bool sure1 = x >= x1t[k1] + ex && x < x1t[k1 + 1] - ex;
-				const bool out1 = x < x1t[0] - ex || x >= x1t[n1] + ex;
-				const float pv = ap - J.p_min;
-				int k2 = J.p_log ? (int)(J.p_ilb_ln2 * __builtin_amdgcn_logf(fmaf(J.p_ils, pv, 1.0f)))
-				                 : (int)(pv * J.p_inv_step);
-				k2 = min(max(k2, 0), n2 - 1);
-				const bool sure2 = ap >= t2t[k2] + dpi && ap < t2t[k2 + 1] - dpi;
-				const bool out2 = ap < t2t[0] - dpi || ap >= t2t[n2] + dpi;
-				if (r_in && (out1 || out2)) {
-					sure = true;   // in range but certainly outside the histogram
-					in_range = false;
-				} else {
-					sure = r_in && sure1 && sure2;
-				}
-				bin = k1 * n2 + k2;
+	// The FP32 decision for the pair of ring slot idx: bin, whether it is
+	// in range, and whether the margins prove both (sure); the ring holds
+	// pairs of the current tile, whose first entry is object t_base of the
+	// second side.
+	int t_base = 0;
+	struct PairEval {
+		bool sure, in_range;
+		int bin, l2, q;
+		float wp;     // product of the staged weights
+		int si;       // region code of the first-side object
+	};
+	auto eval = [&](auto mid, int l2, int q, const float4 &ia, const float4 &ib, const float4 &a, const float4 &b) {
+		PairEval r;
+		r.l2 = l2;
+		r.q = q;
+		r.wp = ia.w * b.w;
+		r.si = __float_as_int(ib.w);
+		const float dx = ia.x - a.x, dy = ia.y - a.y, dz = ia.z - a.z;
+		const float s2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
+		bool sure, in_range = true;
+		int bin;
+		if constexpr (KIND == K_SIGMA_PI) {
+			// range test on s^2, sigma in the x = s^2 - pi^2 domain
+			// against sqrtf-domain thresholds, |pi| against its value
+			// thresholds; margins per pair from the bounds on s^2 and pi
+			float inv_mid;
+			const float p = fp32_pi(mid, dx, dy, dz, ia, ib, a, b, inv_mid);
+			const float ap = fabsf(p);
+			const float s = __builtin_amdgcn_sqrtf(s2);
+			const float dpi = fmaf(J.pi_c1, s, J.pi_c2) + mid_c * s * inv_mid;
+			const float x = fmaf(-p, p, s2);
+			const float ex = fmaf(J.e_c1, s2, J.e_c2 * s) + dpi * fmaf(2.0f, ap, dpi) + 3.6e-7f * fmaf(p, p, s2);
+			const bool r_in = s2 >= rA0 && s2 < rBn;   // min_r_sq < s2_ref < max_r_sq
+			const float sg = __builtin_amdgcn_sqrtf(fmaxf(x, 0.0f)) - s_min;
+			int k1 = s_log ? (int)(s_ilb_ln2 * __builtin_amdgcn_logf(fmaf(s_ils, sg, 1.0f))) : (int)(sg * s_inv_step);
+			k1 = min(max(k1, 0), n1 - 1);
+			const bool sure1 = x >= x1t[k1] + ex && x < x1t[k1 + 1] - ex;
+			const bool out1 = x < x1t[0] - ex || x >= x1t[n1] + ex;
+			const float pv = ap - J.p_min;
+			int k2 = J.p_log ? (int)(J.p_ilb_ln2 * __builtin_amdgcn_logf(fmaf(J.p_ils, pv, 1.0f)))
+			                 : (int)(pv * J.p_inv_step);
+			k2 = min(max(k2, 0), n2 - 1);
+			const bool sure2 = ap >= t2t[k2] + dpi && ap < t2t[k2 + 1] - dpi;
+			const bool out2 = ap < t2t[0] - dpi || ap >= t2t[n2] + dpi;
+			if (r_in && (out1 || out2)) {
+				sure = true;   // in range but certainly outside the histogram
+				in_range = false;
 			} else {
+				sure = r_in && sure1 && sure2;
+			}
+			if (!(s2 < s2_hi && (!check_lo || s2 >= s2_lo))) {
+				sure = true;   // from the screening's error shell: certainly out of range
+				in_range = false;
+			}
+			bin = k1 * n2 + k2;
+		} else {
 			// bin: closed-form estimate, checked against the exact thresholds,
 			// proved by the error margins; the axis variable is s, or
-			// 1 - cos = chord^2 / 2 for angular counts
+			// 1 - cos = chord^2 / 2 for angular counts.  Pairs the screening
+			// let through from its error shell are certainly out of range.
+			in_range = s2 < s2_hi && (!check_lo || s2 >= s2_lo);
 			const float sv = (KIND == K_ANGULAR ? 0.5f * s2 : __builtin_amdgcn_sqrtf(s2)) - s_min;
 			int k = s_log ? (int)(s_ilb_ln2 * __builtin_amdgcn_logf(fmaf(s_ils, sv, 1.0f))) : (int)(sv * s_inv_step);
 			k = min(max(k, 0), n1 - 1);
@@ -724,7 +751,7 @@ __device__ void process_item_fast(const JobDev &J, const WorkItem &W, const int4
 			sure = (s2 >= q0.y) && (s2 < q1.z);   // T_k + E <= s2 < T_k+1 - E
 			if (KIND == K_S_MU) {
 				float inv_mid;
-				const float p = fp32_pi(dx, dy, dz, ia, ib, a, b, inv_mid);
+				const float p = fp32_pi(mid, dx, dy, dz, ia, ib, a, b, inv_mid);
 				const float rs = __builtin_amdgcn_rsqf(s2);
 				const float nmu = (float)n2;
 				const float xm = fabsf(p) * rs * nmu;
@@ -736,34 +763,75 @@ __device__ void process_item_fast(const JobDev &J, const WorkItem &W, const int4
 			} else {
 				bin = k;
 			}
-			}
-			if (sure && in_range) {
-				++n_acc;
-				add(bin, weight(l2, __float_as_int(a.w), ia.w * b.w), __float_as_int(ib.w), tslot[q]);
-			}
+			sure = sure || !in_range;
 		}
-		const unsigned long long mr = __builtin_amdgcn_ballot_w64(lane < (int)n && !sure);
-		if (mr) {
-			if (__builtin_amdgcn_inverse_ballot_w64(mr)) {
-				const unsigned off = __builtin_amdgcn_mbcnt_hi((unsigned)(mr >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)mr, 0u));
-				rq[rq_n + off] = make_int2(l2, __float_as_int(tfa[q].w));
-			}
-			rq_n += (unsigned)__popcll(mr);
+		r.sure = sure;
+		r.in_range = in_range;
+		r.bin = bin;
+		return r;
+	};
+	// bin n <= 2 * WAVE queued pairs, two per lane (independent chains for
+	// the scheduler to interleave); pairs the margins cannot decide go to the
+	// recheck queue
+	auto consume = [&](unsigned head, unsigned n) {
+		__builtin_amdgcn_wave_barrier();
+		const bool v0 = lane < (int)n, v1 = lane + WAVE < (int)n;
+		// both pairs' operands are loaded before either is evaluated
+		const unsigned e0 = ring[(head + lane) & (RING - 1)], e1 = ring[(head + WAVE + lane) & (RING - 1)];
+		const int l0 = ig * WAVE + (int)(e0 & 63u), q0 = (int)(e0 >> 6) & (TILE - 1);
+		const int l1 = ig * WAVE + (int)(e1 & 63u), q1 = (int)(e1 >> 6) & (TILE - 1);
+		const float4 ia0 = cia[l0], a0 = tfa[q0], ia1 = cia[l1], a1 = tfa[q1];
+		const float4 ib0 = cib[l0], b0 = tfb[q0], ib1 = cib[l1], b1 = tfb[q1];
+		PairEval p0, p1;
+		if ((KIND == K_S_MU || KIND == K_SIGMA_PI) && pi_mid) {
+			p0 = eval(std::true_type{}, l0, q0, ia0, ib0, a0, b0);
+			p1 = eval(std::true_type{}, l1, q1, ia1, ib1, a1, b1);
+		} else {
+			p0 = eval(std::false_type{}, l0, q0, ia0, ib0, a0, b0);
+			p1 = eval(std::false_type{}, l1, q1, ia1, ib1, a1, b1);
 		}
+		if (v0 && p0.sure && p0.in_range) {
+			++n_acc;
+			add(p0.bin, weight(p0.l2, t_base + p0.q, p0.wp), p0.si, tslot[p0.q]);
+		}
+		if (v1 && p1.sure && p1.in_range) {
+			++n_acc;
+			add(p1.bin, weight(p1.l2, t_base + p1.q, p1.wp), p1.si, tslot[p1.q]);
+		}
+		auto push = [&](bool c, const PairEval &p) {
+			const unsigned long long mr = __builtin_amdgcn_ballot_w64(c);
+			if (mr) {
+				if (__builtin_amdgcn_inverse_ballot_w64(mr)) {
+					const unsigned off = __builtin_amdgcn_mbcnt_hi((unsigned)(mr >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)mr, rq_n));
+					rq[off] = make_int2(p.l2, t_base + p.q);
+				}
+				rq_n += (unsigned)__popcll(mr);
+			}
+		};
+		push(v0 && !p0.sure, p0);
+		push(v1 && !p1.sure, p1);
 		__builtin_amdgcn_wave_barrier();
 		if (rq_n >= WAVE) flush_rq();
 	};
 
-	// append the lanes of mask m (pair: the lane's object, tile entry q)
+	// append the lanes of mask m with their ring entries
 	unsigned head = 0, tail = 0;
-	auto append = [&](unsigned long long m, int q) {
+	auto append = [&](unsigned long long m, unsigned entry) {
 		if (m == 0ull) return;
 		if (__builtin_amdgcn_inverse_ballot_w64(m)) {
-			const unsigned off = __builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u));
-			ring[(tail + off) & (RING - 1)] = ((unsigned)q << 8) | packed_li;
+			const unsigned off = __builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)m, tail));
+			ring[off & (RING - 1)] = (unsigned short)entry;
 		}
 		tail += (unsigned)__popcll(m);
 	};
+	// matrix-core screening: a 16 x 16 block of pairs per MFMA, rows = 16
+	// first-side objects (A: -2 o, 1), columns = 16 tile entries (B: o, |o|^2),
+	// D = |b|^2 - 2 a.b.  Lane l supplies A[l % 16][l / 16] and B[l / 16][l % 16]
+	// and receives D[4 (l / 16) + r][l % 16] in register r.
+	const int lrow = lane & 15, lk = lane >> 4;
+	const unsigned lane_code = ((unsigned)lrow << 6) | (unsigned)(4 * lk);
+	const int lane_d = lrow - 4 * lk;   // (column - row) within the block, for the order test
+	const float *tff = (const float *)tfa;
 
 	for (int r = 0; r < J.n_rows; ++r) {
 		if (!((W.row_mask >> r) & 1ull)) continue;
@@ -783,6 +851,7 @@ __device__ void process_item_fast(const JobDev &J, const WorkItem &W, const int4
 		                        ? __builtin_amdgcn_readfirstlane(J.B.start[W.i_cell + 1]) : jb;
 		for (int t0 = jb; t0 < je; t0 += TILE) {
 			const int nt = min(TILE, je - t0);
+			t_base = t0;
 			if (tid < nt) {
 				const int gj = t0 + tid;
 				int sl = 0;
@@ -795,41 +864,55 @@ __device__ void process_item_fast(const JobDev &J, const WorkItem &W, const int4
 				const double *px = KIND == K_ANGULAR ? J.B.nx : J.B.x;
 				const double *py = KIND == K_ANGULAR ? J.B.ny : J.B.y;
 				const double *pz = KIND == K_ANGULAR ? J.B.nz : J.B.z;
-				tfa[tid] = make_float4((float)(px[gj] - c0), (float)(py[gj] - c1), (float)(pz[gj] - c2),
-				                       __int_as_float(gj));
+				const float ox = (float)(px[gj] - c0), oy = (float)(py[gj] - c1), oz = (float)(pz[gj] - c2);
+				tfa[tid] = make_float4(ox, oy, oz, fmaf(oz, oz, fmaf(oy, oy, ox * ox)));
 				tfb[tid] = make_float4((float)J.B.nx[gj], (float)J.B.ny[gj], (float)J.B.nz[gj],
 				                       stage_iw ? J.B.iw[gj] : J.B.w[gj]);
 				tslot[tid] = sl;
 			}
 			__syncthreads();
-			if (valid_mask) {
+			if (rb_n > 0) {
 				const bool order = t0 < own_end;
-				// screening, two tile entries per step; the ring holds < 64
-				// pairs between steps and at most 191 within one
-				for (int q = jg; q < nt; q += 2 * jg_n) {
-					const int q1 = q + jg_n;
-					const bool has1 = q1 < nt;
-					const float4 a0 = tfa[q];
-					const float4 a1 = tfa[has1 ? q1 : q];
-					const float dx0 = me.x - a0.x, dy0 = me.y - a0.y, dz0 = me.z - a0.z;
-					const float dx1 = me.x - a1.x, dy1 = me.y - a1.y, dz1 = me.z - a1.z;
-					const float s20 = fmaf(dz0, dz0, fmaf(dy0, dy0, dx0 * dx0));
-					const float s21 = fmaf(dz1, dz1, fmaf(dy1, dy1, dx1 * dx1));
-					unsigned long long m0 = __builtin_amdgcn_ballot_w64(s20 < s2_hi) & valid_mask;
-					unsigned long long m1 = __builtin_amdgcn_ballot_w64(s21 < s2_hi) & (has1 ? valid_mask : 0ull);
-					if (check_lo) {
-						m0 &= __builtin_amdgcn_ballot_w64(s20 >= s2_lo);
-						m1 &= __builtin_amdgcn_ballot_w64(s21 >= s2_lo);
-					}
-					if (order) {
-						m0 &= __builtin_amdgcn_ballot_w64(__float_as_int(a0.w) > gi);
-						m1 &= __builtin_amdgcn_ballot_w64(__float_as_int(a1.w) > gi);
-					}
-					append(m0, q);
-					append(m1, q1);
-					while (tail - head >= WAVE) {
-						consume(head, WAVE);
-						head += WAVE;
+				for (int rb = 0; rb < rb_n; ++rb) {
+					const int o = ig * WAVE + rb * 16;   // first object of the row block
+					const float4 ca = cia[o + lrow];
+					const float av = lk == 0 ? -2.0f * ca.x : lk == 1 ? -2.0f * ca.y : lk == 2 ? -2.0f * ca.z : 1.0f;
+					const float4 th = *(const float4 *)&cth[o + 4 * lk];
+					const float4 tl = *(const float4 *)&ctl[o + 4 * lk];
+					// gj > gi  <=>  lane_d > i_begin + o + r - t0 - qb
+					const int ord0 = W.i_begin + o - t0;
+					// B operand of block qb: entries past the tile end get
+					// |b|^2 = 1e30 and are never in range; the next block's
+					// operand is loaded while this one is screened
+					auto b_load = [&](int qb) { return tff[4 * min(qb + lrow, TILE - 1) + lk]; };
+					const float b_past = lk == 3 ? 1e30f : 0.0f;
+					float b_next = b_load(jg * 16);
+					for (int qb = jg * 16; qb < nt; qb += jg_n * 16) {
+						const float bv = qb + lrow < nt ? b_next : b_past;
+						b_next = b_load(qb + jg_n * 16);
+						const f32x4 d = __builtin_amdgcn_mfma_f32_16x16x4f32(av, bv, f32x4{0.0f, 0.0f, 0.0f, 0.0f}, 0, 0, 0);
+						const unsigned base = ((unsigned)qb << 6) + (unsigned)(rb * 16) + lane_code;
+						unsigned long long m[4];
+						m[0] = __builtin_amdgcn_ballot_w64(d[0] < th.x);
+						m[1] = __builtin_amdgcn_ballot_w64(d[1] < th.y);
+						m[2] = __builtin_amdgcn_ballot_w64(d[2] < th.z);
+						m[3] = __builtin_amdgcn_ballot_w64(d[3] < th.w);
+						if (check_lo) {
+							m[0] &= __builtin_amdgcn_ballot_w64(d[0] >= tl.x);
+							m[1] &= __builtin_amdgcn_ballot_w64(d[1] >= tl.y);
+							m[2] &= __builtin_amdgcn_ballot_w64(d[2] >= tl.z);
+							m[3] &= __builtin_amdgcn_ballot_w64(d[3] >= tl.w);
+						}
+						if (order) {
+#pragma unroll
+							for (int r = 0; r < 4; ++r) m[r] &= __builtin_amdgcn_ballot_w64(lane_d > ord0 + r - qb);
+						}
+#pragma unroll
+						for (int r = 0; r < 4; ++r) append(m[r], base + (unsigned)r);
+						while (tail - head >= 2 * WAVE) {
+							consume(head, 2 * WAVE);
+							head += 2 * WAVE;
+						}
 					}
 				}
 				if (tail != head) {
@@ -1543,6 +1626,11 @@ int build_plan(twopcf_ctx *ctx, const twopcf_job *jobs, int n_jobs, int method, 
 			D.lo[k] = H.g.lo[k];
 		}
 		D.h = H.g.h;
+		// screening forms s^2 as |a|^2 + (|b|^2 - 2 a.b) with the matrix cores;
+		// offsets are within R per axis, so every term is below 6 R^2 and the
+		// FP32 roundings of this form and of the difference form stay far
+		// below 2^-15 R^2
+		D.scr_eps = (float)std::ldexp(std::pow(1.01 * (subdiv + 1) * H.g.h, 2), -15);
 		D.kind = H.j->kind;
 		D.is_auto = H.b == nullptr;
 		D.n1 = H.ax1.n;
