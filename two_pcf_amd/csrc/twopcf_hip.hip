@@ -692,14 +692,15 @@ __device__ void process_item_fast(const JobDev &J, const WorkItem &W, const int4
 		bool sure, in_range;
 		int bin, l2, q;
 		float wp;     // product of the staged weights
-		int si;       // region code of the first-side object
+		int si, sj;   // region codes of the two objects
 	};
 	auto eval = [&](auto mid, int l2, int q, const float4 &ia, const float4 &ib, const float4 &a, const float4 &b) {
 		PairEval r;
 		r.l2 = l2;
 		r.q = q;
-		r.wp = ia.w * b.w;
+		r.wp = WEIGHTED ? ia.w * b.w : 1.0f;
 		r.si = __float_as_int(ib.w);
+		r.sj = WEIGHTED ? tslot[q] : __float_as_int(b.w);
 		const float dx = ia.x - a.x, dy = ia.y - a.y, dz = ia.z - a.z;
 		const float s2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
 		bool sure, in_range = true;
@@ -712,7 +713,8 @@ __device__ void process_item_fast(const JobDev &J, const WorkItem &W, const int4
 			const float p = fp32_pi(mid, dx, dy, dz, ia, ib, a, b, inv_mid);
 			const float ap = fabsf(p);
 			const float s = __builtin_amdgcn_sqrtf(s2);
-			const float dpi = fmaf(J.pi_c1, s, J.pi_c2) + mid_c * s * inv_mid;
+			float dpi = fmaf(J.pi_c1, s, J.pi_c2);
+			if constexpr (decltype(mid)::value) dpi += mid_c * s * inv_mid;
 			const float x = fmaf(-p, p, s2);
 			const float ex = fmaf(J.e_c1, s2, J.e_c2 * s) + dpi * fmaf(2.0f, ap, dpi) + 3.6e-7f * fmaf(p, p, s2);
 			const bool r_in = s2 >= rA0 && s2 < rBn;   // min_r_sq < s2_ref < max_r_sq
@@ -755,7 +757,8 @@ __device__ void process_item_fast(const JobDev &J, const WorkItem &W, const int4
 				const float rs = __builtin_amdgcn_rsqf(s2);
 				const float nmu = (float)n2;
 				const float xm = fabsf(p) * rs * nmu;
-				const float del = fmaf(mu_c2, rs, mu_c1) + nmu * mid_c * inv_mid;
+				float del = fmaf(mu_c2, rs, mu_c1);
+				if constexpr (decltype(mid)::value) del += nmu * mid_c * inv_mid;
 				const int km = (int)xm;
 				const float fr = xm - (float)km;
 				sure = sure && (xm < nmu - del) && (fr <= 1.0f - del) && (km == 0 || fr >= del);
@@ -792,11 +795,11 @@ __device__ void process_item_fast(const JobDev &J, const WorkItem &W, const int4
 		}
 		if (v0 && p0.sure && p0.in_range) {
 			++n_acc;
-			add(p0.bin, weight(p0.l2, t_base + p0.q, p0.wp), p0.si, tslot[p0.q]);
+			add(p0.bin, weight(p0.l2, t_base + p0.q, p0.wp), p0.si, p0.sj);
 		}
 		if (v1 && p1.sure && p1.in_range) {
 			++n_acc;
-			add(p1.bin, weight(p1.l2, t_base + p1.q, p1.wp), p1.si, tslot[p1.q]);
+			add(p1.bin, weight(p1.l2, t_base + p1.q, p1.wp), p1.si, p1.sj);
 		}
 		auto push = [&](bool c, const PairEval &p) {
 			const unsigned long long mr = __builtin_amdgcn_ballot_w64(c);
@@ -866,8 +869,9 @@ __device__ void process_item_fast(const JobDev &J, const WorkItem &W, const int4
 				const double *pz = KIND == K_ANGULAR ? J.B.nz : J.B.z;
 				const float ox = (float)(px[gj] - c0), oy = (float)(py[gj] - c1), oz = (float)(pz[gj] - c2);
 				tfa[tid] = make_float4(ox, oy, oz, fmaf(oz, oz, fmaf(oy, oy, ox * ox)));
+				// unweighted jobs keep the region code where the weight would be
 				tfb[tid] = make_float4((float)J.B.nx[gj], (float)J.B.ny[gj], (float)J.B.nz[gj],
-				                       stage_iw ? J.B.iw[gj] : J.B.w[gj]);
+				                       WEIGHTED ? (stage_iw ? J.B.iw[gj] : J.B.w[gj]) : __int_as_float(sl));
 				tslot[tid] = sl;
 			}
 			__syncthreads();
@@ -884,13 +888,21 @@ __device__ void process_item_fast(const JobDev &J, const WorkItem &W, const int4
 					// B operand of block qb: entries past the tile end get
 					// |b|^2 = 1e30 and are never in range; the next block's
 					// operand is loaded while this one is screened
+					// software pipeline: the next block's operand is loaded and
+					// its product started while this block's masks are appended
 					auto b_load = [&](int qb) { return tff[4 * min(qb + lrow, TILE - 1) + lk]; };
 					const float b_past = lk == 3 ? 1e30f : 0.0f;
-					float b_next = b_load(jg * 16);
-					for (int qb = jg * 16; qb < nt; qb += jg_n * 16) {
-						const float bv = qb + lrow < nt ? b_next : b_past;
-						b_next = b_load(qb + jg_n * 16);
-						const f32x4 d = __builtin_amdgcn_mfma_f32_16x16x4f32(av, bv, f32x4{0.0f, 0.0f, 0.0f, 0.0f}, 0, 0, 0);
+					const int step = jg_n * 16;
+					auto product = [&](int qb, float raw) {
+						const float bv = qb + lrow < nt ? raw : b_past;
+						return __builtin_amdgcn_mfma_f32_16x16x4f32(av, bv, f32x4{0.0f, 0.0f, 0.0f, 0.0f}, 0, 0, 0);
+					};
+					f32x4 d_next = product(jg * 16, b_load(jg * 16));
+					float b_next = b_load(jg * 16 + step);
+					for (int qb = jg * 16; qb < nt; qb += step) {
+						const f32x4 d = d_next;
+						d_next = product(qb + step, b_next);
+						b_next = b_load(qb + 2 * step);
 						const unsigned base = ((unsigned)qb << 6) + (unsigned)(rb * 16) + lane_code;
 						unsigned long long m[4];
 						m[0] = __builtin_amdgcn_ballot_w64(d[0] < th.x);
