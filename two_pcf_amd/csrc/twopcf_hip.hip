@@ -46,11 +46,12 @@ constexpr int TILE_FAST = 512;              // staged second-side objects (fp32 
 constexpr int TILE_EXACT = 512;             // staged second-side objects (fp64 path)
 constexpr int RING = 512;                   // per-wave queue of in-range pairs, u16 entries (fp32 path)
 constexpr int RQ = 192;                     // per-wave queue of pairs to re-evaluate exactly (fp32 path)
+constexpr int REGION_MAP = 512;             // region -> slot code table (fp32 path, n_jk up to this)
 __host__ __device__ constexpr int tile_of(bool fast) { return fast ? TILE_FAST : TILE_EXACT; }
 // tile + (fp32 path) first-side chunk with its two screening thresholds and
 // per-wave pair queues
 __host__ __device__ constexpr int tile_bytes(bool fast) {
-	return fast ? TILE_FAST * (2 * 16 + 4) + ICHUNK * (32 + 8) + (BLOCK / WAVE) * (RING * 2 + RQ * 8)
+	return fast ? TILE_FAST * (2 * 16 + 4) + ICHUNK * (32 + 8) + (BLOCK / WAVE) * (RING * 2 + RQ * 8) + REGION_MAP * 4
 	            : TILE_EXACT * (6 * 8 + 4 + 4 + 4);
 }
 constexpr int MAX_SLOTS = 16;
@@ -532,6 +533,8 @@ __device__ void process_item_fast(const JobDev &J, const WorkItem &W, const int4
 	// ring entries: (tile entry << 6) | first-side object within the wave's 64
 	unsigned short *ring = (unsigned short *)(ctl + ICHUNK) + wave * RING;
 	int2 *rq = (int2 *)((unsigned short *)(ctl + ICHUNK) + (BLOCK / WAVE) * RING) + wave * RQ;   // (l2, gj)
+	// region -> code (s * nb for LDS slot s, else -(region + 1)) for this item
+	int *rmap = (int *)((int2 *)((unsigned short *)(ctl + ICHUNK) + (BLOCK / WAVE) * RING) + (BLOCK / WAVE) * RQ);
 	float *tab = (float *)(smem + L.tables);
 	int *sreg = (int *)(smem + L.slots);
 	// the full histogram: 2^f_shift copies, bin-major, lane l adding to copy
@@ -542,6 +545,21 @@ __device__ void process_item_fast(const JobDev &J, const WorkItem &W, const int4
 	HT *H = F + (nb << f_shift);
 
 	for (int k = tid; k < n_slots; k += BLOCK) sreg[k] = slot_regions[W.slot_base + k];
+	const bool use_map = jk && J.n_jk <= REGION_MAP;
+	if (use_map)
+		for (int r = tid; r < J.n_jk; r += BLOCK) {
+			int code = -(r + 1);
+			for (int k = 0; k < n_slots; ++k)
+				if (slot_regions[W.slot_base + k] == r) code = k * nb;
+			rmap[r] = code;
+		}
+	auto code_of = [&](int r) {
+		if (use_map) return rmap[r];
+		int code = -(r + 1);
+		for (int k = 0; k < n_slots; ++k)
+			if (sreg[k] == r) code = k * nb;
+		return code;
+	};
 	// below s2_lo: certainly out of range; at or above s2_hi: certainly out
 	const float s2_lo = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(tab[J.tB_off])));
 	const float s2_hi = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(tab[J.tA_off + n1])));
@@ -552,13 +570,7 @@ __device__ void process_item_fast(const JobDev &J, const WorkItem &W, const int4
 	__syncthreads();
 	if (tid < ICHUNK) {
 		const int g = W.i_begin + min(tid, W.i_count - 1);
-		const int r = J.A.jk[g];
-		int code = 0;
-		if (jk) {
-			code = -(r + 1);
-			for (int s = 0; s < n_slots; ++s)
-				if (sreg[s] == r) code = s * nb;
-		}
+		const int code = jk ? code_of(J.A.jk[g]) : 0;
 		// angular counts work on the unit vectors (1 - n1.n2 = |n1 - n2|^2 / 2)
 		const double *px = KIND == K_ANGULAR ? J.A.nx : J.A.x;
 		const double *py = KIND == K_ANGULAR ? J.A.ny : J.A.y;
@@ -644,13 +656,7 @@ __device__ void process_item_fast(const JobDev &J, const WorkItem &W, const int4
 				int bin;
 				if (exact_bin<KIND>(xi, ni, xj, nj, J.min_r_sq, J.max_r_sq, t_s2, t1, t2, J.ax1, J.ax2, bin, J.pi_def)) {
 					++n_acc;
-					int sj = 0;
-					if (jk) {
-						const int rj = J.B.jk[g2];
-						sj = -(rj + 1);
-						for (int s = 0; s < n_slots; ++s)
-							if (sreg[s] == rj) sj = s * nb;
-					}
+					const int sj = jk ? code_of(J.B.jk[g2]) : 0;
 					const float w = weight(e.x, g2, cia[e.x].w * (stage_iw ? J.B.iw[g2] : J.B.w[g2]));
 					add(bin, w, __float_as_int(cib[e.x].w), sj);
 				}
@@ -857,13 +863,7 @@ __device__ void process_item_fast(const JobDev &J, const WorkItem &W, const int4
 			t_base = t0;
 			if (tid < nt) {
 				const int gj = t0 + tid;
-				int sl = 0;
-				if (jk) {
-					const int rj = J.B.jk[gj];
-					sl = -(rj + 1);
-					for (int s = 0; s < n_slots; ++s)
-						if (sreg[s] == rj) sl = s * nb;
-				}
+				const int sl = jk ? code_of(J.B.jk[gj]) : 0;
 				const double *px = KIND == K_ANGULAR ? J.B.nx : J.B.x;
 				const double *py = KIND == K_ANGULAR ? J.B.ny : J.B.y;
 				const double *pz = KIND == K_ANGULAR ? J.B.nz : J.B.z;
