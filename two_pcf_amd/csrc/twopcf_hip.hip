@@ -174,6 +174,21 @@ struct Lds {
 
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
+// Global-memory views of device pointers.  Loads and atomics through them
+// compile to global_* instructions, which only the vector-memory counter
+// tracks; through generic pointers they are flat_* instructions, which the
+// LDS counter tracks too, so that every LDS wait after one also waits for
+// the global access to complete.
+#define GLOBAL_AS __attribute__((address_space(1)))
+template <class T>
+__device__ __forceinline__ const T GLOBAL_AS *gld(const T *p) { return (const T GLOBAL_AS *)p; }
+__device__ __forceinline__ void gadd(unsigned long long *p, unsigned long long v) {
+	__hip_atomic_fetch_add((unsigned long long GLOBAL_AS *)p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void gadd(double *p, double v) {
+	__hip_atomic_fetch_add((double GLOBAL_AS *)p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
 template <bool WEIGHTED>
 struct HistT;
 template <>
@@ -191,15 +206,16 @@ struct HistT<true> {
 // reference's double expression in the same order.
 __device__ double interp_up(const JobDev &J, double v) {
 	const int n = J.up_n;
-	if (v < J.up_min) return J.up_bins[0];
-	if (v > J.up_max) return J.up_bins[n - 1];
+	const double GLOBAL_AS *b = gld(J.up_bins), *c = gld(J.up_centre);
+	if (v < J.up_min) return b[0];
+	if (v > J.up_max) return b[n - 1];
 	const float fv = (float)v;
+	const float GLOBAL_AS *thr = gld(J.up_thr);
 	int cnt = 0;
-	for (int k = 0; k <= n; ++k) cnt += (J.up_thr[k] <= fv);
+	for (int k = 0; k <= n; ++k) cnt += (thr[k] <= fv);
 	const int seg = cnt - 1;
-	if (seg < 0) return J.up_bins[0];
-	if (seg >= n - 1) return J.up_bins[n - 1];
-	const double *b = J.up_bins, *c = J.up_centre;
+	if (seg < 0) return b[0];
+	if (seg >= n - 1) return b[n - 1];
 	return b[seg] + (b[seg + 1] - b[seg]) * (v - c[seg]) / (c[seg + 1] - c[seg]);
 }
 
@@ -207,8 +223,8 @@ __device__ double interp_up(const JobDev &J, double v) {
 // object's masks (the bit mask at 0, the dither mask after it), 0 counting
 // as 1 (correlators_invpweight.cpp:55-59, 62-68).
 __device__ __forceinline__ float popc_and(const JobDev &J, int gi, int gj, int off) {
-	const uint64_t *m1 = J.A.masks + (size_t)gi * J.A.m_stride + off;
-	const uint64_t *m2 = J.B.masks + (size_t)gj * J.B.m_stride + off;
+	const uint64_t GLOBAL_AS *m1 = gld(J.A.masks) + (size_t)gi * J.A.m_stride + off;
+	const uint64_t GLOBAL_AS *m2 = gld(J.B.masks) + (size_t)gj * J.B.m_stride + off;
 	int c = 0;
 	for (int k = 0; k < J.n_words; ++k) c += __popcll(m1[k] & m2[k]);
 	float pc = (float)c;
@@ -347,8 +363,8 @@ __device__ void process_item(const JobDev &J, const WorkItem &W, const int4 *__r
 			else atomicAdd((unsigned int *)&H[code * nb + bin], 1u);
 		} else {
 			const size_t o = (size_t)(-code - 1) * nb + bin;
-			if (WEIGHTED) atomicAdd(&J.jk_d[o], (double)w);
-			else atomicAdd(&J.jk_u[o], 1ull);
+			if (WEIGHTED) gadd(&J.jk_d[o], (double)w);
+			else gadd(&J.jk_u[o], 1ull);
 		}
 	};
 	auto add = [&](int bin, float w, int slot_j) {
@@ -461,8 +477,8 @@ __device__ void process_item(const JobDev &J, const WorkItem &W, const int4 *__r
 	for (int b = tid; b < nb; b += BLOCK) {
 		HT v = F[b];
 		if (v != (HT)0) {
-			if (WEIGHTED) atomicAdd(&J.full_d[b], (double)v);
-			else atomicAdd(&J.full_u[b], (unsigned long long)v);
+			if (WEIGHTED) gadd(&J.full_d[b], (double)v);
+			else gadd(&J.full_u[b], (unsigned long long)v);
 			F[b] = (HT)0;
 		}
 	}
@@ -472,8 +488,8 @@ __device__ void process_item(const JobDev &J, const WorkItem &W, const int4 *__r
 			if (v != (HT)0) {
 				const int s = e / nb, b = e - s * nb;
 				const size_t o = (size_t)sreg[s] * nb + b;
-				if (WEIGHTED) atomicAdd(&J.jk_d[o], (double)v);
-				else atomicAdd(&J.jk_u[o], (unsigned long long)v);
+				if (WEIGHTED) gadd(&J.jk_d[o], (double)v);
+				else gadd(&J.jk_u[o], (unsigned long long)v);
 				H[e] = (HT)0;
 			}
 		}
@@ -484,8 +500,8 @@ __device__ void process_item(const JobDev &J, const WorkItem &W, const int4 *__r
 		n_rc += __shfl_xor(n_rc, off);
 	}
 	if (lane == 0) {
-		if (n_acc) atomicAdd(&J.counters[0], (unsigned long long)n_acc);
-		if (n_rc) atomicAdd(&J.counters[1], (unsigned long long)n_rc);
+		if (n_acc) gadd(&J.counters[0], (unsigned long long)n_acc);
+		if (n_rc) gadd(&J.counters[1], (unsigned long long)n_rc);
 	}
 	__syncthreads();
 }
@@ -550,7 +566,7 @@ __device__ void process_item_fast(const JobDev &J, const WorkItem &W, const int4
 		for (int r = tid; r < J.n_jk; r += BLOCK) {
 			int code = -(r + 1);
 			for (int k = 0; k < n_slots; ++k)
-				if (slot_regions[W.slot_base + k] == r) code = k * nb;
+				if (gld(slot_regions)[W.slot_base + k] == r) code = k * nb;
 			rmap[r] = code;
 		}
 	auto code_of = [&](int r) {
@@ -570,14 +586,14 @@ __device__ void process_item_fast(const JobDev &J, const WorkItem &W, const int4
 	__syncthreads();
 	if (tid < ICHUNK) {
 		const int g = W.i_begin + min(tid, W.i_count - 1);
-		const int code = jk ? code_of(J.A.jk[g]) : 0;
+		const int code = jk ? code_of(gld(J.A.jk)[g]) : 0;
 		// angular counts work on the unit vectors (1 - n1.n2 = |n1 - n2|^2 / 2)
-		const double *px = KIND == K_ANGULAR ? J.A.nx : J.A.x;
-		const double *py = KIND == K_ANGULAR ? J.A.ny : J.A.y;
-		const double *pz = KIND == K_ANGULAR ? J.A.nz : J.A.z;
+		const double GLOBAL_AS *px = gld(KIND == K_ANGULAR ? J.A.nx : J.A.x);
+		const double GLOBAL_AS *py = gld(KIND == K_ANGULAR ? J.A.ny : J.A.y);
+		const double GLOBAL_AS *pz = gld(KIND == K_ANGULAR ? J.A.nz : J.A.z);
 		const float ox = (float)(px[g] - c0), oy = (float)(py[g] - c1), oz = (float)(pz[g] - c2);
-		cia[tid] = make_float4(ox, oy, oz, stage_iw ? J.A.iw[g] : J.A.w[g]);
-		cib[tid] = make_float4((float)J.A.nx[g], (float)J.A.ny[g], (float)J.A.nz[g], __int_as_float(code));
+		cia[tid] = make_float4(ox, oy, oz, stage_iw ? gld(J.A.iw)[g] : gld(J.A.w)[g]);
+		cib[tid] = make_float4((float)gld(J.A.nx)[g], (float)gld(J.A.ny)[g], (float)gld(J.A.nz)[g], __int_as_float(code));
 		// s^2 = |a|^2 + (|b|^2 - 2 a.b) may be in range iff the bracket lies
 		// within [s2_lo - eps - |a|^2, s2_hi + eps - |a|^2); objects past the
 		// end of the item never pass
@@ -602,8 +618,8 @@ __device__ void process_item_fast(const JobDev &J, const WorkItem &W, const int4
 			else atomicAdd((unsigned int *)&H[code + bin], 1u);
 		} else {
 			const size_t o = (size_t)(-code - 1) * nb + bin;
-			if (WEIGHTED) atomicAdd(&J.jk_d[o], (double)w);
-			else atomicAdd(&J.jk_u[o], 1ull);
+			if (WEIGHTED) gadd(&J.jk_d[o], (double)w);
+			else gadd(&J.jk_u[o], 1ull);
 		}
 	};
 	auto add = [&](int bin, float w, int si, int sj) {
@@ -630,8 +646,8 @@ __device__ void process_item_fast(const JobDev &J, const WorkItem &W, const int4
 	auto weight = [&](int l2, int g2, float wp) {
 		if (WEIGHTED && needs_more) {
 			const int g1 = W.i_begin + l2;
-			const double ni[3] = {J.A.nx[g1], J.A.ny[g1], J.A.nz[g1]};
-			const double nj[3] = {J.B.nx[g2], J.B.ny[g2], J.B.nz[g2]};
+			const double ni[3] = {gld(J.A.nx)[g1], gld(J.A.ny)[g1], gld(J.A.nz)[g1]};
+			const double nj[3] = {gld(J.B.nx)[g2], gld(J.B.ny)[g2], gld(J.B.nz)[g2]};
 			wp = pair_weight_staged(J, g1, g2, ni, nj, wp);
 		}
 		return wp;
@@ -646,18 +662,18 @@ __device__ void process_item_fast(const JobDev &J, const WorkItem &W, const int4
 				++n_rc;
 				const int2 e = rq[b + lane];
 				const int g1 = W.i_begin + e.x, g2 = e.y;
-				const double xi[3] = {J.A.x[g1], J.A.y[g1], J.A.z[g1]};
-				const double ni[3] = {J.A.nx[g1], J.A.ny[g1], J.A.nz[g1]};
-				const double xj[3] = {J.B.x[g2], J.B.y[g2], J.B.z[g2]};
-				const double nj[3] = {J.B.nx[g2], J.B.ny[g2], J.B.nz[g2]};
+				const double xi[3] = {gld(J.A.x)[g1], gld(J.A.y)[g1], gld(J.A.z)[g1]};
+				const double ni[3] = {gld(J.A.nx)[g1], gld(J.A.ny)[g1], gld(J.A.nz)[g1]};
+				const double xj[3] = {gld(J.B.x)[g2], gld(J.B.y)[g2], gld(J.B.z)[g2]};
+				const double nj[3] = {gld(J.B.nx)[g2], gld(J.B.ny)[g2], gld(J.B.nz)[g2]};
 				const float *t_s2 = tab + J.t_s2_off;
 				const float *t1 = tab + J.t1_off;
 				const float *t2 = tab + J.t2_off;
 				int bin;
 				if (exact_bin<KIND>(xi, ni, xj, nj, J.min_r_sq, J.max_r_sq, t_s2, t1, t2, J.ax1, J.ax2, bin, J.pi_def)) {
 					++n_acc;
-					const int sj = jk ? code_of(J.B.jk[g2]) : 0;
-					const float w = weight(e.x, g2, cia[e.x].w * (stage_iw ? J.B.iw[g2] : J.B.w[g2]));
+					const int sj = jk ? code_of(gld(J.B.jk)[g2]) : 0;
+					const float w = weight(e.x, g2, cia[e.x].w * (stage_iw ? gld(J.B.iw)[g2] : gld(J.B.w)[g2]));
 					add(bin, w, __float_as_int(cib[e.x].w), sj);
 				}
 			}
@@ -852,26 +868,26 @@ __device__ void process_item_fast(const JobDev &J, const WorkItem &W, const int4
 		if (is_auto && row.x == 0 && row.y == 0) z0 = max(z0, cz);
 		if (z0 > z1) continue;
 		const int base = (qx * d1 + qy) * d2;
-		const int jb = __builtin_amdgcn_readfirstlane(J.B.start[base + z0]);
-		const int je = __builtin_amdgcn_readfirstlane(J.B.start[base + z1 + 1]);
+		const int jb = __builtin_amdgcn_readfirstlane(gld(J.B.start)[base + z0]);
+		const int je = __builtin_amdgcn_readfirstlane(gld(J.B.start)[base + z1 + 1]);
 		// only the row through the first-side cell of an auto count holds
 		// pairs of the excluded half (cell-major order)
 		const int own_end = is_auto && row.x == 0 && row.y == 0
-		                        ? __builtin_amdgcn_readfirstlane(J.B.start[W.i_cell + 1]) : jb;
+		                        ? __builtin_amdgcn_readfirstlane(gld(J.B.start)[W.i_cell + 1]) : jb;
 		for (int t0 = jb; t0 < je; t0 += TILE) {
 			const int nt = min(TILE, je - t0);
 			t_base = t0;
 			if (tid < nt) {
 				const int gj = t0 + tid;
-				const int sl = jk ? code_of(J.B.jk[gj]) : 0;
-				const double *px = KIND == K_ANGULAR ? J.B.nx : J.B.x;
-				const double *py = KIND == K_ANGULAR ? J.B.ny : J.B.y;
-				const double *pz = KIND == K_ANGULAR ? J.B.nz : J.B.z;
+				const int sl = jk ? code_of(gld(J.B.jk)[gj]) : 0;
+				const double GLOBAL_AS *px = gld(KIND == K_ANGULAR ? J.B.nx : J.B.x);
+				const double GLOBAL_AS *py = gld(KIND == K_ANGULAR ? J.B.ny : J.B.y);
+				const double GLOBAL_AS *pz = gld(KIND == K_ANGULAR ? J.B.nz : J.B.z);
 				const float ox = (float)(px[gj] - c0), oy = (float)(py[gj] - c1), oz = (float)(pz[gj] - c2);
 				tfa[tid] = make_float4(ox, oy, oz, fmaf(oz, oz, fmaf(oy, oy, ox * ox)));
 				// unweighted jobs keep the region code where the weight would be
-				tfb[tid] = make_float4((float)J.B.nx[gj], (float)J.B.ny[gj], (float)J.B.nz[gj],
-				                       WEIGHTED ? (stage_iw ? J.B.iw[gj] : J.B.w[gj]) : __int_as_float(sl));
+				tfb[tid] = make_float4((float)gld(J.B.nx)[gj], (float)gld(J.B.ny)[gj], (float)gld(J.B.nz)[gj],
+				                       WEIGHTED ? (stage_iw ? gld(J.B.iw)[gj] : gld(J.B.w)[gj]) : __int_as_float(sl));
 				tslot[tid] = sl;
 			}
 			__syncthreads();
@@ -945,8 +961,8 @@ __device__ void process_item_fast(const JobDev &J, const WorkItem &W, const int4
 			F[(b << f_shift) + r] = (HT)0;
 		}
 		if (v != (HT)0) {
-			if (WEIGHTED) atomicAdd(&J.full_d[b], (double)v);
-			else atomicAdd(&J.full_u[b], (unsigned long long)v);
+			if (WEIGHTED) gadd(&J.full_d[b], (double)v);
+			else gadd(&J.full_u[b], (unsigned long long)v);
 		}
 	}
 	if (jk) {
@@ -955,8 +971,8 @@ __device__ void process_item_fast(const JobDev &J, const WorkItem &W, const int4
 			if (v != (HT)0) {
 				const int s_ = e / nb, b = e - s_ * nb;
 				const size_t o = (size_t)sreg[s_] * nb + b;
-				if (WEIGHTED) atomicAdd(&J.jk_d[o], (double)v);
-				else atomicAdd(&J.jk_u[o], (unsigned long long)v);
+				if (WEIGHTED) gadd(&J.jk_d[o], (double)v);
+				else gadd(&J.jk_u[o], (unsigned long long)v);
 				H[e] = (HT)0;
 			}
 		}
@@ -966,8 +982,8 @@ __device__ void process_item_fast(const JobDev &J, const WorkItem &W, const int4
 		n_rc += __shfl_xor(n_rc, off);
 	}
 	if (lane == 0) {
-		if (n_acc) atomicAdd(&J.counters[0], (unsigned long long)n_acc);
-		if (n_rc) atomicAdd(&J.counters[1], (unsigned long long)n_rc);
+		if (n_acc) gadd(&J.counters[0], (unsigned long long)n_acc);
+		if (n_rc) gadd(&J.counters[1], (unsigned long long)n_rc);
 	}
 	__syncthreads();
 }
