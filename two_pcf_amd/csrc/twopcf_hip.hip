@@ -514,7 +514,9 @@ __device__ void process_item(const JobDev &J, const WorkItem &W, const int4 *__r
 // host-computed margins.  Pairs the margins cannot decide go to the wave's
 // recheck queue (global indices), which is re-evaluated 64 at a time with
 // the exact pipeline, away from the screening loop.
-template <int KIND, bool WEIGHTED>
+// MID: the job's pi is the pair-midpoint definition (sigma-pi and s-mu),
+// chosen per item so that the per-pair code has no branch on it.
+template <int KIND, bool WEIGHTED, bool MID>
 __device__ void process_item_fast(const JobDev &J, const WorkItem &W, const int4 *__restrict__ rows,
                                   const int *__restrict__ slot_regions, char *smem, const Lds &L) {
 	using HT = typename HistT<WEIGHTED>::T;
@@ -690,11 +692,9 @@ __device__ void process_item_fast(const JobDev &J, const WorkItem &W, const int4
 	// line of sight in FP32: the angular definition from the float unit
 	// vectors, or the pair midpoint from 2c + the two cell-local offsets;
 	// inv_mid = 1 / |x1 + x2| for the midpoint (it widens the margins), else 0
-	const bool pi_mid = J.pi_def == TWOPCF_PI_MIDPOINT;
 	const float mid_c = J.mid_c;
 	const float c2x = (float)(2.0 * c0), c2y = (float)(2.0 * c1), c2z = (float)(2.0 * c2);
-	// (mid: the pi definition as a compile-time constant, std::true_type for
-	// the midpoint, so that the per-pair code has no branch on it)
+	// (mid: std::integral_constant<bool, MID>)
 	auto fp32_pi = [&](auto mid, float dx, float dy, float dz, const float4 &ia, const float4 &ib, const float4 &a,
 	                   const float4 &b, float &inv_mid) {
 		if constexpr (decltype(mid)::value) {
@@ -761,7 +761,7 @@ __device__ void process_item_fast(const JobDev &J, const WorkItem &W, const int4
 				sure = true;   // from the screening's error shell: certainly out of range
 				in_range = false;
 			}
-			bin = k1 * n2 + k2;
+			bin = __mul24(k1, n2) + k2;
 		} else {
 			// bin: closed-form estimate, checked against the exact thresholds,
 			// proved by the error margins; the axis variable is s, or
@@ -784,7 +784,7 @@ __device__ void process_item_fast(const JobDev &J, const WorkItem &W, const int4
 				const int km = (int)xm;
 				const float fr = xm - (float)km;
 				sure = sure && (xm < nmu - del) && (fr <= 1.0f - del) && (km == 0 || fr >= del);
-				bin = k * n2 + km;
+				bin = __mul24(k, n2) + km;
 			} else {
 				bin = k;
 			}
@@ -798,43 +798,54 @@ __device__ void process_item_fast(const JobDev &J, const WorkItem &W, const int4
 	// bin n <= 2 * WAVE queued pairs, two per lane (independent chains for
 	// the scheduler to interleave); pairs the margins cannot decide go to the
 	// recheck queue
+	auto push = [&](bool c, const PairEval &p) {
+		const unsigned long long mr = __builtin_amdgcn_ballot_w64(c);
+		if (mr) {
+			if (__builtin_amdgcn_inverse_ballot_w64(mr)) {
+				const unsigned off = __builtin_amdgcn_mbcnt_hi((unsigned)(mr >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)mr, rq_n));
+				rq[off] = make_int2(p.l2, t_base + p.q);
+			}
+			rq_n += (unsigned)__popcll(mr);
+		}
+	};
+	const std::integral_constant<bool, MID> mid;
 	auto consume = [&](unsigned head, unsigned n) {
 		__builtin_amdgcn_wave_barrier();
-		const bool v0 = lane < (int)n, v1 = lane + WAVE < (int)n;
-		// both pairs' operands are loaded before either is evaluated
-		const unsigned e0 = ring[(head + lane) & (RING - 1)], e1 = ring[(head + WAVE + lane) & (RING - 1)];
-		const int l0 = ig * WAVE + (int)(e0 & 63u), q0 = (int)(e0 >> 6) & (TILE - 1);
-		const int l1 = ig * WAVE + (int)(e1 & 63u), q1 = (int)(e1 >> 6) & (TILE - 1);
-		const float4 ia0 = cia[l0], a0 = tfa[q0], ia1 = cia[l1], a1 = tfa[q1];
-		const float4 ib0 = cib[l0], b0 = tfb[q0], ib1 = cib[l1], b1 = tfb[q1];
-		PairEval p0, p1;
-		if ((KIND == K_S_MU || KIND == K_SIGMA_PI) && pi_mid) {
-			p0 = eval(std::true_type{}, l0, q0, ia0, ib0, a0, b0);
-			p1 = eval(std::true_type{}, l1, q1, ia1, ib1, a1, b1);
-		} else {
-			p0 = eval(std::false_type{}, l0, q0, ia0, ib0, a0, b0);
-			p1 = eval(std::false_type{}, l1, q1, ia1, ib1, a1, b1);
-		}
-		if (v0 && p0.sure && p0.in_range) {
-			++n_acc;
-			add(p0.bin, weight(p0.l2, t_base + p0.q, p0.wp), p0.si, p0.sj);
-		}
-		if (v1 && p1.sure && p1.in_range) {
-			++n_acc;
-			add(p1.bin, weight(p1.l2, t_base + p1.q, p1.wp), p1.si, p1.sj);
-		}
-		auto push = [&](bool c, const PairEval &p) {
-			const unsigned long long mr = __builtin_amdgcn_ballot_w64(c);
-			if (mr) {
-				if (__builtin_amdgcn_inverse_ballot_w64(mr)) {
-					const unsigned off = __builtin_amdgcn_mbcnt_hi((unsigned)(mr >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)mr, rq_n));
-					rq[off] = make_int2(p.l2, t_base + p.q);
+		if constexpr (MID) {
+			// the midpoint's longer per-pair code: one pair per lane at a time
+			for (unsigned h = 0; h < n; h += WAVE) {
+				const bool v = h + lane < n;
+				const unsigned e = ring[(head + h + lane) & (RING - 1)];
+				const int l = ig * WAVE + (int)(e & 63u), q = (int)(e >> 6) & (TILE - 1);
+				const PairEval p = eval(mid, l, q, cia[l], cib[l], tfa[q], tfb[q]);
+				if (v && p.sure && p.in_range) {
+					++n_acc;
+					add(p.bin, weight(p.l2, t_base + p.q, p.wp), p.si, p.sj);
 				}
-				rq_n += (unsigned)__popcll(mr);
+				push(v && !p.sure, p);
+				__builtin_amdgcn_wave_barrier();
 			}
-		};
-		push(v0 && !p0.sure, p0);
-		push(v1 && !p1.sure, p1);
+		} else {
+			const bool v0 = lane < (int)n, v1 = lane + WAVE < (int)n;
+			// both pairs' operands are loaded before either is evaluated
+			const unsigned e0 = ring[(head + lane) & (RING - 1)], e1 = ring[(head + WAVE + lane) & (RING - 1)];
+			const int l0 = ig * WAVE + (int)(e0 & 63u), q0 = (int)(e0 >> 6) & (TILE - 1);
+			const int l1 = ig * WAVE + (int)(e1 & 63u), q1 = (int)(e1 >> 6) & (TILE - 1);
+			const float4 ia0 = cia[l0], a0 = tfa[q0], ia1 = cia[l1], a1 = tfa[q1];
+			const float4 ib0 = cib[l0], b0 = tfb[q0], ib1 = cib[l1], b1 = tfb[q1];
+			const PairEval p0 = eval(mid, l0, q0, ia0, ib0, a0, b0);
+			const PairEval p1 = eval(mid, l1, q1, ia1, ib1, a1, b1);
+			if (v0 && p0.sure && p0.in_range) {
+				++n_acc;
+				add(p0.bin, weight(p0.l2, t_base + p0.q, p0.wp), p0.si, p0.sj);
+			}
+			if (v1 && p1.sure && p1.in_range) {
+				++n_acc;
+				add(p1.bin, weight(p1.l2, t_base + p1.q, p1.wp), p1.si, p1.sj);
+			}
+			push(v0 && !p0.sure, p0);
+			push(v1 && !p1.sure, p1);
+		}
 		__builtin_amdgcn_wave_barrier();
 		if (rq_n >= WAVE) flush_rq();
 	};
@@ -991,7 +1002,7 @@ __device__ void process_item_fast(const JobDev &J, const WorkItem &W, const int4
 // One launch: item = blockIdx.x (plain) or pulled from a global queue
 // (persistent, queue != nullptr).  Every block leaves when the queue is
 // drained, so the grid always finishes.
-template <int KIND, bool FAST, bool WEIGHTED>
+template <int KIND, bool FAST, bool WEIGHTED, bool MID = false>
 __global__ void __launch_bounds__(BLOCK) k_count(const JobDev *__restrict__ jobs, const WorkItem *__restrict__ items,
                                                  int n_items, unsigned int *queue, const int4 *__restrict__ rows,
                                                  const int *__restrict__ slot_regions, const float *__restrict__ tables,
@@ -1015,10 +1026,12 @@ __global__ void __launch_bounds__(BLOCK) k_count(const JobDev *__restrict__ jobs
 		}
 		if (it >= n_items) break;
 		const WorkItem W = items[it];
-		if constexpr (FAST)
-			process_item_fast<KIND, WEIGHTED>(jobs[W.job], W, rows, slot_regions, smem, L);
-		else
+		if constexpr (FAST) {
+			// MID: every job of this launch has the pair-midpoint pi
+			process_item_fast<KIND, WEIGHTED, MID>(jobs[W.job], W, rows, slot_regions, smem, L);
+		} else {
 			process_item<KIND, false, WEIGHTED>(jobs[W.job], W, rows, slot_regions, smem, L);
+		}
 		if (!queue) break;
 	}
 }
@@ -2045,8 +2058,8 @@ int build_plan(twopcf_ctx *ctx, const twopcf_job *jobs, int n_jobs, int method, 
 		L.total = L.hist + (f_elems + nb * slots) * (weighted ? 8 : 4);
 		return L;
 	};
-	auto launch = [&](int kind, bool weighted, const WorkItem *it, int n_it, int nb, int slots, unsigned *queue,
-	                  int grid, int f_elems) -> int {
+	auto launch = [&](int kind, bool weighted, bool mid, const WorkItem *it, int n_it, int nb, int slots,
+	                  unsigned *queue, int grid, int f_elems) -> int {
 		const bool fast_kernel = fast;
 		Lds L = layout(fast_kernel, weighted, nb, slots, f_elems);
 		if ((size_t)L.total > ctx->lds_max)
@@ -2054,10 +2067,11 @@ int build_plan(twopcf_ctx *ctx, const twopcf_job *jobs, int n_jobs, int method, 
 		void (*fn)(const JobDev *, const WorkItem *, int, unsigned *, const int4 *, const int *, const float *, int,
 		           Lds) = nullptr;
 #define PICK(K, F, W) fn = k_count<K, F, W>
+#define PICK_PI(K, W) fn = mid ? k_count<K, true, W, true> : k_count<K, true, W, false>
 		if (fast) {
 			if (kind == K_MONO) { if (weighted) PICK(K_MONO, true, true); else PICK(K_MONO, true, false); }
-			else if (kind == K_S_MU) { if (weighted) PICK(K_S_MU, true, true); else PICK(K_S_MU, true, false); }
-			else if (kind == K_SIGMA_PI) { if (weighted) PICK(K_SIGMA_PI, true, true); else PICK(K_SIGMA_PI, true, false); }
+			else if (kind == K_S_MU) { if (weighted) PICK_PI(K_S_MU, true); else PICK_PI(K_S_MU, false); }
+			else if (kind == K_SIGMA_PI) { if (weighted) PICK_PI(K_SIGMA_PI, true); else PICK_PI(K_SIGMA_PI, false); }
 			else { if (weighted) PICK(K_ANGULAR, true, true); else PICK(K_ANGULAR, true, false); }
 		} else {
 			if (kind == K_MONO) { if (weighted) PICK(K_MONO, false, true); else PICK(K_MONO, false, false); }
@@ -2066,6 +2080,7 @@ int build_plan(twopcf_ctx *ctx, const twopcf_job *jobs, int n_jobs, int method, 
 			else { if (weighted) PICK(K_ANGULAR, false, true); else PICK(K_ANGULAR, false, false); }
 		}
 #undef PICK
+#undef PICK_PI
 		HIPCK(ctx, hipFuncSetAttribute((const void *)fn, hipFuncAttributeMaxDynamicSharedMemorySize, L.total));
 		if (n_it <= 0) return 0;
 		plan.launches.push_back(PlanLaunch{fn, grid, L, it, n_it, queue});
@@ -2079,23 +2094,27 @@ int build_plan(twopcf_ctx *ctx, const twopcf_job *jobs, int n_jobs, int method, 
 			while (k1 < items.size() && items[k1].job == q) ++k1;
 			int slots = 0;
 			for (size_t k = k0; k < k1; ++k) slots = std::max(slots, items[k].n_slots);
-			if ((rc = launch(hj[q].j->kind, prep[q].weighted, d_items + k0, (int)(k1 - k0), prep[q].nb, slots, nullptr,
-			                 (int)(k1 - k0), prep[q].f_elems))) {
+			if ((rc = launch(hj[q].j->kind, prep[q].weighted, false, d_items + k0, (int)(k1 - k0), prep[q].nb, slots,
+			                 nullptr, (int)(k1 - k0), prep[q].f_elems))) {
 				release();
 				return rc;
 			}
 			k0 = k1;
 		}
 	} else {
-		// batched: one persistent launch per (kind, weighted) class over the
-		// cost-sorted queue of all jobs in that class
-		std::map<std::pair<int, int>, std::vector<WorkItem>> cls;
-		for (auto &W : items) cls[{hj[W.job].j->kind, prep[W.job].weighted ? 1 : 0}].push_back(W);
+		// batched: one persistent launch per (kind, weighted, pi definition)
+		// class over the cost-sorted queue of all jobs in that class
+		auto mid_of = [&](int q) {
+			const int k = hj[q].j->kind;
+			return (k == TWOPCF_SIGMA_PI || k == TWOPCF_S_MU) && hj[q].j->pi_def == TWOPCF_PI_MIDPOINT ? 1 : 0;
+		};
+		std::map<std::tuple<int, int, int>, std::vector<WorkItem>> cls;
+		for (auto &W : items) cls[{hj[W.job].j->kind, prep[W.job].weighted ? 1 : 0, mid_of(W.job)}].push_back(W);
 		size_t off = 0;
 		std::vector<WorkItem> flat;
-		std::vector<std::tuple<int, int, size_t, size_t>> spans;
+		std::vector<std::tuple<int, int, size_t, size_t, int>> spans;
 		for (auto &kv : cls) {
-			spans.emplace_back(kv.first.first, kv.first.second, off, kv.second.size());
+			spans.emplace_back(std::get<0>(kv.first), std::get<1>(kv.first), off, kv.second.size(), std::get<2>(kv.first));
 			flat.insert(flat.end(), kv.second.begin(), kv.second.end());
 			off += kv.second.size();
 		}
@@ -2107,9 +2126,10 @@ int build_plan(twopcf_ctx *ctx, const twopcf_job *jobs, int n_jobs, int method, 
 		for (auto &sp : spans) {
 			const int kind = std::get<0>(sp);
 			const bool weighted = std::get<1>(sp) != 0;
+			const int mid = std::get<4>(sp);
 			int nb = 1, f_elems = 1;
 			for (int q = 0; q < n_jobs; ++q)
-				if (hj[q].j->kind == kind && prep[q].weighted == weighted) {
+				if (hj[q].j->kind == kind && prep[q].weighted == weighted && mid_of(q) == mid) {
 					nb = std::max(nb, hj[q].nb);
 					f_elems = std::max(f_elems, prep[q].f_elems);
 				}
@@ -2119,8 +2139,8 @@ int build_plan(twopcf_ctx *ctx, const twopcf_job *jobs, int n_jobs, int method, 
 			Lds L = layout(true, weighted, nb, slots, f_elems);
 			const int per_cu = std::max(1, std::min(8, (int)(ctx->lds_max / std::max(L.total, 1))));
 			const int grid = std::min<long long>((long long)ctx->n_cu * per_cu, (long long)std::get<3>(sp));
-			if ((rc = launch(kind, weighted, d_items + std::get<2>(sp), (int)std::get<3>(sp), nb, slots, d_queue + qi,
-			                 std::max(grid, 1), f_elems))) {
+			if ((rc = launch(kind, weighted, mid != 0, d_items + std::get<2>(sp), (int)std::get<3>(sp), nb, slots,
+			                 d_queue + qi, std::max(grid, 1), f_elems))) {
 				release();
 				return rc;
 			}
