@@ -582,6 +582,7 @@ __device__ void process_item_fast(const JobDev &J, const WorkItem &W, const int4
 	const float s2_lo = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(tab[J.tB_off])));
 	const float s2_hi = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(tab[J.tA_off + n1])));
 	const bool check_lo = s2_lo > 0.0f;       // with s_min = 0 every s^2 >= 0 passes
+	const float s2_lo_eff = check_lo ? s2_lo : -INFINITY;
 	const int d1 = J.dim[1], d2 = J.dim[2];
 	const int cx = W.i_cell / (d1 * d2), cy = (W.i_cell / d2) % d1, cz = W.i_cell % d2;
 	const double c0 = J.lo[0] + (cx + 0.5) * J.h, c1 = J.lo[1] + (cy + 0.5) * J.h, c2 = J.lo[2] + (cz + 0.5) * J.h;
@@ -684,8 +685,16 @@ __device__ void process_item_fast(const JobDev &J, const WorkItem &W, const int4
 		__builtin_amdgcn_wave_barrier();
 	};
 
-	const bool s_log = J.s_log != 0;
-	const float s_min = J.s_min, s_ilb_ln2 = J.s_ilb_ln2, s_ils = J.s_ils, s_inv_step = J.s_inv_step;
+	// closed-form bin estimates without a per-pair select between log and
+	// linear axes: k = (int)(inv_step v + ilb ln2 log2(1 + ils v)), with
+	// inv_step = 0 on a log axis and ils = ilb = 0 on a linear one (log2(1) = 0)
+	const bool s_log = J.s_log != 0, p_log = J.p_log != 0;
+	const float s_min = J.s_min;
+	const float s_ilb = s_log ? J.s_ilb_ln2 : 0.0f, s_ils = s_log ? J.s_ils : 0.0f, s_inv = s_log ? 0.0f : J.s_inv_step;
+	const float p_ilb = p_log ? J.p_ilb_ln2 : 0.0f, p_ils = p_log ? J.p_ils : 0.0f, p_inv = p_log ? 0.0f : J.p_inv_step;
+	auto bin_estimate = [](float v, float inv, float ilb, float ils) {
+		return (int)fmaf(inv, v, ilb * __builtin_amdgcn_logf(fmaf(ils, v, 1.0f)));
+	};
 	const float mu_c1 = J.mu_c1, mu_c2 = J.mu_c2;
 	const float *x1t = tab + J.x1_off, *t2t = tab + J.t2_off;
 	const float rA0 = tab[J.tA_off], rBn = tab[J.tB_off + n1];
@@ -741,13 +750,12 @@ __device__ void process_item_fast(const JobDev &J, const WorkItem &W, const int4
 			const float ex = fmaf(J.e_c1, s2, J.e_c2 * s) + dpi * fmaf(2.0f, ap, dpi) + 3.6e-7f * fmaf(p, p, s2);
 			const bool r_in = s2 >= rA0 && s2 < rBn;   // min_r_sq < s2_ref < max_r_sq
 			const float sg = __builtin_amdgcn_sqrtf(fmaxf(x, 0.0f)) - s_min;
-			int k1 = s_log ? (int)(s_ilb_ln2 * __builtin_amdgcn_logf(fmaf(s_ils, sg, 1.0f))) : (int)(sg * s_inv_step);
+			int k1 = bin_estimate(sg, s_inv, s_ilb, s_ils);
 			k1 = min(max(k1, 0), n1 - 1);
 			const bool sure1 = x >= x1t[k1] + ex && x < x1t[k1 + 1] - ex;
 			const bool out1 = x < x1t[0] - ex || x >= x1t[n1] + ex;
 			const float pv = ap - J.p_min;
-			int k2 = J.p_log ? (int)(J.p_ilb_ln2 * __builtin_amdgcn_logf(fmaf(J.p_ils, pv, 1.0f)))
-			                 : (int)(pv * J.p_inv_step);
+			int k2 = bin_estimate(pv, p_inv, p_ilb, p_ils);
 			k2 = min(max(k2, 0), n2 - 1);
 			const bool sure2 = ap >= t2t[k2] + dpi && ap < t2t[k2 + 1] - dpi;
 			const bool out2 = ap < t2t[0] - dpi || ap >= t2t[n2] + dpi;
@@ -757,7 +765,7 @@ __device__ void process_item_fast(const JobDev &J, const WorkItem &W, const int4
 			} else {
 				sure = r_in && sure1 && sure2;
 			}
-			if (!(s2 < s2_hi && (!check_lo || s2 >= s2_lo))) {
+			if (!(s2 < s2_hi && s2 >= s2_lo_eff)) {
 				sure = true;   // from the screening's error shell: certainly out of range
 				in_range = false;
 			}
@@ -767,9 +775,9 @@ __device__ void process_item_fast(const JobDev &J, const WorkItem &W, const int4
 			// proved by the error margins; the axis variable is s, or
 			// 1 - cos = chord^2 / 2 for angular counts.  Pairs the screening
 			// let through from its error shell are certainly out of range.
-			in_range = s2 < s2_hi && (!check_lo || s2 >= s2_lo);
+			in_range = s2 < s2_hi && s2 >= s2_lo_eff;
 			const float sv = (KIND == K_ANGULAR ? 0.5f * s2 : __builtin_amdgcn_sqrtf(s2)) - s_min;
-			int k = s_log ? (int)(s_ilb_ln2 * __builtin_amdgcn_logf(fmaf(s_ils, sv, 1.0f))) : (int)(sv * s_inv_step);
+			int k = bin_estimate(sv, s_inv, s_ilb, s_ils);
 			k = min(max(k, 0), n1 - 1);
 			const float4 q0 = tq[k], q1 = tq[k + 1];
 			sure = (s2 >= q0.y) && (s2 < q1.z);   // T_k + E <= s2 < T_k+1 - E
