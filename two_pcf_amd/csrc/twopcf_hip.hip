@@ -776,21 +776,31 @@ __device__ void process_item_fast(const JobDev &J, const WorkItem &W, const int4
 			// 1 - cos = chord^2 / 2 for angular counts.  Pairs the screening
 			// let through from its error shell are certainly out of range.
 			in_range = s2 < s2_hi && s2 >= s2_lo_eff;
-			const float sv = (KIND == K_ANGULAR ? 0.5f * s2 : __builtin_amdgcn_sqrtf(s2)) - s_min;
+			// s-mu takes s for its estimate as s2 rsq(s2) (s2 = 0 gives 0)
+			const float rs = KIND == K_S_MU ? __builtin_amdgcn_rsqf(fmaxf(s2, 1e-30f)) : 0.0f;
+			const float sv = (KIND == K_ANGULAR ? 0.5f * s2 : KIND == K_S_MU ? s2 * rs : __builtin_amdgcn_sqrtf(s2)) - s_min;
 			int k = bin_estimate(sv, s_inv, s_ilb, s_ils);
 			k = min(max(k, 0), n1 - 1);
 			const float4 q0 = tq[k], q1 = tq[k + 1];
 			sure = (s2 >= q0.y) && (s2 < q1.z);   // T_k + E <= s2 < T_k+1 - E
 			if (KIND == K_S_MU) {
-				float inv_mid;
-				const float p = fp32_pi(mid, dx, dy, dz, ia, ib, a, b, inv_mid);
-				const float rs = __builtin_amdgcn_rsqf(s2);
 				const float nmu = (float)n2;
-				const float xm = fabsf(p) * rs * nmu;
+				// mu n2 = |pi| rsq(s2) n2; the angular pi's factor 1/2 is taken
+				// into the scale (exact, so the product is the same float)
+				float inv_mid, pm, scale;
+				if constexpr (decltype(mid)::value) {
+					pm = fp32_pi(mid, dx, dy, dz, ia, ib, a, b, inv_mid);
+					scale = nmu;
+				} else {
+					inv_mid = 0.0f;
+					pm = fmaf(dz, ib.z + b.z, fmaf(dy, ib.y + b.y, dx * (ib.x + b.x)));
+					scale = 0.5f * nmu;
+				}
+				const float xm = fabsf(pm) * rs * scale;
 				float del = fmaf(mu_c2, rs, mu_c1);
 				if constexpr (decltype(mid)::value) del += nmu * mid_c * inv_mid;
 				const int km = (int)xm;
-				const float fr = xm - (float)km;
+				const float fr = __builtin_amdgcn_fractf(xm);
 				sure = sure && (xm < nmu - del) && (fr <= 1.0f - del) && (km == 0 || fr >= del);
 				bin = __mul24(k, n2) + km;
 			} else {
