@@ -53,9 +53,10 @@ enum {
 
 /* Counting methods. */
 enum {
-	/* The drop-in and the judge: one launch per count, the reference's cell
-	 * size (rmax) and 27-cell neighbourhood, and per pair the reference's own
-	 * mixed FP64/FP32 arithmetic (correlators.cpp:309-343) evaluated exactly. */
+	/* The drop-in, and the check for the batched method: one launch per
+	 * count, the reference's cell size (rmax) and 27-cell neighbourhood, and
+	 * per pair the reference's own mixed FP64/FP32 arithmetic
+	 * (correlators.cpp:309-343) evaluated exactly. */
 	TWOPCF_PLAIN_FP64 = 0,
 	/* Batched: every job of the call in one persistent launch over a
 	 * cost-sorted cell-pair work queue; cells subdivided; pair separations in
