@@ -45,7 +45,14 @@ constexpr int ICHUNK = WAVE * IG;           // first-side objects per work item
 constexpr int TILE_FAST = 512;              // staged second-side objects (fp32 path)
 constexpr int TILE_EXACT = 512;             // staged second-side objects (fp64 path)
 constexpr int RING = 512;                   // per-wave queue of in-range pairs, u16 entries (fp32 path)
-constexpr int RQ = 192;                     // per-wave queue of pairs to re-evaluate exactly (fp32 path)
+// pairs per lane in one binning pass of the fp32 path (three per lane
+// measured slower than two on the s-mu jackknife batch)
+__host__ __device__ constexpr int np_of(int, bool) { return 2; }
+constexpr int NP_MAX = 2;
+static_assert(RING >= NP_MAX * 64 + 4 * 64, "ring: pending pairs plus one screening step");
+// the ring holds < NP * 64 pending pairs plus one screening step's 256; the
+// recheck queue < 64 plus one binning pass's NP * 64
+constexpr int RQ = WAVE * (NP_MAX + 1);     // per-wave queue of pairs to re-evaluate exactly (fp32 path)
 constexpr int REGION_MAP = 512;             // region -> slot code table (fp32 path, n_jk up to this)
 __host__ __device__ constexpr int tile_of(bool fast) { return fast ? TILE_FAST : TILE_EXACT; }
 // tile + (fp32 path) first-side chunk with its two screening thresholds and
@@ -521,6 +528,7 @@ __device__ void process_item_fast(const JobDev &J, const WorkItem &W, const int4
                                   const int *__restrict__ slot_regions, char *smem, const Lds &L) {
 	using HT = typename HistT<WEIGHTED>::T;
 	constexpr int TILE = TILE_FAST;
+	constexpr int NP = np_of(KIND, WEIGHTED);   // pairs per lane in binning (MID: one)
 	const int tid = threadIdx.x, lane = tid & (WAVE - 1);
 	// wave index, made visibly uniform so that the loops below run on the
 	// scalar unit
@@ -844,25 +852,37 @@ __device__ void process_item_fast(const JobDev &J, const WorkItem &W, const int4
 				__builtin_amdgcn_wave_barrier();
 			}
 		} else {
-			const bool v0 = lane < (int)n, v1 = lane + WAVE < (int)n;
-			// both pairs' operands are loaded before either is evaluated
-			const unsigned e0 = ring[(head + lane) & (RING - 1)], e1 = ring[(head + WAVE + lane) & (RING - 1)];
-			const int l0 = ig * WAVE + (int)(e0 & 63u), q0 = (int)(e0 >> 6) & (TILE - 1);
-			const int l1 = ig * WAVE + (int)(e1 & 63u), q1 = (int)(e1 >> 6) & (TILE - 1);
-			const float4 ia0 = cia[l0], a0 = tfa[q0], ia1 = cia[l1], a1 = tfa[q1];
-			const float4 ib0 = cib[l0], b0 = tfb[q0], ib1 = cib[l1], b1 = tfb[q1];
-			const PairEval p0 = eval(mid, l0, q0, ia0, ib0, a0, b0);
-			const PairEval p1 = eval(mid, l1, q1, ia1, ib1, a1, b1);
-			if (v0 && p0.sure && p0.in_range) {
-				++n_acc;
-				add(p0.bin, weight(p0.l2, t_base + p0.q, p0.wp), p0.si, p0.sj);
+			// NP pairs per lane, all their operands loaded before any is
+			// evaluated (independent chains for the scheduler to interleave)
+			int l[NP], q[NP];
+			float4 ia[NP], ib[NP], a[NP], b[NP];
+#pragma unroll
+			for (int k = 0; k < NP; ++k) {
+				const unsigned e = ring[(head + k * WAVE + lane) & (RING - 1)];
+				l[k] = ig * WAVE + (int)(e & 63u);
+				q[k] = (int)(e >> 6) & (TILE - 1);
 			}
-			if (v1 && p1.sure && p1.in_range) {
-				++n_acc;
-				add(p1.bin, weight(p1.l2, t_base + p1.q, p1.wp), p1.si, p1.sj);
+#pragma unroll
+			for (int k = 0; k < NP; ++k) {
+				ia[k] = cia[l[k]];
+				a[k] = tfa[q[k]];
 			}
-			push(v0 && !p0.sure, p0);
-			push(v1 && !p1.sure, p1);
+#pragma unroll
+			for (int k = 0; k < NP; ++k) {
+				ib[k] = cib[l[k]];
+				b[k] = tfb[q[k]];
+			}
+			PairEval p[NP];
+#pragma unroll
+			for (int k = 0; k < NP; ++k) p[k] = eval(mid, l[k], q[k], ia[k], ib[k], a[k], b[k]);
+#pragma unroll
+			for (int k = 0; k < NP; ++k)
+				if (lane + k * WAVE < (int)n && p[k].sure && p[k].in_range) {
+					++n_acc;
+					add(p[k].bin, weight(p[k].l2, t_base + p[k].q, p[k].wp), p[k].si, p[k].sj);
+				}
+#pragma unroll
+			for (int k = 0; k < NP; ++k) push(lane + k * WAVE < (int)n && !p[k].sure, p[k]);
 		}
 		__builtin_amdgcn_wave_barrier();
 		if (rq_n >= WAVE) flush_rq();
@@ -966,9 +986,9 @@ __device__ void process_item_fast(const JobDev &J, const WorkItem &W, const int4
 						}
 #pragma unroll
 						for (int r = 0; r < 4; ++r) append(m[r], base + (unsigned)r);
-						while (tail - head >= 2 * WAVE) {
-							consume(head, 2 * WAVE);
-							head += 2 * WAVE;
+						while (tail - head >= NP * WAVE) {
+							consume(head, NP * WAVE);
+							head += NP * WAVE;
 						}
 					}
 				}
