@@ -42,7 +42,7 @@ constexpr int IG = 4;                       // wave groups over first-side objec
 constexpr int JG = 4;                       // wave groups over each staged tile
 constexpr int BLOCK = WAVE * IG * JG;       // 1024 threads, 16 waves
 constexpr int ICHUNK = WAVE * IG;           // first-side objects per work item
-constexpr int TILE_FAST = 512;              // staged second-side objects (fp32 path)
+constexpr int TILE_FAST = 768;              // staged second-side objects (fp32 path)
 constexpr int TILE_EXACT = 512;             // staged second-side objects (fp64 path)
 constexpr int RING = 512;                   // per-wave queue of in-range pairs, u16 entries (fp32 path)
 // pairs per lane in one binning pass of the fp32 path (three per lane
@@ -50,6 +50,7 @@ constexpr int RING = 512;                   // per-wave queue of in-range pairs,
 __host__ __device__ constexpr int np_of(int, bool) { return 2; }
 constexpr int NP_MAX = 2;
 static_assert(RING >= NP_MAX * 64 + 4 * 64, "ring: pending pairs plus one screening step");
+static_assert(TILE_FAST <= BLOCK && TILE_FAST <= 1024, "tile: one entry per thread, 10-bit ring index");
 // the ring holds < NP * 64 pending pairs plus one screening step's 256; the
 // recheck queue < 64 plus one binning pass's NP * 64
 constexpr int RQ = WAVE * (NP_MAX + 1);     // per-wave queue of pairs to re-evaluate exactly (fp32 path)
@@ -842,7 +843,7 @@ __device__ void process_item_fast(const JobDev &J, const WorkItem &W, const int4
 			for (unsigned h = 0; h < n; h += WAVE) {
 				const bool v = h + lane < n;
 				const unsigned e = ring[(head + h + lane) & (RING - 1)];
-				const int l = ig * WAVE + (int)(e & 63u), q = (int)(e >> 6) & (TILE - 1);
+				const int l = ig * WAVE + (int)(e & 63u), q = min((int)(e >> 6), TILE - 1);
 				const PairEval p = eval(mid, l, q, cia[l], cib[l], tfa[q], tfb[q]);
 				if (v && p.sure && p.in_range) {
 					++n_acc;
@@ -860,7 +861,7 @@ __device__ void process_item_fast(const JobDev &J, const WorkItem &W, const int4
 			for (int k = 0; k < NP; ++k) {
 				const unsigned e = ring[(head + k * WAVE + lane) & (RING - 1)];
 				l[k] = ig * WAVE + (int)(e & 63u);
-				q[k] = (int)(e >> 6) & (TILE - 1);
+				q[k] = min((int)(e >> 6), TILE - 1);
 			}
 #pragma unroll
 			for (int k = 0; k < NP; ++k) {
