@@ -42,7 +42,7 @@ constexpr int IG = 4;                       // wave groups over first-side objec
 constexpr int JG = 4;                       // wave groups over each staged tile
 constexpr int BLOCK = WAVE * IG * JG;       // 1024 threads, 16 waves
 constexpr int ICHUNK = WAVE * IG;           // first-side objects per work item
-constexpr int TILE_FAST = 768;              // staged second-side objects (fp32 path)
+constexpr int TILE_FAST = 1024;             // staged second-side objects (fp32 path)
 constexpr int TILE_EXACT = 512;             // staged second-side objects (fp64 path)
 constexpr int RING = 512;                   // per-wave queue of in-range pairs, u16 entries (fp32 path)
 // pairs per lane in one binning pass of the fp32 path (three per lane
@@ -51,6 +51,7 @@ __host__ __device__ constexpr int np_of(int, bool) { return 2; }
 constexpr int NP_MAX = 2;
 static_assert(RING >= NP_MAX * 64 + 4 * 64, "ring: pending pairs plus one screening step");
 static_assert(TILE_FAST <= BLOCK && TILE_FAST <= 1024, "tile: one entry per thread, 10-bit ring index");
+static_assert(ICHUNK <= 256, "first-side index in one byte of the recheck queue");
 // the ring holds < NP * 64 pending pairs plus one screening step's 256; the
 // recheck queue < 64 plus one binning pass's NP * 64
 constexpr int RQ = WAVE * (NP_MAX + 1);     // per-wave queue of pairs to re-evaluate exactly (fp32 path)
@@ -59,7 +60,7 @@ __host__ __device__ constexpr int tile_of(bool fast) { return fast ? TILE_FAST :
 // tile + (fp32 path) first-side chunk with its two screening thresholds and
 // per-wave pair queues
 __host__ __device__ constexpr int tile_bytes(bool fast) {
-	return fast ? TILE_FAST * (2 * 16 + 4) + ICHUNK * (32 + 8) + (BLOCK / WAVE) * (RING * 2 + RQ * 8) + REGION_MAP * 4
+	return fast ? TILE_FAST * (2 * 16 + 4) + ICHUNK * (32 + 8) + (BLOCK / WAVE) * (RING * 2 + RQ * 5) + REGION_MAP * 4
 	            : TILE_EXACT * (6 * 8 + 4 + 4 + 4);
 }
 constexpr int MAX_SLOTS = 16;
@@ -559,9 +560,13 @@ __device__ void process_item_fast(const JobDev &J, const WorkItem &W, const int4
 	float *ctl = cth + ICHUNK;                 //   and lower
 	// ring entries: (tile entry << 6) | first-side object within the wave's 64
 	unsigned short *ring = (unsigned short *)(ctl + ICHUNK) + wave * RING;
-	int2 *rq = (int2 *)((unsigned short *)(ctl + ICHUNK) + (BLOCK / WAVE) * RING) + wave * RQ;   // (l2, gj)
+	// recheck queue: second-side object (global index) and first-side object
+	int *rq_g = (int *)((unsigned short *)(ctl + ICHUNK) + (BLOCK / WAVE) * RING) + wave * RQ;
+	unsigned char *rq_l = (unsigned char *)((int *)((unsigned short *)(ctl + ICHUNK) + (BLOCK / WAVE) * RING) +
+	                                        (BLOCK / WAVE) * RQ) + wave * RQ;
 	// region -> code (s * nb for LDS slot s, else -(region + 1)) for this item
-	int *rmap = (int *)((int2 *)((unsigned short *)(ctl + ICHUNK) + (BLOCK / WAVE) * RING) + (BLOCK / WAVE) * RQ);
+	int *rmap = (int *)((unsigned char *)((int *)((unsigned short *)(ctl + ICHUNK) + (BLOCK / WAVE) * RING) +
+	                                      (BLOCK / WAVE) * RQ) + (BLOCK / WAVE) * RQ);
 	float *tab = (float *)(smem + L.tables);
 	int *sreg = (int *)(smem + L.slots);
 	// the full histogram: 2^f_shift copies, bin-major, lane l adding to copy
@@ -672,7 +677,7 @@ __device__ void process_item_fast(const JobDev &J, const WorkItem &W, const int4
 		for (unsigned b = 0; b < rq_n; b += WAVE) {
 			if (b + lane < rq_n) {
 				++n_rc;
-				const int2 e = rq[b + lane];
+				const int2 e = make_int2(rq_l[b + lane], rq_g[b + lane]);
 				const int g1 = W.i_begin + e.x, g2 = e.y;
 				const double xi[3] = {gld(J.A.x)[g1], gld(J.A.y)[g1], gld(J.A.z)[g1]};
 				const double ni[3] = {gld(J.A.nx)[g1], gld(J.A.ny)[g1], gld(J.A.nz)[g1]};
@@ -830,7 +835,8 @@ __device__ void process_item_fast(const JobDev &J, const WorkItem &W, const int4
 		if (mr) {
 			if (__builtin_amdgcn_inverse_ballot_w64(mr)) {
 				const unsigned off = __builtin_amdgcn_mbcnt_hi((unsigned)(mr >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)mr, rq_n));
-				rq[off] = make_int2(p.l2, t_base + p.q);
+				rq_l[off] = (unsigned char)p.l2;
+				rq_g[off] = t_base + p.q;
 			}
 			rq_n += (unsigned)__popcll(mr);
 		}
