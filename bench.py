@@ -31,15 +31,17 @@ sys.path.insert(0, ROOT)
 FLOP_PER_PAIR = 26            # FP32 flop per accepted s-mu pair
 # per accepted pair and correlator kind: monopole 15 (d 3, s^2 5, range 2,
 # sqrt 1, log index 3, weight 1), s-mu 26 (adds n1+n2 3, pi dot 5, x1/2 1,
-# mu divide 1, mu index 1, less nothing)
-FLOP_BY_KIND = {0: 15, 2: 26}
+# mu divide 1, mu index 1), sigma-pi 29 (monopole's without sqrt and log
+# index, plus n1+n2 3, pi dot 5, x1/2 1, s^2 - pi^2 2, sqrt 1, sigma log
+# index 3, |pi| index 3), angular 15 (chord 3, chord^2 5, range 2, x1/2 1,
+# log index 3, weight 1)
+FLOP_BY_KIND = {0: 15, 1: 29, 2: 26, 3: 15}
 # HBM bytes per launch of the counting kernel on the default workload (scale
 # 1, 100 regions) under the tuning the autotuner picks for it, from two
 # rocprofv3 --pmc runs: FETCH_SIZE x 2 + WRITE_SIZE x 1 (factors from a 1 GiB
 # copy), KiB -> B.  Source: profiles/k_count_fp32_jackknife_batch_hbm.md
-TRAFFIC_BYTES = {(1.0, 100, 2, 8, 256): (7.744e7 * 2 + 3.757e6) * 1024,
-                 (1.0, 100, 2, 12, 256): (7.867e7 * 2 + 2.725e6) * 1024,
-                 (1.0, 100, 2, 16, 256): (7.872e7 * 2 + 2.725e6) * 1024}
+TRAFFIC_BYTES = {(1.0, 100, 2, 8, 256): (7.789e7 * 2 + 3.758e6) * 1024,
+                 (1.0, 100, 2, 12, 256): (7.892e7 * 2 + 3.002e6) * 1024}
 PEAK_FP32_TFLOPS = 157.3      # MI355X dense FP32 peak (matrix cores; the vector peak is the same figure)
 AXES = [(0.0, 100.0, 40, 1.3), (0.0, 1.0, 50, 0.0)]
 
@@ -397,7 +399,7 @@ def main():
                      "traffic": TRAFFIC_BYTES.get(traffic_key),
                      "note": "FP32 compute bound (pair screening on the matrix cores, binning on the vector "
                              "ALUs; both FP32 peaks are 157.3 TFLOP/s); algorithmic flop per accepted pair: "
-                             "s-mu 26, monopole 15"},
+                             "s-mu 26, sigma-pi 29, monopole and angular 15"},
     }
     if not args.no_cpu and W["cpu"] is not None:
         try:
