@@ -263,3 +263,41 @@ def test_uneven_multi_catalogue_batch(ctx, oracle_lib, method):
         assert r["n_pairs"] == acc
     for c in dcs:
         c.free()
+
+
+@pytest.mark.parametrize("kind", [0, 1, 2])
+def test_dense_cells_multi_tile(ctx, oracle_lib, kind):
+    """Dense cells (about 1500 objects per cell of side rmax/2): every
+    stencil row spans several 1024-entry tiles, every cell several
+    256-object work items, and an auto count's own cell more than one tile,
+    so the tile loop and the auto-count order test run across tile
+    boundaries.  Points sit 1000 Mpc/h from the origin; 4 regions; both
+    methods equal the oracle."""
+    from two_pcf_amd import lib as L
+    o = oracle_lib
+    rng = np.random.Generator(np.random.PCG64(11 + kind))
+    axes = {0: [(0.0, 40.0, 20, 1.3)], 1: [(0.0, 30.0, 15, 0.0), (0.0, 25.0, 10, 0.0)],
+            2: [(0.0, 40.0, 20, 1.3), (0.0, 1.0, 10, 0.0)]}[kind]
+    cats, dc, oc = {}, {}, {}
+    for tag, n in (("a", 12000), ("b", 8000)):
+        p = 1000.0 + rng.uniform(0.0, 40.0, (n, 3))
+        x, y, z = (np.ascontiguousarray(p[:, k]) for k in range(3))
+        _, nx, ny, nz = L.cart_norm(x, y, z)
+        jk = ((x > 1020.0).astype(np.int32) + 2 * (y > 1020.0).astype(np.int32))
+        w = np.ones(n, np.float32)
+        dc[tag] = ctx.upload(x, y, z, nx, ny, nz, w, jk)
+        oc[tag] = o.Catalog(x, y, z, nx, ny, nz, w, jk)
+    oax = [o.axis(*a) for a in axes]
+    want = [o.count(kind, oc["a"], None, oax[0], oax[-1] if kind else None, 4),
+            o.count(kind, oc["a"], oc["b"], oax[0], oax[-1] if kind else None, 4)]
+    jobs = [dict(kind=kind, a=dc["a"], b=None, axes=axes, n_jk=4),
+            dict(kind=kind, a=dc["a"], b=dc["b"], axes=axes, n_jk=4)]
+    for method in (0, 1):
+        print(f"[gpu] dense cells kind={kind} method={method}", flush=True)
+        res, _ = ctx.count(jobs, method=method)
+        for r, (full, sub, acc) in zip(res, want):
+            assert r["n_pairs"] == acc
+            np.testing.assert_array_equal(r["full"], full)
+            np.testing.assert_array_equal(r["jk"], sub)
+    for c in dc.values():
+        c.free()
