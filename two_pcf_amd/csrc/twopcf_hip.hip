@@ -945,6 +945,10 @@ __device__ void process_item_fast(const JobDev &J, const WorkItem &W, const int4
 				tfb[tid] = make_float4((float)gld(J.B.nx)[gj], (float)gld(J.B.ny)[gj], (float)gld(J.B.nz)[gj],
 				                       WEIGHTED ? (stage_iw ? gld(J.B.iw)[gj] : gld(J.B.w)[gj]) : __int_as_float(sl));
 				tslot[tid] = sl;
+			} else if (tid < ((nt + 15) & ~15)) {
+				// the last 16-entry block is padded with entries that are
+				// never in range (|b|^2 = 1e30)
+				tfa[tid] = make_float4(0.0f, 0.0f, 0.0f, 1e30f);
 			}
 			__syncthreads();
 			if (rb_n > 0) {
@@ -957,23 +961,22 @@ __device__ void process_item_fast(const JobDev &J, const WorkItem &W, const int4
 					const float4 tl = *(const float4 *)&ctl[o + 4 * lk];
 					// gj > gi  <=>  lane_d > i_begin + o + r - t0 - qb
 					const int ord0 = W.i_begin + o - t0;
-					// B operand of block qb: entries past the tile end get
-					// |b|^2 = 1e30 and are never in range; the next block's
-					// operand is loaded while this one is screened
-					// software pipeline: the next block's operand is loaded and
-					// its product started while this block's masks are appended
-					auto b_load = [&](int qb) { return tff[4 * min(qb + lrow, TILE - 1) + lk]; };
-					const float b_past = lk == 3 ? 1e30f : 0.0f;
+					// Software pipeline: the next block's operand is loaded and
+					// its product started while this block's masks are
+					// appended.  The tile's last block is padded with entries
+					// never in range; the loads past it feed only the unused
+					// last product and stay inside the tile arrays (the tile
+					// is followed by its normals, TILE entries of 16 B).
+					auto b_load = [&](int qb) { return tff[4 * (qb + lrow) + lk]; };
 					const int step = jg_n * 16;
-					auto product = [&](int qb, float raw) {
-						const float bv = qb + lrow < nt ? raw : b_past;
+					auto product = [&](float bv) {
 						return __builtin_amdgcn_mfma_f32_16x16x4f32(av, bv, f32x4{0.0f, 0.0f, 0.0f, 0.0f}, 0, 0, 0);
 					};
-					f32x4 d_next = product(jg * 16, b_load(jg * 16));
+					f32x4 d_next = product(b_load(jg * 16));
 					float b_next = b_load(jg * 16 + step);
 					for (int qb = jg * 16; qb < nt; qb += step) {
 						const f32x4 d = d_next;
-						d_next = product(qb + step, b_next);
+						d_next = product(b_next);
 						b_next = b_load(qb + 2 * step);
 						const unsigned base = ((unsigned)qb << 6) + (unsigned)(rb * 16) + lane_code;
 						unsigned long long m[4];
