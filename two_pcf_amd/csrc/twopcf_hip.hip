@@ -710,6 +710,10 @@ __device__ void process_item_fast(const JobDev &J, const WorkItem &W, const int4
 		return (int)fmaf(inv, v, ilb * __builtin_amdgcn_logf(fmaf(ils, v, 1.0f)));
 	};
 	const float mu_c1 = J.mu_c1, mu_c2 = J.mu_c2;
+	// mu_c1 held in a vector register: the margin's multiply-add then reads
+	// one scalar operand, with no per-pair copy
+	float mu_c1_v;
+	asm volatile("v_mov_b32 %0, %1" : "=v"(mu_c1_v) : "s"(mu_c1));
 	const float *x1t = tab + J.x1_off, *t2t = tab + J.t2_off;
 	const float rA0 = tab[J.tA_off], rBn = tab[J.tB_off + n1];
 	// line of sight in FP32: the angular definition from the float unit
@@ -790,9 +794,12 @@ __device__ void process_item_fast(const JobDev &J, const WorkItem &W, const int4
 			// 1 - cos = chord^2 / 2 for angular counts.  Pairs the screening
 			// let through from its error shell are certainly out of range.
 			in_range = s2 < s2_hi && s2 >= s2_lo_eff;
-			// s-mu takes s for its estimate as s2 rsq(s2) (s2 = 0 gives 0)
-			const float rs = KIND == K_S_MU ? __builtin_amdgcn_rsqf(fmaxf(s2, 1e-30f)) : 0.0f;
-			const float sv = (KIND == K_ANGULAR ? 0.5f * s2 : KIND == K_S_MU ? s2 * rs : __builtin_amdgcn_sqrtf(s2)) - s_min;
+			// s-mu takes s for its estimate as s2 rsq(s2); the 1e-30 keeps
+			// s2 = 0 finite and does not change any s2 above 1e-23
+			const float rs = KIND == K_S_MU ? __builtin_amdgcn_rsqf(s2 + 1e-30f) : 0.0f;
+			const float sv = KIND == K_ANGULAR ? fmaf(0.5f, s2, -s_min)
+			               : KIND == K_S_MU    ? fmaf(s2, rs, -s_min)
+			                                   : __builtin_amdgcn_sqrtf(s2) - s_min;
 			int k = bin_estimate(sv, s_inv, s_ilb, s_ils);
 			k = min(max(k, 0), n1 - 1);
 			const float4 q0 = tq[k], q1 = tq[k + 1];
@@ -811,7 +818,7 @@ __device__ void process_item_fast(const JobDev &J, const WorkItem &W, const int4
 					scale = 0.5f * nmu;
 				}
 				const float xm = fabsf(pm) * rs * scale;
-				float del = fmaf(mu_c2, rs, mu_c1);
+				float del = fmaf(mu_c2, rs, mu_c1_v);
 				if constexpr (decltype(mid)::value) del += nmu * mid_c * inv_mid;
 				const int km = (int)xm;
 				const float fr = __builtin_amdgcn_fractf(xm);
