@@ -40,7 +40,7 @@ FLOP_BY_KIND = {0: 15, 1: 29, 2: 26, 3: 15}
 # 1, 100 regions) under the tuning the autotuner picks for it, from two
 # rocprofv3 --pmc runs: FETCH_SIZE x 2 + WRITE_SIZE x 1 (factors from a 1 GiB
 # copy), KiB -> B.  Source: profiles/k_count_fp32_jackknife_batch_hbm.md
-TRAFFIC_BYTES = {(1.0, 100, 2, 8, 256): (7.789e7 * 2 + 3.758e6) * 1024,
+TRAFFIC_BYTES = {(1.0, 100, 2, 8, 256): (7.811e7 * 2 + 3.758e6) * 1024,
                  (1.0, 100, 2, 12, 256): (7.892e7 * 2 + 3.002e6) * 1024}
 PEAK_FP32_TFLOPS = 157.3      # MI355X dense FP32 peak (matrix cores; the vector peak is the same figure)
 AXES = [(0.0, 100.0, 40, 1.3), (0.0, 1.0, 50, 0.0)]
