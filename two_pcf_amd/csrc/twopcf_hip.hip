@@ -3,14 +3,15 @@
 //
 // Layout on the device (per catalogue and cell geometry):
 //   sorted SoA   double x,y,z,nx,ny,nz; float w, iw; int jk   (cell-major,
-//                region-sorted within a cell: hipCUB radix sort on the key
-//                cell * n_region + region)
+//                sorted along z within a cell: hipCUB radix sort on the key
+//                cell << ZQ_BITS | z quantised within the cell, so that a
+//                row of cells along z is sorted along z as a whole)
 //   cell_start   int[n_cell + 1]
 // Work: a work item is (job, up to 256 first-side objects of one cell, a set
 // of neighbour-cell rows, up to S region slots).  A 1024-thread block (16
 // waves: 4 groups over the first-side objects x 4 groups over each tile) owns
 // one item at a time: each lane holds one first-side object, the second-side
-// objects of each row are staged through LDS in tiles of 512 and every tile
+// objects of each row are staged through LDS in tiles of 1024 and every tile
 // is shared by all 16 waves.  Per-item histograms live in LDS (u32 counts or
 // f64 sums) and are flushed once per item with global atomics.
 #include <hip/hip_runtime.h>
@@ -56,11 +57,13 @@ static_assert(ICHUNK <= 256, "first-side index in one byte of the recheck queue"
 // recheck queue < 64 plus one binning pass's NP * 64
 constexpr int RQ = WAVE * (NP_MAX + 1);     // per-wave queue of pairs to re-evaluate exactly (fp32 path)
 constexpr int REGION_MAP = 512;             // region -> slot code table (fp32 path, n_jk up to this)
-__host__ __device__ constexpr int tile_of(bool fast) { return fast ? TILE_FAST : TILE_EXACT; }
-// tile + (fp32 path) first-side chunk with its two screening thresholds and
-// per-wave pair queues
+constexpr int TILE_BLOCKS = TILE_FAST / 16; // 16-entry screening blocks per tile (one lane each)
+static_assert(TILE_BLOCKS <= 64, "one lane per tile block");
+// tile + (fp32 path) first-side chunk with its two screening thresholds,
+// per-wave pair queues, the region map and the z bounds of the tile blocks
 __host__ __device__ constexpr int tile_bytes(bool fast) {
-	return fast ? TILE_FAST * (2 * 16 + 4) + ICHUNK * (32 + 8) + (BLOCK / WAVE) * (RING * 2 + RQ * 5) + REGION_MAP * 4
+	return fast ? TILE_FAST * (2 * 16 + 4) + ICHUNK * (32 + 8) + (BLOCK / WAVE) * (RING * 2 + RQ * 5) + REGION_MAP * 4 +
+	                  TILE_BLOCKS * 8
 	            : TILE_EXACT * (6 * 8 + 4 + 4 + 4);
 }
 constexpr int MAX_SLOTS = 16;
@@ -91,7 +94,7 @@ struct JobDev {
 	int tq_off;                                      // (T, A, B, 0) per s^2 threshold, 16-byte aligned
 	float s_min, s_ilb_ln2, s_ils, s_inv_step;       // fast s-index estimate
 	int s_log;
-	float mu_c1, mu_c2;                              // fast mu margin: delta = c1 + c2 * rsq(s2)
+	float mu_h0, mu_c2;                              // fast mu test: 1/2 - delta = h0 - c2 * rsq(s2)
 	int f_shift;                                     // fp32 path: full histogram in 2^f_shift lane copies
 	int x1_off;                                      // sigma-pi fp32: sqrtf-domain thresholds of the sigma axis
 	float p_min, p_ilb_ln2, p_ils, p_inv_step;       // sigma-pi fp32: pi-axis index estimate
@@ -122,18 +125,25 @@ struct WorkItem {
 
 // -------------------------------------------------------------- kernels --
 
+// Sort key: the cell, then the position along z within the cell in 2^ZQ_BITS
+// steps (monotone in z, so objects of a row of cells along z come out sorted
+// along z up to ties within one step of h 2^-ZQ_BITS).
+constexpr int ZQ_BITS = 16;
+
 __global__ void k_cell_keys(const double *__restrict__ x, const double *__restrict__ y,
-                            const double *__restrict__ z, const int *__restrict__ jk, int64_t n,
-                            double lo0, double lo1, double lo2, double h, int d0, int d1, int d2,
-                            int n_region, unsigned long long *__restrict__ keys, int *__restrict__ vals) {
+                            const double *__restrict__ z, int64_t n, double lo0, double lo1, double lo2, double h,
+                            int d0, int d1, int d2, unsigned long long *__restrict__ keys, int *__restrict__ vals) {
 	int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
 	if (i >= n) return;
-	int c0 = (int)floor((x[i] - lo0) / h), c1 = (int)floor((y[i] - lo1) / h), c2 = (int)floor((z[i] - lo2) / h);
+	const double fz = (z[i] - lo2) / h;
+	int c0 = (int)floor((x[i] - lo0) / h), c1 = (int)floor((y[i] - lo1) / h), c2 = (int)floor(fz);
 	c0 = min(max(c0, 0), d0 - 1);
 	c1 = min(max(c1, 0), d1 - 1);
 	c2 = min(max(c2, 0), d2 - 1);
-	unsigned long long cell = ((unsigned long long)c0 * d1 + c1) * d2 + c2;
-	keys[i] = cell * (unsigned long long)n_region + (unsigned long long)(n_region > 1 ? jk[i] : 0);
+	const double q = floor((fz - c2) * (double)(1 << ZQ_BITS));
+	const unsigned long long zq = (unsigned long long)min(max(q, 0.0), (double)((1 << ZQ_BITS) - 1));
+	const unsigned long long cell = ((unsigned long long)c0 * d1 + c1) * d2 + c2;
+	keys[i] = (cell << ZQ_BITS) | zq;
 	vals[i] = (int)i;
 }
 
@@ -156,12 +166,12 @@ __global__ void k_gather(const int *__restrict__ perm, int64_t n, const double *
 	for (int k = 0; k < n_words; ++k) omasks[i * n_words + k] = masks[(int64_t)s * n_words + k];
 }
 
-// start[c] = first sorted index whose key >= c * n_region.
-__global__ void k_cell_start(const unsigned long long *__restrict__ keys, int64_t n, int n_cell, int n_region,
+// start[c] = first sorted index whose key >= c << ZQ_BITS.
+__global__ void k_cell_start(const unsigned long long *__restrict__ keys, int64_t n, int n_cell,
                              int *__restrict__ start) {
 	int c = blockIdx.x * blockDim.x + threadIdx.x;
 	if (c > n_cell) return;
-	unsigned long long target = (unsigned long long)c * n_region;
+	unsigned long long target = (unsigned long long)c << ZQ_BITS;
 	int64_t lo = 0, hi = n;
 	while (lo < hi) {
 		int64_t mid = (lo + hi) >> 1;
@@ -300,24 +310,22 @@ __device__ float pair_weight_staged(const JobDev &J, int gi, int gj, const doubl
 	}
 }
 
-template <int KIND, bool FAST, bool WEIGHTED>
+template <int KIND, bool WEIGHTED>
 __device__ void process_item(const JobDev &J, const WorkItem &W, const int4 *__restrict__ rows,
                              const int *__restrict__ slot_regions, char *smem, const Lds &L) {
 	using HT = typename HistT<WEIGHTED>::T;
-	constexpr int TILE = tile_of(FAST);
+	constexpr int TILE = TILE_EXACT;
 	const int tid = threadIdx.x, lane = tid & (WAVE - 1), wave = tid / WAVE;
 	const int ig = wave % IG, jg = wave / IG;
 	const int nb = J.nb;
 	const int n_slots = W.n_slots;
 	const bool jk = J.n_jk > 1;
 
-	// LDS carve-up
-	double *tx = (double *)(smem + L.tile);         // exact path: 6 doubles per entry, then w, gj, slot
+	// LDS carve-up: 6 doubles per tile entry, then w, gj, slot
+	double *tx = (double *)(smem + L.tile);
 	float *tw = (float *)(tx + 6 * TILE);
 	int *tgj = (int *)(tw + TILE);
-	float4 *tfa = (float4 *)(smem + L.tile);        // fast path: (o0,o1,o2,w), (n0,n1,n2,gj), slot
-	float4 *tfb = tfa + TILE;
-	int *tslot = FAST ? (int *)(tfb + TILE) : tgj + TILE;
+	int *tslot = tgj + TILE;
 	float *tab = (float *)(smem + L.tables);
 	int *sreg = (int *)(smem + L.slots);
 	HT *F = (HT *)(smem + L.hist);
@@ -334,7 +342,6 @@ __device__ void process_item(const JobDev &J, const WorkItem &W, const int4 *__r
 	const int gi = W.i_begin + min(li, W.i_count - 1);
 	const int d1 = J.dim[1], d2 = J.dim[2];
 	const int cx = W.i_cell / (d1 * d2), cy = (W.i_cell / d2) % d1, cz = W.i_cell % d2;
-	const double c0 = J.lo[0] + (cx + 0.5) * J.h, c1 = J.lo[1] + (cy + 0.5) * J.h, c2 = J.lo[2] + (cz + 0.5) * J.h;
 	double xi[3], ni[3];
 	xi[0] = J.A.x[gi];
 	xi[1] = J.A.y[gi];
@@ -344,8 +351,6 @@ __device__ void process_item(const JobDev &J, const WorkItem &W, const int4 *__r
 	ni[2] = J.A.nz[gi];
 	const float wi = J.A.w[gi];
 	const int jki = J.A.jk[gi];
-	const float oi0 = (float)(xi[0] - c0), oi1 = (float)(xi[1] - c1), oi2 = (float)(xi[2] - c2);
-	const float fni0 = (float)ni[0], fni1 = (float)ni[1], fni2 = (float)ni[2];
 	__syncthreads();
 	// region code: an LDS slot s >= 0, or -(region + 1) for a region without a
 	// slot in this item, whose sums go straight to global memory
@@ -359,12 +364,8 @@ __device__ void process_item(const JobDev &J, const WorkItem &W, const int4 *__r
 	const float *t_s2 = tab + J.t_s2_off;
 	const float *t1 = tab + J.t1_off;
 	const float *t2 = tab + J.t2_off;
-	const float *tA = tab + J.tA_off;
-	const float *tB = tab + J.tB_off;
-	const float s2_lo = FAST ? tB[0] : 0.0f;
-	const float s2_hi = FAST ? tA[J.n1] : 0.0f;
 
-	unsigned int n_acc = 0, n_rc = 0;
+	unsigned int n_acc = 0;
 
 	auto add_region = [&](int code, int bin, float w) {
 		if (code >= 0) {
@@ -410,72 +411,27 @@ __device__ void process_item(const JobDev &J, const WorkItem &W, const int4 *__r
 					for (int s = 0; s < n_slots; ++s)
 						if (sreg[s] == rj) sl = s;
 				}
-				if (FAST) {
-					tfa[tid] = make_float4((float)(bx - c0), (float)(by - c1), (float)(bz - c2), bw);
-					tfb[tid] = make_float4((float)bnx, (float)bny, (float)bnz, __int_as_float(gj));
-				} else {
-					tx[6 * tid + 0] = bx;
-					tx[6 * tid + 1] = by;
-					tx[6 * tid + 2] = bz;
-					tx[6 * tid + 3] = bnx;
-					tx[6 * tid + 4] = bny;
-					tx[6 * tid + 5] = bnz;
-					tw[tid] = bw;
-					tgj[tid] = gj;
-				}
+				tx[6 * tid + 0] = bx;
+				tx[6 * tid + 1] = by;
+				tx[6 * tid + 2] = bz;
+				tx[6 * tid + 3] = bnx;
+				tx[6 * tid + 4] = bny;
+				tx[6 * tid + 5] = bnz;
+				tw[tid] = bw;
+				tgj[tid] = gj;
 				tslot[tid] = sl;
 			}
 			__syncthreads();
 			if (ivalid) {
 				for (int q = jg; q < nt; q += JG) {
 					int bin;
-					if (FAST) {
-						const float4 a = tfa[q];
-						const float4 b = tfb[q];
-						const int gj = __float_as_int(b.w);
-						const float dx = oi0 - a.x, dy = oi1 - a.y, dz = oi2 - a.z;
-						const float s2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
-						if (!(s2 >= s2_lo && s2 < s2_hi)) continue;
-						if (J.is_auto && gj <= gi) continue;
-						// s bin: closed-form estimate, one-step correction, proof by margins
-						const float sv = __builtin_amdgcn_sqrtf(s2) - J.s_min;
-						int k = J.s_log ? (int)(J.s_ilb_ln2 * __builtin_amdgcn_logf(fmaf(J.s_ils, sv, 1.0f)))
-						                : (int)(sv * J.s_inv_step);
-						k = min(max(k, 0), J.n1 - 1);
-						if (s2 < t_s2[k]) k = max(k - 1, 0);
-						else if (s2 >= t_s2[k + 1]) k = min(k + 1, J.n1 - 1);
-						bool sure = (s2 >= tA[k]) && (s2 < tB[k + 1]);
-						if (KIND == K_S_MU) {
-							const float p = 0.5f * fmaf(dz, fni2 + b.z, fmaf(dy, fni1 + b.y, dx * (fni0 + b.x)));
-							const float rs = __builtin_amdgcn_rsqf(s2);
-							const float nmu = (float)J.n2;
-							const float xm = fabsf(p) * rs * nmu;
-							const float del = fmaf(J.mu_c2, rs, J.mu_c1);
-							const int km = (int)xm;
-							const float fr = xm - (float)km;
-							sure = sure && (xm < nmu - del) && (fr <= 1.0f - del) && (km == 0 || fr >= del);
-							bin = k * J.n2 + km;
-						} else {
-							bin = k;
-						}
-						if (!sure) {
-							++n_rc;
-							double xj[3] = {J.B.x[gj], J.B.y[gj], J.B.z[gj]};
-							double nj[3] = {J.B.nx[gj], J.B.ny[gj], J.B.nz[gj]};
-							if (!exact_bin<KIND>(xi, ni, xj, nj, J.min_r_sq, J.max_r_sq, t_s2, t1, t2, J.ax1, J.ax2, bin, J.pi_def))
-								continue;
-						}
-						++n_acc;
-						add(bin, wi * a.w, tslot[q]);
-					} else {
-						const int gj = tgj[q];
-						if (J.is_auto && gj <= gi) continue;
-						const double *e = tx + 6 * q;
-						if (!exact_bin<KIND>(xi, ni, e, e + 3, J.min_r_sq, J.max_r_sq, t_s2, t1, t2, J.ax1, J.ax2, bin, J.pi_def))
-							continue;
-						++n_acc;
-						add(bin, J.wmode == TWOPCF_W_PLAIN ? wi * tw[q] : pair_weight(J, gi, gj, ni, e + 3), tslot[q]);
-					}
+					const int gj = tgj[q];
+					if (J.is_auto && gj <= gi) continue;
+					const double *e = tx + 6 * q;
+					if (!exact_bin<KIND>(xi, ni, e, e + 3, J.min_r_sq, J.max_r_sq, t_s2, t1, t2, J.ax1, J.ax2, bin, J.pi_def))
+						continue;
+					++n_acc;
+					add(bin, J.wmode == TWOPCF_W_PLAIN ? wi * tw[q] : pair_weight(J, gi, gj, ni, e + 3), tslot[q]);
 				}
 			}
 			__syncthreads();
@@ -504,14 +460,8 @@ __device__ void process_item(const JobDev &J, const WorkItem &W, const int4 *__r
 		}
 	}
 	// pair statistics
-	for (int off = 32; off; off >>= 1) {
-		n_acc += __shfl_xor(n_acc, off);
-		n_rc += __shfl_xor(n_rc, off);
-	}
-	if (lane == 0) {
-		if (n_acc) gadd(&J.counters[0], (unsigned long long)n_acc);
-		if (n_rc) gadd(&J.counters[1], (unsigned long long)n_rc);
-	}
+	for (int off = 32; off; off >>= 1) n_acc += __shfl_xor(n_acc, off);
+	if (lane == 0 && n_acc) gadd(&J.counters[0], (unsigned long long)n_acc);
 	__syncthreads();
 }
 
@@ -567,6 +517,9 @@ __device__ void process_item_fast(const JobDev &J, const WorkItem &W, const int4
 	// region -> code (s * nb for LDS slot s, else -(region + 1)) for this item
 	int *rmap = (int *)((unsigned char *)((int *)((unsigned short *)(ctl + ICHUNK) + (BLOCK / WAVE) * RING) +
 	                                      (BLOCK / WAVE) * RQ) + (BLOCK / WAVE) * RQ);
+	// z of the first and of the last entry of each 16-entry tile block
+	float *tbz = (float *)(rmap + REGION_MAP);
+	float *tbe = tbz + TILE_BLOCKS;
 	float *tab = (float *)(smem + L.tables);
 	int *sreg = (int *)(smem + L.slots);
 	// the full histogram: 2^f_shift copies, bin-major, lane l adding to copy
@@ -639,10 +592,13 @@ __device__ void process_item_fast(const JobDev &J, const WorkItem &W, const int4
 			else gadd(&J.jk_u[o], 1ull);
 		}
 	};
+	// lane's copy of the full histogram; bin b is f_stride << b bytes further
+	char *const F_lane = (char *)(F + f_lane);
+	const int f_bshift = f_shift + (WEIGHTED ? 3 : 2);
 	auto add = [&](int bin, float w, int si, int sj) {
-		const int fb = (bin << f_shift) + f_lane;
-		if (WEIGHTED) atomicAdd((double *)&F[fb], (double)w);
-		else atomicAdd((unsigned int *)&F[fb], 1u);
+		HT *const fp = (HT *)(F_lane + (bin << f_bshift));
+		if (WEIGHTED) atomicAdd((double *)fp, (double)w);
+		else atomicAdd((unsigned int *)fp, 1u);
 		if (jk) {
 			if (__builtin_amdgcn_ballot_w64(si < 0 || sj < 0) == 0ull) {
 				// every active lane's regions have LDS slots
@@ -706,14 +662,18 @@ __device__ void process_item_fast(const JobDev &J, const WorkItem &W, const int4
 	const float s_min = J.s_min;
 	const float s_ilb = s_log ? J.s_ilb_ln2 : 0.0f, s_ils = s_log ? J.s_ils : 0.0f, s_inv = s_log ? 0.0f : J.s_inv_step;
 	const float p_ilb = p_log ? J.p_ilb_ln2 : 0.0f, p_ils = p_log ? J.p_ils : 0.0f, p_inv = p_log ? 0.0f : J.p_inv_step;
-	auto bin_estimate = [](float v, float inv, float ilb, float ils) {
-		return (int)fmaf(inv, v, ilb * __builtin_amdgcn_logf(fmaf(ils, v, 1.0f)));
+	// clamped to [0, n - 1] before the conversion (one v_med3); a NaN
+	// estimate converts to 0, and any index in range is safe because the
+	// thresholds of that bin then decide
+	auto bin_estimate = [](float v, float inv, float ilb, float ils, float top) {
+		return (int)__builtin_amdgcn_fmed3f(fmaf(inv, v, ilb * __builtin_amdgcn_logf(fmaf(ils, v, 1.0f))), 0.0f, top);
 	};
-	const float mu_c1 = J.mu_c1, mu_c2 = J.mu_c2;
-	// mu_c1 held in a vector register: the margin's multiply-add then reads
+	const float n1_top = (float)(n1 - 1), n2_top = (float)(n2 - 1);
+	const float mu_h0 = J.mu_h0, mu_c2 = J.mu_c2;
+	// mu_h0 held in a vector register: the margin's multiply-add then reads
 	// one scalar operand, with no per-pair copy
-	float mu_c1_v;
-	asm volatile("v_mov_b32 %0, %1" : "=v"(mu_c1_v) : "s"(mu_c1));
+	float mu_h0_v;
+	asm volatile("v_mov_b32 %0, %1" : "=v"(mu_h0_v) : "s"(mu_h0));
 	const float *x1t = tab + J.x1_off, *t2t = tab + J.t2_off;
 	const float rA0 = tab[J.tA_off], rBn = tab[J.tB_off + n1];
 	// line of sight in FP32: the angular definition from the float unit
@@ -768,13 +728,11 @@ __device__ void process_item_fast(const JobDev &J, const WorkItem &W, const int4
 			const float ex = fmaf(J.e_c1, s2, J.e_c2 * s) + dpi * fmaf(2.0f, ap, dpi) + 3.6e-7f * fmaf(p, p, s2);
 			const bool r_in = s2 >= rA0 && s2 < rBn;   // min_r_sq < s2_ref < max_r_sq
 			const float sg = __builtin_amdgcn_sqrtf(fmaxf(x, 0.0f)) - s_min;
-			int k1 = bin_estimate(sg, s_inv, s_ilb, s_ils);
-			k1 = min(max(k1, 0), n1 - 1);
+			const int k1 = bin_estimate(sg, s_inv, s_ilb, s_ils, n1_top);
 			const bool sure1 = x >= x1t[k1] + ex && x < x1t[k1 + 1] - ex;
 			const bool out1 = x < x1t[0] - ex || x >= x1t[n1] + ex;
 			const float pv = ap - J.p_min;
-			int k2 = bin_estimate(pv, p_inv, p_ilb, p_ils);
-			k2 = min(max(k2, 0), n2 - 1);
+			const int k2 = bin_estimate(pv, p_inv, p_ilb, p_ils, n2_top);
 			const bool sure2 = ap >= t2t[k2] + dpi && ap < t2t[k2 + 1] - dpi;
 			const bool out2 = ap < t2t[0] - dpi || ap >= t2t[n2] + dpi;
 			if (r_in && (out1 || out2)) {
@@ -794,14 +752,15 @@ __device__ void process_item_fast(const JobDev &J, const WorkItem &W, const int4
 			// 1 - cos = chord^2 / 2 for angular counts.  Pairs the screening
 			// let through from its error shell are certainly out of range.
 			in_range = s2 < s2_hi && s2 >= s2_lo_eff;
-			// s-mu takes s for its estimate as s2 rsq(s2); the 1e-30 keeps
-			// s2 = 0 finite and does not change any s2 above 1e-23
-			const float rs = KIND == K_S_MU ? __builtin_amdgcn_rsqf(s2 + 1e-30f) : 0.0f;
+			// s-mu takes s for its estimate as s2 rsq(s2).  At s2 = 0 that is
+			// 0 * inf = NaN: the estimate clamps to a valid bin, whose lower
+			// threshold T_k + E > 0 fails, and mu is NaN, so the pair is never
+			// sure (it is out of range, or rechecked when s_min = 0)
+			const float rs = KIND == K_S_MU ? __builtin_amdgcn_rsqf(s2) : 0.0f;
 			const float sv = KIND == K_ANGULAR ? fmaf(0.5f, s2, -s_min)
 			               : KIND == K_S_MU    ? fmaf(s2, rs, -s_min)
 			                                   : __builtin_amdgcn_sqrtf(s2) - s_min;
-			int k = bin_estimate(sv, s_inv, s_ilb, s_ils);
-			k = min(max(k, 0), n1 - 1);
+			const int k = bin_estimate(sv, s_inv, s_ilb, s_ils, n1_top);
 			const float4 q0 = tq[k], q1 = tq[k + 1];
 			sure = (s2 >= q0.y) && (s2 < q1.z);   // T_k + E <= s2 < T_k+1 - E
 			if (KIND == K_S_MU) {
@@ -818,11 +777,18 @@ __device__ void process_item_fast(const JobDev &J, const WorkItem &W, const int4
 					scale = 0.5f * nmu;
 				}
 				const float xm = fabsf(pm) * rs * scale;
-				float del = fmaf(mu_c2, rs, mu_c1_v);
-				if constexpr (decltype(mid)::value) del += nmu * mid_c * inv_mid;
+				// Bin km of mu is proved when xm lies at least the margin
+				// delta = c1 + c2 rsq(s2) (+ the midpoint term) inside it:
+				// |fract(xm) - 1/2| <= 1/2 - delta, and km < n2.  The host's
+				// h0 = 1/2 - c1 - 2^-21 leaves room for this test's own
+				// roundings.  At mu = 0 the lower edge is not a bin edge; the
+				// test still asks for the margin there, which only rechecks
+				// pairs with mu below about 1e-5.
+				float hm = fmaf(-mu_c2, rs, mu_h0_v);
+				if constexpr (decltype(mid)::value) hm -= nmu * mid_c * inv_mid;
 				const int km = (int)xm;
 				const float fr = __builtin_amdgcn_fractf(xm);
-				sure = sure && (xm < nmu - del) && (fr <= 1.0f - del) && (km == 0 || fr >= del);
+				sure = sure && (km < n2) && (fabsf(fr - 0.5f) <= hm);
 				bin = __mul24(k, n2) + km;
 			} else {
 				bin = k;
@@ -920,12 +886,20 @@ __device__ void process_item_fast(const JobDev &J, const WorkItem &W, const int4
 	const unsigned lane_code = ((unsigned)lrow << 6) | (unsigned)(4 * lk);
 	const int lane_d = lrow - 4 * lk;   // (column - row) within the block, for the order test
 	const float *tff = (const float *)tfa;
+	// z window slack: float offsets, sort ties within one key step (h 2^-16)
+	const float zpad = (float)(J.h * 0x1p-12);
 
 	for (int r = 0; r < J.n_rows; ++r) {
 		if (!((W.row_mask >> r) & 1ull)) continue;
 		const int4 row = rows[J.row_off + r];
 		const int qx = cx + row.x, qy = cy + row.y;
 		if (qx < 0 || qy < 0 || qx >= J.dim[0] || qy >= d1) continue;
+		// A pair can pass the screening only if its z separation is at most
+		// reach: the first-side cell and this row's cells are at least gx, gy
+		// apart across x and y, and the screening passes s^2 < s2_hi + eps.
+		const float gx = (float)(max(abs(row.x) - 1, 0) * J.h), gy = (float)(max(abs(row.y) - 1, 0) * J.h);
+		const float reach = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(
+		    __builtin_amdgcn_sqrtf(fmaxf(s2_hi + J.scr_eps - fmaf(gx, gx, gy * gy), 0.0f)) * (1.0f + 0x1p-10f) + zpad)));
 		if (is_auto && (row.x < 0 || (row.x == 0 && row.y < 0))) continue;
 		int z0 = max(cz + row.z, 0), z1 = min(cz + row.w, d2 - 1);
 		if (is_auto && row.x == 0 && row.y == 0) z0 = max(z0, cz);
@@ -952,6 +926,10 @@ __device__ void process_item_fast(const JobDev &J, const WorkItem &W, const int4
 				tfb[tid] = make_float4((float)gld(J.B.nx)[gj], (float)gld(J.B.ny)[gj], (float)gld(J.B.nz)[gj],
 				                       WEIGHTED ? (stage_iw ? gld(J.B.iw)[gj] : gld(J.B.w)[gj]) : __int_as_float(sl));
 				tslot[tid] = sl;
+				// the row is sorted along z: each block's z range is its first
+				// and last entry (up to sort ties within the slack)
+				if ((tid & 15) == 0) tbz[tid >> 4] = oz;
+				if ((tid & 15) == 15 || tid == nt - 1) tbe[tid >> 4] = oz;
 			} else if (tid < ((nt + 15) & ~15)) {
 				// the last 16-entry block is padded with entries that are
 				// never in range (|b|^2 = 1e30)
@@ -960,8 +938,25 @@ __device__ void process_item_fast(const JobDev &J, const WorkItem &W, const int4
 			__syncthreads();
 			if (rb_n > 0) {
 				const bool order = t0 < own_end;
+				const int nblk = (nt + 15) >> 4;
+				const unsigned long long valid = nblk >= 64 ? ~0ull : (1ull << nblk) - 1ull;
 				for (int rb = 0; rb < rb_n; ++rb) {
 					const int o = ig * WAVE + rb * 16;   // first object of the row block
+					// The first-side objects are sorted along z too: the row
+					// block spans cia[o].z .. cia[o + 15].z (entries past the
+					// item repeat its last object).  Tile blocks wholly below
+					// or above that span widened by reach hold no pair to
+					// screen; screening runs from the first block not below to
+					// the last block not above (tile block `lane` spans
+					// tbz[lane] .. tbe[lane]).
+					const float zlo = cia[o].z - reach, zhi = cia[o + 15].z + reach;
+					const unsigned long long below =
+					    __builtin_amdgcn_ballot_w64(lane < nblk && tbe[lane] < zlo) & valid;
+					const unsigned long long need =
+					    ~__builtin_amdgcn_ballot_w64(lane < nblk && tbz[lane] > zhi) & valid;
+					if (need == 0ull || below == valid) continue;
+					const int kb = __builtin_ctzll(~below), ke = 64 - __builtin_clzll(need);
+					if (kb >= ke) continue;
 					const float4 ca = cia[o + lrow];
 					const float av = lk == 0 ? -2.0f * ca.x : lk == 1 ? -2.0f * ca.y : lk == 2 ? -2.0f * ca.z : 1.0f;
 					const float4 th = *(const float4 *)&cth[o + 4 * lk];
@@ -979,9 +974,11 @@ __device__ void process_item_fast(const JobDev &J, const WorkItem &W, const int4
 					auto product = [&](float bv) {
 						return __builtin_amdgcn_mfma_f32_16x16x4f32(av, bv, f32x4{0.0f, 0.0f, 0.0f, 0.0f}, 0, 0, 0);
 					};
-					f32x4 d_next = product(b_load(jg * 16));
-					float b_next = b_load(jg * 16 + step);
-					for (int qb = jg * 16; qb < nt; qb += step) {
+					// this wave group's blocks: k = jg mod jg_n in [kb, ke)
+					const int q0 = 16 * (kb + (jg - kb % jg_n + jg_n) % jg_n), q1 = 16 * ke;
+					f32x4 d_next = product(b_load(q0));
+					float b_next = b_load(q0 + step);
+					for (int qb = q0; qb < q1; qb += step) {
 						const f32x4 d = d_next;
 						d_next = product(b_next);
 						b_next = b_load(qb + 2 * step);
@@ -1085,7 +1082,7 @@ __global__ void __launch_bounds__(BLOCK) k_count(const JobDev *__restrict__ jobs
 			// MID: every job of this launch has the pair-midpoint pi
 			process_item_fast<KIND, WEIGHTED, MID>(jobs[W.job], W, rows, slot_regions, smem, L);
 		} else {
-			process_item<KIND, false, WEIGHTED>(jobs[W.job], W, rows, slot_regions, smem, L);
+			process_item<KIND, WEIGHTED>(jobs[W.job], W, rows, slot_regions, smem, L);
 		}
 		if (!queue) break;
 	}
@@ -1167,7 +1164,7 @@ struct SortedCat {
 	int m_stride = 0;
 	int *start = nullptr;
 	std::vector<int> start_h;
-	// regions present in each cell (ascending), CSR over cells (n_region > 1 only)
+	// regions present in each cell, CSR over cells (n_region > 1 only)
 	std::vector<int> reg_off, reg_ids;
 	void release() {
 		for (void *p : {(void *)x, (void *)y, (void *)z, (void *)nx, (void *)ny, (void *)nz, (void *)w, (void *)iw,
@@ -1249,14 +1246,14 @@ int grid_catalog(twopcf_ctx *ctx, twopcf_cat *c, const GridKey &g, SortedCat **o
 		cleanup();
 		return rc;
 	}
-	const unsigned long long max_key = (unsigned long long)n_cell * g.n_region;
+	const unsigned long long max_key = (unsigned long long)n_cell << ZQ_BITS;
 	int end_bit = 1;
 	while (end_bit < 64 && (1ull << end_bit) <= max_key) ++end_bit;
 	const int nblk = (int)((n + 255) / 256);
 	if (n > 0) {
 		k_cell_keys<<<nblk, 256, 0, ctx->stream>>>(g.on_norm ? c->nx : c->x, g.on_norm ? c->ny : c->y,
-		                                           g.on_norm ? c->nz : c->z, c->jk, n, g.lo[0], g.lo[1], g.lo[2], g.h,
-		                                           g.dim[0], g.dim[1], g.dim[2], g.n_region, keys, vals);
+		                                           g.on_norm ? c->nz : c->z, n, g.lo[0], g.lo[1], g.lo[2], g.h,
+		                                           g.dim[0], g.dim[1], g.dim[2], keys, vals);
 		HIPCK(ctx, hipGetLastError());
 		HIPCK(ctx, hipcub::DeviceRadixSort::SortPairs(nullptr, tmp_bytes, keys, keys_s, vals, perm, (int)n, 0,
 		                                               end_bit, ctx->stream));
@@ -1280,7 +1277,7 @@ int grid_catalog(twopcf_ctx *ctx, twopcf_cat *c, const GridKey &g, SortedCat **o
 		                                        s.jk, s.masks);
 		HIPCK(ctx, hipGetLastError());
 	}
-	k_cell_start<<<(n_cell + 1 + 255) / 256, 256, 0, ctx->stream>>>(keys_s, n, n_cell, g.n_region, s.start);
+	k_cell_start<<<(n_cell + 1 + 255) / 256, 256, 0, ctx->stream>>>(keys_s, n, n_cell, s.start);
 	HIPCK(ctx, hipGetLastError());
 	s.start_h.resize((size_t)n_cell + 1);
 	HIPCK(ctx, hipMemcpyAsync(s.start_h.data(), s.start, ((size_t)n_cell + 1) * sizeof(int), hipMemcpyDeviceToHost,
@@ -1294,10 +1291,14 @@ int grid_catalog(twopcf_ctx *ctx, twopcf_cat *c, const GridKey &g, SortedCat **o
 	cleanup();
 	if (g.n_region > 1) {
 		s.reg_off.assign((size_t)n_cell + 1, 0);
+		std::vector<int> seen((size_t)g.n_region, -1);   // last cell that listed the region
 		for (int cc = 0; cc < n_cell; ++cc) {
 			s.reg_off[cc] = (int)s.reg_ids.size();
 			for (int i = s.start_h[cc]; i < s.start_h[cc + 1]; ++i)
-				if (i == s.start_h[cc] || jk_h[i] != jk_h[i - 1]) s.reg_ids.push_back(jk_h[i]);
+				if (seen[jk_h[i]] != cc) {
+					seen[jk_h[i]] = cc;
+					s.reg_ids.push_back(jk_h[i]);
+				}
 		}
 		s.reg_off[n_cell] = (int)s.reg_ids.size();
 	}
@@ -1864,7 +1865,7 @@ int build_plan(twopcf_ctx *ctx, const twopcf_job *jobs, int n_jobs, int method, 
 			// the pair-midpoint line of sight doubles the relative part and
 			// adds 1.02 2^-23 8 R s / |x1 + x2| to the pi bound
 			const double mid = H.j->pi_def == TWOPCF_PI_MIDPOINT ? 2.0 : 1.0;
-			D.mu_c1 = (float)(n2 * std::ldexp(1.0, -23) * 41.0 * mid);
+			D.mu_h0 = (float)(0.5 - n2 * std::ldexp(1.0, -23) * 41.0 * mid - std::ldexp(1.0, -21));
 			D.mu_c2 = (float)(n2 * std::ldexp(1.0, -23) * 4.75 * R * 1.01);
 			D.mid_c = H.j->pi_def == TWOPCF_PI_MIDPOINT ? (float)(1.02 * std::ldexp(8.0 * R, -23)) : 0.0f;
 			max_tab_n1 = std::max(max_tab_n1, H.ax1.n);
@@ -1957,7 +1958,7 @@ int build_plan(twopcf_ctx *ctx, const twopcf_job *jobs, int n_jobs, int method, 
 				// group rows by region budget
 				std::vector<int> regs_i;
 				if (jk) {
-					// regions of this chunk (sorted runs inside the cell)
+					// regions of this chunk: those present in its cell
 					const int ro = A->reg_off[c], re = A->reg_off[c + 1];
 					(void)re;
 					std::vector<int> tmp;
