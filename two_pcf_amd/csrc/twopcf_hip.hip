@@ -553,6 +553,9 @@ __device__ void process_item_fast(const JobDev &J, const WorkItem &W, const int4
 	const int d1 = J.dim[1], d2 = J.dim[2];
 	const int cx = W.i_cell / (d1 * d2), cy = (W.i_cell / d2) % d1, cz = W.i_cell % d2;
 	const double c0 = J.lo[0] + (cx + 0.5) * J.h, c1 = J.lo[1] + (cy + 0.5) * J.h, c2 = J.lo[2] + (cz + 0.5) * J.h;
+	// s-mu with the angular pi stages its normals times n2 / 2, so that
+	// mu n2 = |d.(n1 + n2)| rsq(s2) takes no scale multiply per pair
+	const float nscale = (KIND == K_S_MU && !MID) ? 0.5f * (float)n2 : 1.0f;
 	__syncthreads();
 	if (tid < ICHUNK) {
 		const int g = W.i_begin + min(tid, W.i_count - 1);
@@ -563,7 +566,8 @@ __device__ void process_item_fast(const JobDev &J, const WorkItem &W, const int4
 		const double GLOBAL_AS *pz = gld(KIND == K_ANGULAR ? J.A.nz : J.A.z);
 		const float ox = (float)(px[g] - c0), oy = (float)(py[g] - c1), oz = (float)(pz[g] - c2);
 		cia[tid] = make_float4(ox, oy, oz, stage_iw ? gld(J.A.iw)[g] : gld(J.A.w)[g]);
-		cib[tid] = make_float4((float)gld(J.A.nx)[g], (float)gld(J.A.ny)[g], (float)gld(J.A.nz)[g], __int_as_float(code));
+		cib[tid] = make_float4(nscale * (float)gld(J.A.nx)[g], nscale * (float)gld(J.A.ny)[g],
+		                       nscale * (float)gld(J.A.nz)[g], __int_as_float(code));
 		// s^2 = |a|^2 + (|b|^2 - 2 a.b) may be in range iff the bracket lies
 		// within [s2_lo - eps - |a|^2, s2_hi + eps - |a|^2); objects past the
 		// end of the item never pass
@@ -765,18 +769,15 @@ __device__ void process_item_fast(const JobDev &J, const WorkItem &W, const int4
 			sure = (s2 >= q0.y) && (s2 < q1.z);   // T_k + E <= s2 < T_k+1 - E
 			if (KIND == K_S_MU) {
 				const float nmu = (float)n2;
-				// mu n2 = |pi| rsq(s2) n2; the angular pi's factor 1/2 is taken
-				// into the scale (exact, so the product is the same float)
-				float inv_mid, pm, scale;
+				// mu n2 = |pi| rsq(s2) n2: the midpoint pi times n2, or for the
+				// angular pi d.(n1 + n2) from normals staged times n2 / 2
+				float inv_mid, xm;
 				if constexpr (decltype(mid)::value) {
-					pm = fp32_pi(mid, dx, dy, dz, ia, ib, a, b, inv_mid);
-					scale = nmu;
+					xm = fabsf(fp32_pi(mid, dx, dy, dz, ia, ib, a, b, inv_mid)) * rs * nmu;
 				} else {
 					inv_mid = 0.0f;
-					pm = fmaf(dz, ib.z + b.z, fmaf(dy, ib.y + b.y, dx * (ib.x + b.x)));
-					scale = 0.5f * nmu;
+					xm = fabsf(fmaf(dz, ib.z + b.z, fmaf(dy, ib.y + b.y, dx * (ib.x + b.x)))) * rs;
 				}
-				const float xm = fabsf(pm) * rs * scale;
 				// Bin km of mu is proved when xm lies at least the margin
 				// delta = c1 + c2 rsq(s2) (+ the midpoint term) inside it:
 				// |fract(xm) - 1/2| <= 1/2 - delta, and km < n2.  The host's
@@ -923,7 +924,8 @@ __device__ void process_item_fast(const JobDev &J, const WorkItem &W, const int4
 				const float ox = (float)(px[gj] - c0), oy = (float)(py[gj] - c1), oz = (float)(pz[gj] - c2);
 				tfa[tid] = make_float4(ox, oy, oz, fmaf(oz, oz, fmaf(oy, oy, ox * ox)));
 				// unweighted jobs keep the region code where the weight would be
-				tfb[tid] = make_float4((float)gld(J.B.nx)[gj], (float)gld(J.B.ny)[gj], (float)gld(J.B.nz)[gj],
+				tfb[tid] = make_float4(nscale * (float)gld(J.B.nx)[gj], nscale * (float)gld(J.B.ny)[gj],
+				                       nscale * (float)gld(J.B.nz)[gj],
 				                       WEIGHTED ? (stage_iw ? gld(J.B.iw)[gj] : gld(J.B.w)[gj]) : __int_as_float(sl));
 				tslot[tid] = sl;
 				// the row is sorted along z: each block's z range is its first
@@ -1865,7 +1867,11 @@ int build_plan(twopcf_ctx *ctx, const twopcf_job *jobs, int n_jobs, int method, 
 			// the pair-midpoint line of sight doubles the relative part and
 			// adds 1.02 2^-23 8 R s / |x1 + x2| to the pi bound
 			const double mid = H.j->pi_def == TWOPCF_PI_MIDPOINT ? 2.0 : 1.0;
-			D.mu_h0 = (float)(0.5 - n2 * std::ldexp(1.0, -23) * 41.0 * mid - std::ldexp(1.0, -21));
+			// relative part: 41 ulp (2^-23) for the pi and s chain, doubled
+			// for the midpoint; the angular definition's normals staged times
+			// n2 / 2 add two roundings, taken as 2 ulp more
+			const double c1 = n2 * std::ldexp(1.0, -23) * (mid > 1.0 ? 82.0 : 43.0);
+			D.mu_h0 = (float)(0.5 - c1 - std::ldexp(1.0, -21));
 			D.mu_c2 = (float)(n2 * std::ldexp(1.0, -23) * 4.75 * R * 1.01);
 			D.mid_c = H.j->pi_def == TWOPCF_PI_MIDPOINT ? (float)(1.02 * std::ldexp(8.0 * R, -23)) : 0.0f;
 			max_tab_n1 = std::max(max_tab_n1, H.ax1.n);
