@@ -40,8 +40,11 @@ FLOP_BY_KIND = {0: 15, 1: 29, 2: 26, 3: 15}
 # 1, 100 regions) under the tuning the autotuner picks for it, from two
 # rocprofv3 --pmc runs: FETCH_SIZE x 2 + WRITE_SIZE x 1 (factors from a 1 GiB
 # copy), KiB -> B.  Source: profiles/k_count_fp32_jackknife_batch_hbm.md
-TRAFFIC_BYTES = {(1.0, 100, 2, 8, 256): (7.811e7 * 2 + 3.758e6) * 1024,
-                 (1.0, 100, 2, 12, 256): (7.892e7 * 2 + 3.002e6) * 1024}
+TRAFFIC_BYTES = {(1.0, 100, 2, 8, 256): (7.772e7 * 2 + 3.601e6) * 1024,
+                 (1.0, 100, 2, 9, 256): (7.890e7 * 2 + 2.876e6) * 1024}
+# region slots that fit in LDS on the default workload (2000-bin u32
+# histograms next to the 1024-entry tile, queues and tables)
+SLOTS_FIT = 9
 PEAK_FP32_TFLOPS = 157.3      # MI355X dense FP32 peak (matrix cores; the vector peak is the same figure)
 AXES = [(0.0, 100.0, 40, 1.3), (0.0, 1.0, 50, 0.0)]
 
@@ -374,7 +377,10 @@ def main():
     achieved = flop / kernel_s / 1e12
     tu_used = tuned["choice"] if tuned else dict(cell_subdiv=tuning.get("cell_subdiv", 2),
                                                    max_slots=tuning.get("max_slots", 8), i_chunk=256)
-    traffic_key = (W["traffic_key"] + (tu_used["cell_subdiv"], tu_used["max_slots"], tu_used["i_chunk"])
+    # region slots in use: the LDS holds at most SLOTS_FIT per item on the
+    # default workload, so larger settings run the same plan
+    traffic_key = (W["traffic_key"] + (tu_used["cell_subdiv"], min(tu_used["max_slots"], SLOTS_FIT),
+                                       tu_used["i_chunk"])
                    if W["traffic_key"] else None)
     config = dict(W["config"])
     config.update({"parallelism": "single GPU", "tuning": tuning or (tuned if tuned else "default")})

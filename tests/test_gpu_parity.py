@@ -301,3 +301,48 @@ def test_dense_cells_multi_tile(ctx, oracle_lib, kind):
             np.testing.assert_array_equal(r["jk"], sub)
     for c in dc.values():
         c.free()
+
+
+@pytest.mark.parametrize("kind", [0, 1, 2, 3])
+def test_long_column_z_window(ctx, oracle_lib, kind):
+    """A column 30 Mpc/h wide and 600 long along z, about 300 objects per
+    cell: stencil rows along z hold more than one 1024-entry tile, so the
+    screening's z window skips blocks at both ends of every row.  A third of
+    the objects sit on a 0.5 Mpc/h lattice in z (many equal z values, sort
+    ties), and pairs at every separation up to the maximum occur along z.
+    Monopole, sigma-pi, s-mu and angular counts, auto and cross, 4 regions,
+    both methods equal the oracle."""
+    from two_pcf_amd import lib as L
+    o = oracle_lib
+    rng = np.random.Generator(np.random.PCG64(41 + kind))
+    axes = {0: [(0.0, 40.0, 20, 1.3)], 1: [(0.0, 30.0, 15, 0.0), (0.0, 40.0, 10, 0.0)],
+            2: [(0.0, 40.0, 20, 1.3), (0.0, 1.0, 10, 0.0)],
+            3: [(0.0, float(1.0 - np.cos(0.015)), 12, 1.6)]}[kind]
+    cats, dc, oc = {}, {}, {}
+    for tag, n in (("a", 30000), ("b", 10000)):
+        p = np.stack([rng.uniform(2000.0, 2030.0, n), rng.uniform(-15.0, 15.0, n),
+                      rng.uniform(-300.0, 300.0, n)], 1)
+        snap = rng.random(n) < 1.0 / 3.0
+        p[snap, 2] = np.round(p[snap, 2] * 2.0) / 2.0
+        x, y, z = (np.ascontiguousarray(p[:, k]) for k in range(3))
+        _, nx, ny, nz = L.cart_norm(x, y, z)
+        jk = ((z > 0.0).astype(np.int32) + 2 * (x > 2015.0).astype(np.int32))
+        w = np.ones(n, np.float32)
+        dc[tag] = ctx.upload(x, y, z, nx, ny, nz, w, jk)
+        oc[tag] = o.Catalog(x, y, z, nx, ny, nz, w, jk)
+    oax = [o.axis(*a) for a in axes]
+    two = kind in (1, 2)
+    want = [o.count(kind, oc["a"], None, oax[0], oax[-1] if two else None, 4),
+            o.count(kind, oc["a"], oc["b"], oax[0], oax[-1] if two else None, 4)]
+    jobs = [dict(kind=kind, a=dc["a"], b=None, axes=axes, n_jk=4),
+            dict(kind=kind, a=dc["a"], b=dc["b"], axes=axes, n_jk=4)]
+    for method in (0, 1):
+        print(f"[gpu] long column kind={kind} method={method}", flush=True)
+        res, _ = ctx.count(jobs, method=method)
+        for r, (full, sub, acc) in zip(res, want):
+            assert acc > 0
+            assert r["n_pairs"] == acc
+            np.testing.assert_array_equal(r["full"], full)
+            np.testing.assert_array_equal(r["jk"], sub)
+    for c in dc.values():
+        c.free()
