@@ -192,6 +192,7 @@ struct Lds {
 };
 
 typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
 
 // Global-memory views of device pointers.  Loads and atomics through them
 // compile to global_* instructions, which only the vector-memory counter
@@ -230,9 +231,15 @@ __device__ double interp_up(const JobDev &J, double v) {
 	if (v > J.up_max) return b[n - 1];
 	const float fv = (float)v;
 	const float GLOBAL_AS *thr = gld(J.up_thr);
-	int cnt = 0;
-	for (int k = 0; k <= n; ++k) cnt += (thr[k] <= fv);
-	const int seg = cnt - 1;
+	// the number of thresholds <= fv, by bisection (the thresholds are
+	// monotone; a NaN fv counts none, as in a scan)
+	int lo = 0, hi = n + 1;   // thr[k] <= fv for k < lo, > fv for k >= hi
+	while (lo < hi) {
+		const int mid = (lo + hi) >> 1;
+		if (thr[mid] <= fv) lo = mid + 1;
+		else hi = mid;
+	}
+	const int seg = lo - 1;
 	if (seg < 0) return b[0];
 	if (seg >= n - 1) return b[n - 1];
 	return b[seg] + (b[seg + 1] - b[seg]) * (v - c[seg]) / (c[seg + 1] - c[seg]);
@@ -241,23 +248,45 @@ __device__ double interp_up(const JobDev &J, double v) {
 // float(popcount(m1 & m2)) over n_words words at word offset off of each
 // object's masks (the bit mask at 0, the dither mask after it), 0 counting
 // as 1 (correlators_invpweight.cpp:55-59, 62-68).
+// CH words of each side per step (2 or 4); the batched s-mu kernel takes 2
+// to stay within its registers
+template <int CH = 4>
 __device__ __forceinline__ float popc_and(const JobDev &J, int gi, int gj, int off) {
 	const uint64_t GLOBAL_AS *m1 = gld(J.A.masks) + (size_t)gi * J.A.m_stride + off;
 	const uint64_t GLOBAL_AS *m2 = gld(J.B.masks) + (size_t)gj * J.B.m_stride + off;
-	int c = 0;
-	for (int k = 0; k < J.n_words; ++k) c += __popcll(m1[k] & m2[k]);
+	const int nw = J.n_words;
+	int c = 0, k = 0;
+	// rows at even word offsets (16-byte aligned): CH words of each side per
+	// step in 16-byte loads, all issued before any is used
+	if (((J.A.m_stride | J.B.m_stride | off) & 1) == 0) {
+		for (; k + CH <= nw; k += CH) {
+			u32x4 a[CH / 2], b[CH / 2];
+#pragma unroll
+			for (int t = 0; t < CH / 2; ++t) {
+				a[t] = ((const u32x4 GLOBAL_AS *)(m1 + k))[t];
+				b[t] = ((const u32x4 GLOBAL_AS *)(m2 + k))[t];
+			}
+#pragma unroll
+			for (int t = 0; t < CH / 2; ++t)
+				c += __popc(a[t].x & b[t].x) + __popc(a[t].y & b[t].y) + __popc(a[t].z & b[t].z) +
+				     __popc(a[t].w & b[t].w);
+		}
+	}
+	for (; k < nw; ++k) c += __popcll(m1[k] & m2[k]);
 	float pc = (float)c;
 	if (pc < 0.5f) pc += 1.0f;
 	return pc;
 }
 
-__device__ __forceinline__ float popc_weight(const JobDev &J, int gi, int gj) { return popc_and(J, gi, gj, 0); }
+template <int CH = 4>
+__device__ __forceinline__ float popc_weight(const JobDev &J, int gi, int gj) { return popc_and<CH>(J, gi, gj, 0); }
 
 // The numerator of the inverse-p rules: n_runs, or under dither weights the
 // dither popcount (correlators_invpweight.cpp:62-70, 749-757).  Both are
 // integers, exact in float.
+template <int CH = 4>
 __device__ __forceinline__ float invp_numerator(const JobDev &J, int gi, int gj) {
-	return J.dither ? popc_and(J, gi, gj, J.n_words) : (float)J.n_runs;
+	return J.dither ? popc_and<CH>(J, gi, gj, J.n_words) : (float)J.n_runs;
 }
 
 // Pair weight under the job's rule, in the reference's arithmetic
@@ -291,19 +320,20 @@ __device__ float pair_weight(const JobDev &J, int gi, int gj, const double *ni, 
 // The rules of pair_weight that need more than the two staged weights, given
 // their float product wp (w1 * w2, or iw1 * iw2 for the input-weight rules):
 // the same operations in the same order.
+template <int CH>
 __device__ float pair_weight_staged(const JobDev &J, int gi, int gj, const double *ni, const double *nj, float wp) {
 	switch (J.wmode) {
 	case TWOPCF_W_INVP_3D: {
-		const float pc = popc_weight(J, gi, gj);
-		double t = interp_up(J, ref_one_minus_cos(ni, nj)) * (double)invp_numerator(J, gi, gj);
+		const float pc = popc_weight<CH>(J, gi, gj);
+		double t = interp_up(J, ref_one_minus_cos(ni, nj)) * (double)invp_numerator<CH>(J, gi, gj);
 		t = t / (double)pc;
 		return wp * (float)t;
 	}
 	case TWOPCF_W_ANGUP_3D:
 		return (float)((double)wp * interp_up(J, ref_one_minus_cos(ni, nj)));
 	case TWOPCF_W_ANG_INVP: {
-		const float pc = popc_weight(J, gi, gj);
-		return wp * (invp_numerator(J, gi, gj) / pc);
+		const float pc = popc_weight<CH>(J, gi, gj);
+		return wp * (invp_numerator<CH>(J, gi, gj) / pc);
 	}
 	default:
 		return wp;
@@ -625,7 +655,7 @@ __device__ void process_item_fast(const JobDev &J, const WorkItem &W, const int4
 			const int g1 = W.i_begin + l2;
 			const double ni[3] = {gld(J.A.nx)[g1], gld(J.A.ny)[g1], gld(J.A.nz)[g1]};
 			const double nj[3] = {gld(J.B.nx)[g2], gld(J.B.ny)[g2], gld(J.B.nz)[g2]};
-			wp = pair_weight_staged(J, g1, g2, ni, nj, wp);
+			wp = pair_weight_staged<KIND == K_S_MU ? 2 : 4>(J, g1, g2, ni, nj, wp);
 		}
 		return wp;
 	};
