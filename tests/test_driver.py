@@ -5,6 +5,7 @@ import os
 import shutil
 import subprocess
 
+import numpy as np
 import pytest
 
 from conftest import GOLDEN, ROOT
@@ -100,7 +101,7 @@ INVP_PARAMS = dict(
 )
 
 
-def _expected_invp(o, inp):
+def _expected_invp(o, inp, angular=None):
     """The inverse-p pipeline from the oracle: positions (cosmology table,
     Galaxy::set), IIP weights (io.cpp:578-582, with dither masks
     io.cpp:566-572), angular DD/DR with parent and
@@ -135,6 +136,8 @@ def _expected_invp(o, inp):
     addi = fin(3, D, None, ang, None, o.WOpt(4, N_RUNS, None, none, 0, dith))[0]
     adr = fin(3, D, R, ang, None, o.WOpt(3, N_RUNS, None, none))[0]
     adri = fin(3, D, R, ang, None, o.WOpt(5, N_RUNS, None, none))[0]
+    if angular is not None:
+        angular.update({"add.hdf5": add, "addinv.hdf5": addi, "adr.hdf5": adr, "adrinv.hdf5": adri})
     up_dd = np.ascontiguousarray(add / addi)
     up_dr = np.ascontiguousarray(adr / adri)
     files = {}
@@ -213,3 +216,160 @@ def test_driver_invp_pipeline(method, dither, tmp_path, oracle_lib):
         assert len(got) == len(rows), fn
         for g, w in zip(got, rows):
             np.testing.assert_allclose(g, w, rtol=1.5e-5, atol=0, err_msg=fn)
+
+
+# --------------------------------------------------------------------- hdf5 --
+
+def _fmt(v):
+    """One value as the reference's ascii writer prints it (ostream default:
+    %g with 6 significant digits; NaN with its sign)."""
+    import math
+    if isinstance(v, int):
+        return str(v)
+    if math.isnan(v):
+        return "-nan" if math.copysign(1.0, v) < 0 else "nan"
+    return "%g" % v
+
+
+def _h5_as_text(path, group, two_d):
+    """The rows output_xi_1D/2D would write in ascii, rebuilt from the hdf5
+    file's datasets (axes from full_result, counts and xi from the group)."""
+    import h5mini
+    with h5mini.File(path) as f:
+        g = {t: f.dataset(group, t)[0].reshape(-1) for t in ("DD", "DR", "RR", "xi")}
+        if not two_d:
+            c, w = f.dataset("full_result", "Bin_centre")[0], f.dataset("full_result", "Bin_width")[0]
+            rows = [[i, c[i], w[i]] + [g[t][i] for t in ("xi", "DD", "DR", "RR")] for i in range(c.size)]
+        else:
+            c1, w1 = f.dataset("full_result", "Axis_1_bin_centre")[0], f.dataset("full_result", "Axis_1_bin_width")[0]
+            c2, w2 = f.dataset("full_result", "Axis_2_bin_centre")[0], f.dataset("full_result", "Axis_2_bin_width")[0]
+            rows = [[i, j, c1[i], c2[j], w1[i], w2[j]] + [g[t][i * c2.size + j] for t in ("xi", "DD", "DR", "RR")]
+                    for i in range(c1.size) for j in range(c2.size)]
+    return "".join("\t".join(_fmt(float(v)) if k >= (2 if two_d else 1) else str(int(v)) for k, v in enumerate(r))
+                   + "\n" for r in rows)
+
+
+def _fields(line):
+    """The fields of a catalogue line as the ascii reader splits it: on ','
+    if the line has one, else on tab if it has one, else on space; empty
+    parts skipped, spaces removed."""
+    line = line.rstrip("\r\n")
+    sep = "," if "," in line else ("\t" if "\t" in line else " ")
+    return [f.replace(" ", "") for f in line.split(sep) if f.replace(" ", "")]
+
+
+def _params(path):
+    out = {}
+    for line in open(path):
+        if "=" in line and not line.startswith("#"):
+            k, v = line.split("#")[0].split("=", 1)
+            out[k.strip()] = v.strip()
+    return out
+
+
+H5_NAMES = dict(ra_x_dataset_name="RA", dec_y_dataset_name="DEC", z_z_dataset_name="Z", weight_dataset_name="W",
+                jk_dataset_name="JK", bitwise_weight_dataset_name="BITWEIGHT", dither_weight_dataset_name="DITHER")
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("method", ["fp64", "fp32"])
+@pytest.mark.parametrize("case", CASES)
+def test_driver_hdf5_in_and_out(case, method, tmp_path):
+    """The driver with hdf5 catalogues (io.cpp:417-520) and hdf5 output
+    (io.cpp:653-975): the same catalogues as the golden case, converted to
+    datasets (weights as the float values strtof gives), and every output
+    group, printed as the ascii writer would, equals the reference binary's
+    ascii file of that group byte for byte."""
+    import h5mini
+    from test_host_reader import strtof
+    if h5mini.lib() is None:
+        pytest.skip("no libhdf5")
+    print(f"[gpu] driver hdf5 {case} {method}", flush=True)
+    src = os.path.join(GOLDEN, "driver", case)
+    p = _params(os.path.join(src, "params.txt"))
+    use_w, n_jk = int(p["use_weights"]) != 0, int(p["n_jk_regions"])
+    for tag in ("data", "rand"):
+        rows = [_fields(l) for l in open(os.path.join(src, tag + ".txt")) if l.strip() and not l.startswith("#")]
+        cols = {n: np.array([float(r[k]) for r in rows]) for k, n in enumerate(("RA", "DEC", "Z"))}
+        k = 3
+        if use_w:
+            cols["W"] = np.array([strtof(r[k]) for r in rows])
+            k += 1
+        if n_jk > 1:
+            cols["JK"] = np.array([int(r[k]) for r in rows], np.int32)
+        h5mini.write(str(tmp_path / (tag + ".h5")), cols)
+    q = dict(p, **H5_NAMES, data_file_type="hdf5", random_file_type="hdf5", data_filename="data.h5",
+             random_filename="rand.h5")
+    outs = {}
+    for key, stem, two_d in (("monopole", "mono", False), ("sigma_pi", "sp", True), ("s_mu", "smu", True)):
+        q[f"{key}_output_type"] = "hdf5"
+        q[f"{key}_filename"] = stem + ".h5"
+        outs[stem] = two_d
+    (tmp_path / "params.txt").write_text("".join(f"{k} = {v}\n" for k, v in q.items()))
+    env = dict(os.environ, TWOPCF_METHOD=method)
+    r = subprocess.run([DRIVER, "params.txt"], cwd=tmp_path, env=env, capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr
+    exp = os.path.join(src, "expected")
+    names = sorted(os.listdir(exp))
+    assert names
+    for fn in names:
+        stem = fn[:-4].rstrip("0123456789")
+        sub = fn[len(stem):-4]
+        group = "full_result" if sub == "" else f"jk_reg{sub}"
+        got = _h5_as_text(str(tmp_path / (stem + ".h5")), group, outs[stem])
+        with open(os.path.join(exp, fn)) as f:
+            assert got == f.read(), f"{case}/{fn} ({group}) differs"
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("dither", [False, True], ids=["bitwise", "dither"])
+def test_driver_invp_hdf5_handoff(dither, tmp_path, oracle_lib):
+    """twopcf_invp on hdf5 catalogues with the bit masks in BITWEIGHT0, 1
+    (and dither masks in DITHER0, 1), as the reference reads them
+    (io.cpp:521-588).  Run 1 computes the four angular histograms, writes
+    them to their HDF5 files (JKHists::output_hdf5) and the 3-D outputs as
+    hdf5; run 2 computes no angular histogram and reads the files of run 1
+    (read_hdf5_hist).  The angular files and both runs' outputs equal the
+    oracle's pipeline to 1e-11 (weighted sums added in another order)."""
+    import h5mini
+    if h5mini.lib() is None:
+        pytest.skip("no libhdf5")
+    print(f"[gpu] driver inverse-p hdf5 dither={dither}", flush=True)
+    inp = _invp_inputs(dither=dither)
+    for tag, a in inp.items():
+        cols = dict(RA=a["ra"], DEC=a["dec"], Z=a["z"], W=a["w"].astype(np.float64), JK=a["jk"].astype(np.int32))
+        for k in range(a["masks"].shape[1]):
+            cols[f"BITWEIGHT{k}"] = np.ascontiguousarray(a["masks"][:, k]).view(np.int64)
+            if dither:
+                cols[f"DITHER{k}"] = np.ascontiguousarray(a["dither"][:, k]).view(np.int64)
+        h5mini.write(str(tmp_path / (tag + ".h5")), cols)
+    angular = {}
+    want = _expected_invp(oracle_lib, inp, angular)
+    env = dict(os.environ, TWOPCF_METHOD="fp32", TWOPCF_DITHER="1" if dither else "0")
+    base = dict(INVP_PARAMS, **H5_NAMES, data_file_type="hdf5", random_file_type="hdf5", data_filename="data.h5",
+                random_filename="rand.h5", monopole_output_type="hdf5", s_mu_output_type="hdf5")
+    for run, calc in ((1, 1), (2, 0)):
+        q = dict(base, monopole_filename=f"mono_r{run}.h5", s_mu_filename=f"smu_r{run}.h5",
+                 calculate_angular_dd=calc, calculate_angular_dd_invpweights=calc, calculate_angular_dr=calc,
+                 calculate_angular_dr_invpweights=calc)
+        (tmp_path / "params.txt").write_text("".join(f"{k} = {v}\n" for k, v in q.items()))
+        r = subprocess.run([DRIVER_INVP, "params.txt"], cwd=tmp_path, env=env, capture_output=True, text=True,
+                           timeout=300)
+        assert r.returncode == 0, r.stderr
+        if run == 1:
+            for fn, full in angular.items():
+                with h5mini.File(str(tmp_path / fn)) as f:
+                    np.testing.assert_allclose(f.dataset("full_result", "hist")[0], full, rtol=1e-11, atol=0,
+                                               err_msg=fn)
+        else:
+            assert "Reading file: add.hdf5" in r.stdout
+        for fn, rows in want.items():
+            stem = fn[:-4].rstrip("0123456789")
+            sub = fn[len(stem):-4]
+            group = "full_result" if sub == "" else f"jk_reg{sub}"
+            with h5mini.File(str(tmp_path / f"{stem}_r{run}.h5")) as f:
+                g = {t: f.dataset(group, t)[0].reshape(-1) for t in ("xi", "DD", "DR", "RR")}
+            k0 = 6 if stem == "smu" else 3
+            w = np.array(rows, dtype=np.float64)
+            for k, t in enumerate(("xi", "DD", "DR", "RR")):
+                np.testing.assert_allclose(g[t], w[:, k0 + k], rtol=1e-11, atol=1e-13, err_msg=f"run {run} {fn} {t}")

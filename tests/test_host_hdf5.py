@@ -1,0 +1,288 @@
+"""The host layer's hdf5 file type (CPU, no GPU): the catalogue reader
+(io.cpp:417-590) against the ascii reader on the same values, its errors,
+the correlation-function writers (io.cpp:653-975) and the angular histogram
+hand-off (JKHists::output_hdf5 and read_hdf5_hist, hist.cpp:115-221).
+
+Files are read back through libhdf5 with tests/h5mini.py, and the layout is
+checked a second time with the h5dump tool where it is installed."""
+import os
+import shutil
+import subprocess
+
+import numpy as np
+import pytest
+
+from conftest import ROOT
+import h5mini
+
+pytestmark = pytest.mark.skipif(h5mini.lib() is None, reason="no libhdf5 on this host")
+
+HOST = os.path.join(ROOT, "two_pcf_amd", "csrc", "host")
+LIB_DIR = os.path.join(ROOT, "two_pcf_amd")
+
+
+def _compile(main, out):
+    subprocess.run(["g++", "-O2", "-std=c++17", "-pthread", "-ffp-contract=off", main,
+                    os.path.join(HOST, "twopcf_ref_api.cpp"), os.path.join(HOST, "h5lib.cpp"),
+                    "-L", LIB_DIR, "-ltwopcf_hip", f"-Wl,-rpath,{LIB_DIR}", "-ldl", "-o", out], check=True)
+
+
+@pytest.fixture(scope="module")
+def harnesses(tmp_path_factory):
+    if not os.path.exists(os.path.join(LIB_DIR, "libtwopcf_hip.so")):
+        pytest.skip("libtwopcf_hip.so is not built")
+    d = tmp_path_factory.mktemp("h5harness")
+    reader, writer = str(d / "host_reader"), str(d / "host_hdf5")
+    _compile(os.path.join(ROOT, "tests", "host_reader_main.cpp"), reader)
+    _compile(os.path.join(ROOT, "tests", "host_hdf5_main.cpp"), writer)
+    return reader, writer
+
+
+BASE = dict(omega_m=0.25, h=1.0, z_min=0.0, z_max=1.0, ra_x_dataset_name="RA", dec_y_dataset_name="DEC",
+            z_z_dataset_name="Z", weight_dataset_name="W", jk_dataset_name="JK",
+            bitwise_weight_dataset_name="BITWEIGHT", dither_weight_dataset_name="DITHER")
+
+
+def _read(reader, tmp_path, params, tag, mode):
+    p = tmp_path / f"p_{tag}.txt"
+    p.write_text("".join(f"{k} = {v}\n" for k, v in params.items()))
+    return subprocess.run([reader, str(p), str(mode)], capture_output=True, text=True, timeout=120)
+
+
+def _rows(out):
+    lines = out.splitlines()
+    k = max(i for i, l in enumerate(lines) if l.startswith("n "))
+    return lines[k:]
+
+
+def _catalogue(n, seed, eq, n_words=0, dither=False):
+    rng = np.random.Generator(np.random.PCG64(seed))
+    if eq:
+        c1, c2, c3 = rng.uniform(150, 250, n), rng.uniform(0, 60, n), rng.uniform(0.2, 0.6, n)
+        c3[:2] = [0.0, -0.5]   # table start and a negative redshift
+    else:
+        c1, c2, c3 = (rng.uniform(-300, 900, n) for _ in range(3))
+    w = rng.uniform(0.5, 1.5, n).astype(np.float32).astype(np.float64)   # float values: both readers agree
+    jk = rng.integers(0, 5, n).astype(np.int32)
+    cols = dict(RA=c1, DEC=c2, Z=c3, W=w, JK=jk)
+    masks = []
+    for k in range(n_words):
+        m = rng.integers(0, 2 ** 63, n, dtype=np.int64) * rng.choice([-1, 1], n)
+        m[:3] = [0, -1, 1]
+        cols[f"BITWEIGHT{k}"] = m
+        masks.append(m)
+    dmasks = []
+    if dither:
+        for k in range(n_words):
+            m = rng.integers(0, 2 ** 20, n, dtype=np.int64)
+            cols[f"DITHER{k}"] = m
+            dmasks.append(m)
+    text = "".join(
+        "%.17g %.17g %.17g %.9g %d%s\n" % (c1[i], c2[i], c3[i], w[i], jk[i],
+                                          "".join(" %d" % (int(m[i]) % 2 ** 64) for m in masks + dmasks))
+        for i in range(n))
+    return cols, text
+
+
+@pytest.mark.parametrize("eq", [False, True], ids=["cartesian", "equatorial"])
+@pytest.mark.parametrize("mode", [0, 1, 2], ids=["plain", "inverse-p", "dither"])
+def test_hdf5_reader_matches_ascii(harnesses, tmp_path, eq, mode):
+    """Positions, unit vectors, distances, weights, input weights, regions,
+    bit and dither masks and the IIP weights from an hdf5 catalogue equal the
+    ascii reader's on the same values, bit for bit."""
+    reader, _ = harnesses
+    n_words = 2 if mode else 0
+    cols, text = _catalogue(257, 11 + mode, eq, n_words, dither=mode == 2)
+    h5mini.write(str(tmp_path / "cat.h5"), cols)
+    (tmp_path / "cat.txt").write_text(text)
+    p = dict(BASE, coord_system="equatorial" if eq else "cartesian", use_weights=1, n_jk_regions=5,
+             n_bitwise_runs=100)
+    a = _read(reader, tmp_path, dict(p, data_filename=tmp_path / "cat.txt", data_file_type="ascii"), "a", mode)
+    h = _read(reader, tmp_path, dict(p, data_filename=tmp_path / "cat.h5", data_file_type="hdf5"), "h", mode)
+    assert a.returncode == 0, a.stderr
+    assert h.returncode == 0, h.stderr
+    assert "File type hdf5" in h.stdout and "N points in file: 257" in h.stdout
+    ra, rh = _rows(a.stdout), _rows(h.stdout)
+    assert ra[0] == f"n 257 {n_words}"
+    assert rh == ra
+
+
+def test_hdf5_reader_unweighted_no_regions(harnesses, tmp_path):
+    """use_weights = 0 and n_jk_regions <= 1: the weight and region datasets
+    are not read (they may be absent); weights are 1 and regions 0."""
+    reader, _ = harnesses
+    cols, _ = _catalogue(50, 3, False)
+    del cols["W"], cols["JK"]
+    h5mini.write(str(tmp_path / "cat.h5"), cols)
+    p = dict(BASE, coord_system="cartesian", use_weights=0, n_jk_regions=0, data_filename=tmp_path / "cat.h5",
+             data_file_type="hdf5")
+    r = _read(reader, tmp_path, p, "u", 0)
+    assert r.returncode == 0, r.stderr
+    rows = [l.split() for l in _rows(r.stdout)[1:]]
+    assert len(rows) == 50
+    assert all(float.fromhex(x[6]) == 1.0 and x[8] == "0" for x in rows)
+    np.testing.assert_array_equal([float.fromhex(x[0]) for x in rows], cols["RA"])
+
+
+def test_hdf5_reader_errors(harnesses, tmp_path):
+    reader, _ = harnesses
+    cols, _ = _catalogue(20, 5, False, n_words=2)
+    p = dict(BASE, coord_system="cartesian", use_weights=1, n_jk_regions=5, n_bitwise_runs=100,
+             data_file_type="hdf5", data_filename=tmp_path / "cat.h5")
+    # a missing mask dataset
+    c = dict(cols)
+    del c["BITWEIGHT1"]
+    h5mini.write(str(tmp_path / "cat.h5"), c)
+    r = _read(reader, tmp_path, p, "e1", 1)
+    assert r.returncode != 0 and "dataset 'BITWEIGHT1'" in r.stderr
+    # a dataset of another length
+    c = dict(cols, W=cols["W"][:-1])
+    h5mini.write(str(tmp_path / "cat.h5"), c)
+    r = _read(reader, tmp_path, p, "e2", 1)
+    assert r.returncode != 0 and "'W' holds 19 values, expected 20" in r.stderr
+    # a region outside [0, n_jk_regions)
+    c = dict(cols, JK=np.where(np.arange(20) == 7, 5, cols["JK"]).astype(np.int32))
+    h5mini.write(str(tmp_path / "cat.h5"), c)
+    r = _read(reader, tmp_path, p, "e3", 1)
+    assert r.returncode != 0 and "entry 8: jackknife region 5 outside [0, 5)" in r.stderr
+    # not an HDF5 file
+    (tmp_path / "cat.h5").write_text("1 2 3\n")
+    r = _read(reader, tmp_path, p, "e4", 1)
+    assert r.returncode != 0 and "HDF5 open failed" in r.stderr
+
+
+def _arrays(out):
+    a = {}
+    for line in out.splitlines():
+        f = line.split()
+        if f and f[0] != "back_n" and all(v.startswith(("0x", "-0x")) for v in f[1:]) and len(f) > 1:
+            a[f[0]] = np.array([float.fromhex(v) for v in f[1:]])
+        elif f and f[0] == "back_n":
+            a["back_n"] = (int(f[1]), int(f[2]))
+    return a
+
+
+def _ls(DD, DR, RR):
+    return np.array([0.0 if r == 0 else d / r - 2.0 * x / r + 1.0 for d, x, r in zip(DD, DR, RR)])
+
+
+@pytest.fixture(scope="module")
+def written(harnesses, tmp_path_factory):
+    _, writer = harnesses
+    d = tmp_path_factory.mktemp("h5out")
+    r = subprocess.run([writer, str(d), "3"], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    return str(d), _arrays(r.stdout)
+
+
+def test_xi_1d_hdf5_layout_and_values(written):
+    """output_xi_1D, hdf5 (io.cpp:762-830): full_result with Axis_name
+    (H5T_C_S1, so one character), Bin_centre, Bin_width, n_jk_regions, DD,
+    DR, RR, xi; jk_reg<k> with DD, DR, RR, xi; values exact, xi the LS
+    estimate with 0 where RR is 0."""
+    d, A = written
+    with h5mini.File(os.path.join(d, "mono.h5")) as f:
+        assert f.members("/") == ["full_result", "jk_reg0", "jk_reg1", "jk_reg2"]
+        assert f.members("full_result") == ["Bin_centre", "Bin_width", "DD", "DR", "RR", "xi"]
+        assert f.attr_names("full_result") == ["Axis_name", "n_jk_regions"]
+        assert f.attr("full_result", "Axis_name", "str") == (b"s", 1)
+        assert f.attr("full_result", "n_jk_regions", "int") == 3
+        for name, key in (("Bin_centre", "mono_centre"), ("Bin_width", "mono_width")):
+            v, shape, size = f.dataset("full_result", name)
+            assert shape == (12,) and size == 8
+            np.testing.assert_array_equal(v, A[key])
+        full = {t: f.dataset("full_result", t)[0] for t in ("DD", "DR", "RR", "xi")}
+        for t in ("DD", "DR", "RR"):
+            np.testing.assert_array_equal(full[t], A[f"mono_{t}_full"])
+        assert (full["RR"] == 0).any()
+        np.testing.assert_array_equal(full["xi"], _ls(full["DD"], full["DR"], full["RR"]))
+        for r in range(3):
+            g = f"jk_reg{r}"
+            assert f.members(g) == ["DD", "DR", "RR", "xi"]
+            sub = {t: f.dataset(g, t)[0] for t in ("DD", "DR", "RR", "xi")}
+            for t in ("DD", "DR", "RR"):
+                np.testing.assert_array_equal(sub[t], A[f"mono_{t}_sub"][r * 12:(r + 1) * 12])
+            np.testing.assert_array_equal(sub["xi"], _ls(sub["DD"], sub["DR"], sub["RR"]))
+
+
+@pytest.mark.parametrize("stem,n1,n2,names", [("sp", 8, 5, (b"s", b"p")), ("smu", 9, 4, (b"s", b"m"))])
+def test_xi_2d_hdf5_layout_and_values(written, stem, n1, n2, names):
+    """output_xi_2D, hdf5 (io.cpp:882-971): 2-d datasets of n1 x n2 in
+    row-major order (bin = i * n2 + j), axis datasets per axis, one-character
+    axis names, jk_reg<k> groups."""
+    d, A = written
+    nb = n1 * n2
+    with h5mini.File(os.path.join(d, stem + ".h5")) as f:
+        assert f.members("/") == ["full_result", "jk_reg0", "jk_reg1", "jk_reg2"]
+        assert f.members("full_result") == ["Axis_1_bin_centre", "Axis_1_bin_width", "Axis_2_bin_centre",
+                                            "Axis_2_bin_width", "DD", "DR", "RR", "xi"]
+        assert f.attr_names("full_result") == ["Axis_1_name", "Axis_2_name", "n_jk_regions"]
+        assert f.attr("full_result", "Axis_1_name", "str") == (names[0], 1)
+        assert f.attr("full_result", "Axis_2_name", "str") == (names[1], 1)
+        for k, n in ((1, n1), (2, n2)):
+            for part in ("centre", "width"):
+                v, shape, _ = f.dataset("full_result", f"Axis_{k}_bin_{part}")
+                assert shape == (n,)
+                np.testing.assert_array_equal(v, A[f"{stem}{k}_{part}"])
+        full = {}
+        for t in ("DD", "DR", "RR", "xi"):
+            v, shape, size = f.dataset("full_result", t)
+            assert shape == (n1, n2) and size == 8
+            full[t] = v.reshape(-1)
+        for t in ("DD", "DR", "RR"):
+            np.testing.assert_array_equal(full[t], A[f"{stem}_{t}_full"])
+        np.testing.assert_array_equal(full["xi"], _ls(full["DD"], full["DR"], full["RR"]))
+        for r in range(3):
+            sub = {t: f.dataset(f"jk_reg{r}", t)[0].reshape(-1) for t in ("DD", "DR", "RR", "xi")}
+            for t in ("DD", "DR", "RR"):
+                np.testing.assert_array_equal(sub[t], A[f"{stem}_{t}_sub"][r * nb:(r + 1) * nb])
+            np.testing.assert_array_equal(sub["xi"], _ls(sub["DD"], sub["DR"], sub["RR"]))
+
+
+def test_angular_hist_hdf5_round_trip(written):
+    """JKHists::output_hdf5 (hist.cpp:180-221): n_jk_regions, n_bins and
+    min, max, base as float attributes; hist, Bin_centre, Bin_width;
+    jk_reg<k>/hist.  read_hdf5_hist (hist.cpp:115-178) rebuilds the same
+    axis constants and bins."""
+    d, A = written
+    with h5mini.File(os.path.join(d, "ang.h5")) as f:
+        assert f.members("/") == ["full_result", "jk_reg0", "jk_reg1", "jk_reg2"]
+        assert f.members("full_result") == ["Bin_centre", "Bin_width", "hist"]
+        assert f.attr_names("full_result") == ["base", "max", "min", "n_bins", "n_jk_regions"]
+        assert f.attr("full_result", "n_jk_regions", "int") == 3
+        assert f.attr("full_result", "n_bins", "int") == 40
+        for k, name in enumerate(("min", "max", "base")):
+            assert f.attr_size("full_result", name) == 4
+            assert f.attr("full_result", name, "double") == A["ang_axis"][k]
+        np.testing.assert_array_equal(f.dataset("full_result", "hist")[0], A["ang_full"])
+        np.testing.assert_array_equal(f.dataset("full_result", "Bin_centre")[0], A["ang_centre"])
+        np.testing.assert_array_equal(f.dataset("full_result", "Bin_width")[0], A["ang_width"])
+        for r in range(3):
+            assert f.members(f"jk_reg{r}") == ["hist"]
+            np.testing.assert_array_equal(f.dataset(f"jk_reg{r}", "hist")[0], A["ang_sub"][r * 40:(r + 1) * 40])
+    assert A["back_n"] == (3, 40)
+    np.testing.assert_array_equal(A["back_full"], A["ang_full"])
+    np.testing.assert_array_equal(A["back_sub"], A["ang_sub"])
+    np.testing.assert_array_equal(A["back_axis"], A["orig_axis"])
+
+
+H5DUMP = shutil.which("h5dump") or ("/opt/conda/bin/h5dump" if os.path.exists("/opt/conda/bin/h5dump") else None)
+
+
+@pytest.mark.skipif(H5DUMP is None, reason="no h5dump")
+def test_layout_seen_by_h5dump(written):
+    """The same layout through an independent reader: h5dump's header view
+    of the sigma-pi file (types, shapes, one-character names) and of the
+    angular file (float attributes)."""
+    d, _ = written
+    sp = subprocess.run([H5DUMP, "-H", os.path.join(d, "sp.h5")], capture_output=True, text=True, timeout=60)
+    assert sp.returncode == 0, sp.stderr
+    t = sp.stdout
+    assert 'GROUP "jk_reg2"' in t and 'GROUP "full_result"' in t
+    assert t.count("DATASPACE  SIMPLE { ( 8, 5 ) / ( 8, 5 ) }") == 4 * 4
+    assert t.count("H5T_IEEE_F64LE") == 4 * 4 + 4
+    assert t.count("STRSIZE 1;") == 2
+    ang = subprocess.run([H5DUMP, "-A", os.path.join(d, "ang.h5")], capture_output=True, text=True, timeout=60)
+    assert ang.returncode == 0, ang.stderr
+    assert ang.stdout.count("H5T_IEEE_F32LE") == 3
+    assert 'ATTRIBUTE "n_bins"' in ang.stdout

@@ -29,7 +29,8 @@ def harness(tmp_path_factory):
     subprocess.run(["g++", "-O2", "-std=c++17", "-pthread", "-ffp-contract=off",
                     os.path.join(ROOT, "tests", "host_reader_main.cpp"),
                     os.path.join(ROOT, "two_pcf_amd", "csrc", "host", "twopcf_ref_api.cpp"),
-                    "-L", lib_dir, "-ltwopcf_hip", f"-Wl,-rpath,{lib_dir}", "-o", out], check=True)
+                    os.path.join(ROOT, "two_pcf_amd", "csrc", "host", "h5lib.cpp"),
+                    "-L", lib_dir, "-ltwopcf_hip", f"-Wl,-rpath,{lib_dir}", "-ldl", "-o", out], check=True)
     return out
 
 

@@ -56,8 +56,9 @@ def build(verbose_isa=False):
         drv = os.path.join(HERE, name)
         run(["g++", "-O2", "-std=c++17", "-pthread", "-ffp-contract=off", "-fno-fast-math", "-Wall",
              f"-DTWOPCF_INVERSE_P={inv}",
-             os.path.join(host, "twopcf_ref_api.cpp"), os.path.join(host, "main.cpp"),
-             "-L", HERE, "-ltwopcf_hip", f"-Wl,-rpath,{HERE}", "-Wl,-rpath,$ORIGIN", "-o", drv + ".tmp"])
+             os.path.join(host, "twopcf_ref_api.cpp"), os.path.join(host, "h5lib.cpp"),
+             os.path.join(host, "main.cpp"),
+             "-L", HERE, "-ltwopcf_hip", f"-Wl,-rpath,{HERE}", "-Wl,-rpath,$ORIGIN", "-ldl", "-o", drv + ".tmp"])
         os.replace(drv + ".tmp", drv)
     return LIB
 

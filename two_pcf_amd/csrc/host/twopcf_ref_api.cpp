@@ -3,6 +3,8 @@
 // bin centres, normalisation, LS estimator) rounds as in the reference build.
 #include "twopcf_ref_api.hpp"
 
+#include "h5lib.hpp"
+
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
@@ -527,9 +529,15 @@ void store_angular_hist(const std::string &fn, const JKHists &h) {
 JKHists *read_angular_hist(const std::string &fn) {
 	auto it = angular_store().find(fn);
 	if (it == angular_store().end()) {
-		std::fprintf(stderr, "twopcf: angular histogram %s was not computed in this run (set the matching "
-		                     "calculate_angular_* key; HDF5 input is not supported)\n", fn.c_str());
-		std::exit(1);
+		// not computed in this run: the file an earlier run wrote
+		if (!std::ifstream(fn).good()) {
+			std::fprintf(stderr, "twopcf: angular histogram %s was neither computed in this run nor found as a file "
+			                     "(set the matching calculate_angular_* key)\n", fn.c_str());
+			std::exit(1);
+		}
+		JKHists *h = nullptr;
+		read_hdf5_hist(fn, &h);
+		return h;
 	}
 	const StoredHist &s = it->second;
 	Hist t(s.min, s.max, s.n_bins, s.base);
@@ -551,8 +559,17 @@ void cross_pair_count_angular_invpweights(const GalaxyPixels *a, const GalaxyPix
 	count_ang(TWOPCF_W_ANG_W, a, b, h, 0);
 }
 
-// correlators_invpweight.cpp:985-1033
+// correlators_invpweight.cpp:985-1033: each histogram is written to its
+// HDF5 file (JKHists::output_hdf5) and kept in the process for the 3-D
+// counts; without a libhdf5 it is only kept in the process.
 void calc_angular_upweight(GalaxyCatalogue *data, GalaxyCatalogue *randoms, Params p) {
+	auto store_angular_hist = [](const std::string &fn, JKHists &h) {
+		::store_angular_hist(fn, h);
+		if (h5::available())
+			h.output_hdf5(fn);
+		else
+			std::cout << "No libhdf5: " << fn << " is kept in this process only\n";
+	};
 	Hist t(0.0, 1.0 - std::cos(p.theta_max), p.theta_n_bins, p.theta_log_base);
 	if (p.calculate_angular_dd) {
 		std::cout << "Calculating angular DDs\n";
@@ -894,75 +911,32 @@ static void parallel_chunks(F work) {
 	for (auto &th : pool) th.join();
 }
 
-// io.cpp:240-415 on a whole-file buffer: lines (not empty, not starting with
-// '#') are found in one pass, then parsed by up to 16 threads into
-// per-object columns; values are the reference's conversions of the same
-// fields.
-GalaxyCatalogue *read_data_ascii(std::string filename, const Params &p) {
-	std::ifstream file(filename, std::ios::binary);
-	if (!file.is_open()) {
-		std::cout << "File not open\n";
-		return new GalaxyCatalogue(0, p.n_jk_regions);
-	}
-	file.seekg(0, std::ios::end);
-	std::string data((size_t)std::max<std::streamoff>(file.tellg(), 0), '\0');
-	file.seekg(0, std::ios::beg);
-	file.read(&data[0], (std::streamsize)data.size());
-	std::vector<std::pair<size_t, size_t>> lines;
-	for (size_t pos = 0; pos < data.size();) {
-		const char *nl = (const char *)std::memchr(data.data() + pos, '\n', data.size() - pos);
-		const size_t end = nl ? (size_t)(nl - data.data()) : data.size();
-		if (end > pos && data[pos] != '#') lines.emplace_back(pos, end);
-		pos = end + 1;
-	}
-	const int n = (int)lines.size();
-	auto *cat = new GalaxyCatalogue(n, p.n_jk_regions);
-	const bool eq = p.coord_system == "equatorial";
-	if (!eq && p.coord_system != "cartesian") {
-		std::cout << "Coord system not found\n";
-		return cat;
-	}
-	// inverse-p build: bit masks as extra integer columns
+// Mask words per object in the inverse-p build (0 otherwise).
+static int mask_words(const Params &p) {
 	if (twopcf_inverse_p_enabled() && p.n_bitwise_runs <= 0) {
 		std::fprintf(stderr, "twopcf: the inverse-p pipeline needs n_bitwise_runs > 0\n");
 		std::exit(1);
 	}
-	const int n_words = twopcf_inverse_p_enabled() ? (p.n_bitwise_runs + 63) / 64 : 0;
-	// with dither weights, n_words more columns follow: the dither mask
-	const int n_cols = n_words * (twopcf_dither_weights_enabled() ? 2 : 1);
-	Columns c;
-	c.c1.resize(n);
-	c.c2.resize(n);
-	c.c3.resize(n);
-	c.w.resize(n);
-	c.jk.resize(n);
-	c.masks.assign((size_t)n * n_cols, 0);
-	std::vector<int> fail_at(n_threads, -1), fail_code(n_threads, PARSE_OK);
-	auto work = [&](int t) {
-		const size_t b = (size_t)n * t / n_threads, e = (size_t)n * (t + 1) / n_threads;
-		for (size_t i = b; i < e; ++i) {
-			const int rc = parse_line(data.data() + lines[i].first, data.data() + lines[i].second, p, n_cols, i, c);
-			if (rc) {
-				fail_at[t] = (int)i;
-				fail_code[t] = rc;
-				return;
-			}
-		}
-	};
-	parallel_chunks(work);
-	for (int t = 0; t < n_threads; ++t)
-		if (fail_at[t] >= 0) {
-			std::fprintf(stderr, "twopcf: %s catalogue line %d: %s\n", filename.c_str(), fail_at[t] + 1,
-			             fail_code[t] == PARSE_MISSING ? "too few columns" : "not a number");
-			std::exit(1);
-		}
+	return twopcf_inverse_p_enabled() ? (p.n_bitwise_runs + 63) / 64 : 0;
+}
+
+// The part of both readers after the columns are read (io.cpp:323-415,
+// 454-588): weights and regions, the bit masks and their IIP weights, then
+// positions: comoving distances and Galaxy::set for equatorial catalogues,
+// the columns themselves for cartesian ones.  Regions outside
+// [0, n_jk_regions) are an error (the reference writes out of bounds); the
+// message names the object by its `unit` ("line" or "entry") number.
+static GalaxyCatalogue *finish_catalogue(GalaxyCatalogue *cat, Columns &c, int n_words, int n_cols, const Params &p,
+                                         const std::string &filename, const char *unit) {
+	const int n = cat->n_gals;
+	const bool eq = p.coord_system == "equatorial";
 	for (int i = 0; i < n; ++i) {
 		Galaxy &g = cat->gals[i];
 		g.weight = c.w[i];
 		g.jk_region = c.jk[i];
 		g.input_weight = g.weight;
 		if (g.jk_region < 0 || g.jk_region >= p.n_jk_regions) {
-			std::fprintf(stderr, "twopcf: %s line %d: jackknife region %d outside [0, %d)\n", filename.c_str(), i + 1,
+			std::fprintf(stderr, "twopcf: %s %s %d: jackknife region %d outside [0, %d)\n", filename.c_str(), unit, i + 1,
 			             g.jk_region, p.n_jk_regions);
 			std::exit(1);
 		}
@@ -1019,11 +993,163 @@ GalaxyCatalogue *read_data_ascii(std::string filename, const Params &p) {
 	return cat;
 }
 
+// io.cpp:240-415 on a whole-file buffer: lines (not empty, not starting with
+// '#') are found in one pass, then parsed by up to 16 threads into
+// per-object columns; values are the reference's conversions of the same
+// fields.
+GalaxyCatalogue *read_data_ascii(std::string filename, const Params &p) {
+	std::ifstream file(filename, std::ios::binary);
+	if (!file.is_open()) {
+		std::cout << "File not open\n";
+		return new GalaxyCatalogue(0, p.n_jk_regions);
+	}
+	file.seekg(0, std::ios::end);
+	std::string data((size_t)std::max<std::streamoff>(file.tellg(), 0), '\0');
+	file.seekg(0, std::ios::beg);
+	file.read(&data[0], (std::streamsize)data.size());
+	std::vector<std::pair<size_t, size_t>> lines;
+	for (size_t pos = 0; pos < data.size();) {
+		const char *nl = (const char *)std::memchr(data.data() + pos, '\n', data.size() - pos);
+		const size_t end = nl ? (size_t)(nl - data.data()) : data.size();
+		if (end > pos && data[pos] != '#') lines.emplace_back(pos, end);
+		pos = end + 1;
+	}
+	const int n = (int)lines.size();
+	auto *cat = new GalaxyCatalogue(n, p.n_jk_regions);
+	const bool eq = p.coord_system == "equatorial";
+	if (!eq && p.coord_system != "cartesian") {
+		std::cout << "Coord system not found\n";
+		return cat;
+	}
+	// inverse-p build: bit masks as extra integer columns; with dither
+	// weights, n_words more columns follow: the dither mask
+	const int n_words = mask_words(p);
+	const int n_cols = n_words * (twopcf_dither_weights_enabled() ? 2 : 1);
+	Columns c;
+	c.c1.resize(n);
+	c.c2.resize(n);
+	c.c3.resize(n);
+	c.w.resize(n);
+	c.jk.resize(n);
+	c.masks.assign((size_t)n * n_cols, 0);
+	std::vector<int> fail_at(n_threads, -1), fail_code(n_threads, PARSE_OK);
+	auto work = [&](int t) {
+		const size_t b = (size_t)n * t / n_threads, e = (size_t)n * (t + 1) / n_threads;
+		for (size_t i = b; i < e; ++i) {
+			const int rc = parse_line(data.data() + lines[i].first, data.data() + lines[i].second, p, n_cols, i, c);
+			if (rc) {
+				fail_at[t] = (int)i;
+				fail_code[t] = rc;
+				return;
+			}
+		}
+	};
+	parallel_chunks(work);
+	for (int t = 0; t < n_threads; ++t)
+		if (fail_at[t] >= 0) {
+			std::fprintf(stderr, "twopcf: %s catalogue line %d: %s\n", filename.c_str(), fail_at[t] + 1,
+			             fail_code[t] == PARSE_MISSING ? "too few columns" : "not a number");
+			std::exit(1);
+		}
+	return finish_catalogue(cat, c, n_words, n_cols, p, filename, "line");
+}
+
+namespace {
+
+[[noreturn]] void h5_fail(const std::string &file, const std::string &what) {
+	std::fprintf(stderr, "twopcf: %s: HDF5 %s failed\n", file.c_str(), what.c_str());
+	std::exit(1);
+}
+
+// One dataset of the catalogue file read whole into dst as `type`; it must
+// hold n values (n < 0: any number, returned).
+long long h5_read_column(h5::hid_t f, const std::string &file, const std::string &name, h5::hid_t type, void *dst,
+                         long long n) {
+	const h5::Lib &H = h5::lib();
+	h5::hid_t d = H.Dopen2(f, name.c_str(), h5::P_DEFAULT);
+	if (d < 0) h5_fail(file, "open of dataset '" + name + "'");
+	h5::hid_t sp = H.Dget_space(d);
+	const long long m = sp < 0 ? -1 : (long long)H.Sget_simple_extent_npoints(sp);
+	if (sp >= 0) H.Sclose(sp);
+	if (m < 0) h5_fail(file, "size of dataset '" + name + "'");
+	if (n >= 0 && m != n) {
+		std::fprintf(stderr, "twopcf: %s: dataset '%s' holds %lld values, expected %lld\n", file.c_str(), name.c_str(),
+		             m, n);
+		std::exit(1);
+	}
+	if (dst && H.Dread(d, type, h5::S_ALL, h5::S_ALL, h5::P_DEFAULT, dst) < 0)
+		h5_fail(file, "read of dataset '" + name + "'");
+	H.Dclose(d);
+	return m;
+}
+
+}  // namespace
+
+// io.cpp:417-590: the columns come from the datasets the parameter file
+// names (ra_x, dec_y, z_z as double; the weight as double, converted to
+// float; the region as int; in the inverse-p build the bit masks from
+// <bitwise_weight_dataset_name>0, 1, ... as long, and with dither weights the
+// dither masks from <dither_weight_dataset_name>0, 1, ...), one value per
+// object; the object count is the ra_x dataset's size.  A file or dataset
+// that cannot be read, or a dataset of another size, is an error.
+GalaxyCatalogue *read_data_hdf5(std::string filename, const Params &p) {
+	const h5::Lib &H = h5::lib();
+	h5::hid_t f = H.Fopen(filename.c_str(), h5::ACC_RDONLY, h5::P_DEFAULT);
+	if (f < 0) h5_fail(filename, "open");
+	std::cout << "File successfully opened\n";
+	const long long n64 = h5_read_column(f, filename, p.ra_x_dataset_name, H.NATIVE_DOUBLE, nullptr, -1);
+	if (n64 > 0x7fffffffLL) h5_fail(filename, "read (more than 2^31 - 1 objects)");
+	const int n = (int)n64;
+	std::cout << "N points in file: " << n << "\n";
+	auto *cat = new GalaxyCatalogue(n, p.n_jk_regions);
+	if (p.coord_system != "equatorial" && p.coord_system != "cartesian") {
+		std::cout << "Cannot find coordinate system\n";
+		H.Fclose(f);
+		return cat;
+	}
+	const int n_words = mask_words(p);
+	const bool dith = twopcf_dither_weights_enabled();
+	const int n_cols = n_words * (dith ? 2 : 1);
+	Columns c;
+	c.c1.resize(n);
+	c.c2.resize(n);
+	c.c3.resize(n);
+	c.w.assign(n, 1.0f);
+	c.jk.assign(n, 0);
+	c.masks.assign((size_t)n * n_cols, 0);
+	h5_read_column(f, filename, p.ra_x_dataset_name, H.NATIVE_DOUBLE, c.c1.data(), n);
+	h5_read_column(f, filename, p.dec_y_dataset_name, H.NATIVE_DOUBLE, c.c2.data(), n);
+	h5_read_column(f, filename, p.z_z_dataset_name, H.NATIVE_DOUBLE, c.c3.data(), n);
+	if (p.use_weights) {
+		std::vector<double> w(n);
+		h5_read_column(f, filename, p.weight_dataset_name, H.NATIVE_DOUBLE, w.data(), n);
+		for (int i = 0; i < n; ++i) c.w[i] = (float)w[i];
+	}
+	if (p.n_jk_regions > 1) h5_read_column(f, filename, p.jk_dataset_name, H.NATIVE_INT, c.jk.data(), n);
+	std::vector<long long> col(n_words > 0 ? n : 0);
+	for (int set = 0; set < (dith ? 2 : 1) && n_words > 0; ++set) {
+		const std::string &base = set ? p.dither_weight_dataset_name : p.bitwise_weight_dataset_name;
+		for (int k = 0; k < n_words; ++k) {
+			h5_read_column(f, filename, base + std::to_string(k), H.NATIVE_LONG, col.data(), n);
+			for (int i = 0; i < n; ++i) c.masks[(size_t)i * n_cols + set * n_words + k] = (uint64_t)col[i];
+		}
+	}
+	H.Fclose(f);
+	return finish_catalogue(cat, c, n_words, n_cols, p, filename, "entry");
+}
+
 // io.cpp:601-627
 GalaxyCatalogue *read_file(std::string filename, std::string file_type, const Params &p) {
 	std::cout << "\nReading file: " << filename << "\n";
-	if (file_type == "ascii") return read_data_ascii(filename, p);
-	std::cout << "Can't find file type for " << filename << " (this build reads ascii catalogues)\n";
+	if (file_type == "ascii") {
+		std::cout << "File type ASCII\n";
+		return read_data_ascii(filename, p);
+	}
+	if (file_type == "hdf5") {
+		std::cout << "File type hdf5\n";
+		return read_data_hdf5(filename, p);
+	}
+	std::cout << "Can't find file type for " << filename << "\n";
 	return new GalaxyCatalogue(0, p.n_jk_regions);
 }
 
@@ -1066,9 +1192,145 @@ static void single_2d(std::string fn, Axis *a1, Axis *a2, double *DD, double *DR
 		}
 }
 
-// io.cpp:741-834 (ascii branch)
+namespace {
+
+// io.cpp:653-697: scalar attributes.  A string attribute has the type
+// H5T_C_S1, one character, so the file holds the first character of the
+// name, as the reference's files do.
+void h5_attr(h5::hid_t loc, const char *name, h5::hid_t type, const void *v, const std::string &file) {
+	const h5::Lib &H = h5::lib();
+	h5::hid_t sp = H.Screate(h5::S_SCALAR);
+	h5::hid_t a = H.Acreate2(loc, name, type, sp, h5::P_DEFAULT, h5::P_DEFAULT);
+	if (a < 0 || H.Awrite(a, type, v) < 0) h5_fail(file, std::string("write of attribute '") + name + "'");
+	H.Aclose(a);
+	H.Sclose(sp);
+}
+
+// io.cpp:699-722: a 1-d dataset of the native double type, or a 2-d one of
+// a copy of it set little-endian, written from row-major doubles.
+void h5_dataset(h5::hid_t loc, const char *name, const double *v, int rank, const h5::hsize_t *dims,
+                const std::string &file) {
+	const h5::Lib &H = h5::lib();
+	h5::hid_t type = H.NATIVE_DOUBLE;
+	if (rank == 2) {
+		type = H.Tcopy(H.NATIVE_DOUBLE);
+		H.Tset_order(type, h5::T_ORDER_LE);
+	}
+	h5::hid_t sp = H.Screate_simple(rank, dims, nullptr);
+	h5::hid_t d = H.Dcreate2(loc, name, type, sp, h5::P_DEFAULT, h5::P_DEFAULT, h5::P_DEFAULT);
+	if (d < 0 || H.Dwrite(d, H.NATIVE_DOUBLE, h5::S_ALL, h5::S_ALL, h5::P_DEFAULT, v) < 0)
+		h5_fail(file, std::string("write of dataset '") + name + "'");
+	H.Dclose(d);
+	H.Sclose(sp);
+	if (rank == 2) H.Tclose(type);
+}
+
+h5::hid_t h5_create(const std::string &file) {
+	h5::hid_t f = h5::lib().Fcreate(file.c_str(), h5::ACC_TRUNC, h5::P_DEFAULT, h5::P_DEFAULT);
+	if (f < 0) h5_fail(file, "create");
+	return f;
+}
+
+h5::hid_t h5_group(h5::hid_t f, const std::string &name, const std::string &file) {
+	h5::hid_t g = h5::lib().Gcreate2(f, name.c_str(), h5::P_DEFAULT, h5::P_DEFAULT, h5::P_DEFAULT);
+	if (g < 0) h5_fail(file, "create of group '" + name + "'");
+	return g;
+}
+
+std::string jk_group(int r) { return "jk_reg" + std::to_string((long long)r); }
+
+// The LS estimate of every bin of one histogram triple.
+std::vector<double> xi_of(const double *DD, const double *DR, const double *RR, int nb) {
+	std::vector<double> xi(nb);
+	for (int b = 0; b < nb; ++b) xi[b] = ls_estimator(DD[b], DR[b], RR[b]);
+	return xi;
+}
+
+// io.cpp:762-830: group full_result (attribute Axis_name "s", datasets
+// Bin_centre, Bin_width, attribute n_jk_regions, datasets DD, DR, RR, xi),
+// then a group jk_reg<k> (DD, DR, RR, xi) per region.
+void xi_1d_hdf5(const std::string &fn, JKHists *DD, JKHists *DR, JKHists *RR) {
+	const h5::Lib &H = h5::lib();
+	Axis &ax = DD->full_hist.ax;
+	const int nb = ax.n_bins;
+	const h5::hsize_t dims[1] = {(h5::hsize_t)nb};
+	h5::hid_t f = h5_create(fn);
+	h5::hid_t g = h5_group(f, "full_result", fn);
+	h5_attr(g, "Axis_name", H.C_S1, "s", fn);
+	std::vector<double> axis(nb);
+	for (int i = 0; i < nb; ++i) axis[i] = ax.bin_centre(i);
+	h5_dataset(g, "Bin_centre", axis.data(), 1, dims, fn);
+	for (int i = 0; i < nb; ++i) axis[i] = ax.bin_width(i);
+	h5_dataset(g, "Bin_width", axis.data(), 1, dims, fn);
+	h5_attr(g, "n_jk_regions", H.NATIVE_INT, &DD->n_jk_regions, fn);
+	h5_dataset(g, "DD", DD->full_hist.bins, 1, dims, fn);
+	h5_dataset(g, "DR", DR->full_hist.bins, 1, dims, fn);
+	h5_dataset(g, "RR", RR->full_hist.bins, 1, dims, fn);
+	h5_dataset(g, "xi", xi_of(DD->full_hist.bins, DR->full_hist.bins, RR->full_hist.bins, nb).data(), 1, dims, fn);
+	H.Gclose(g);
+	for (int r = 0; r < DD->n_jk_regions; ++r) {
+		const size_t o = (size_t)r * nb;
+		g = h5_group(f, jk_group(r), fn);
+		h5_dataset(g, "DD", DD->sub_hist + o, 1, dims, fn);
+		h5_dataset(g, "DR", DR->sub_hist + o, 1, dims, fn);
+		h5_dataset(g, "RR", RR->sub_hist + o, 1, dims, fn);
+		h5_dataset(g, "xi", xi_of(DD->sub_hist + o, DR->sub_hist + o, RR->sub_hist + o, nb).data(), 1, dims, fn);
+		H.Gclose(g);
+	}
+	H.Fclose(f);
+}
+
+// io.cpp:882-971: group full_result (attributes Axis_1_name, Axis_2_name,
+// datasets Axis_1/2_bin_centre and _bin_width, attribute n_jk_regions, 2-d
+// datasets DD, DR, RR, xi of n1 x n2), then jk_reg<k> groups (DD, DR, RR, xi).
+void xi_2d_hdf5(const std::string &fn, const std::string &name1, const std::string &name2, JKHists2D *DD,
+                JKHists2D *DR, JKHists2D *RR) {
+	const h5::Lib &H = h5::lib();
+	Axis &a1 = DD->full_hist.ax1, &a2 = DD->full_hist.ax2;
+	const int nb = DD->full_hist.total_bins;
+	const h5::hsize_t dims[2] = {(h5::hsize_t)a1.n_bins, (h5::hsize_t)a2.n_bins};
+	h5::hid_t f = h5_create(fn);
+	h5::hid_t g = h5_group(f, "full_result", fn);
+	h5_attr(g, "Axis_1_name", H.C_S1, name1.c_str(), fn);
+	h5_attr(g, "Axis_2_name", H.C_S1, name2.c_str(), fn);
+	Axis *axes[2] = {&a1, &a2};
+	const char *names[2][2] = {{"Axis_1_bin_centre", "Axis_1_bin_width"}, {"Axis_2_bin_centre", "Axis_2_bin_width"}};
+	for (int k = 0; k < 2; ++k) {
+		const int n = axes[k]->n_bins;
+		const h5::hsize_t d1[1] = {(h5::hsize_t)n};
+		std::vector<double> axis(n);
+		for (int i = 0; i < n; ++i) axis[i] = axes[k]->bin_centre(i);
+		h5_dataset(g, names[k][0], axis.data(), 1, d1, fn);
+		for (int i = 0; i < n; ++i) axis[i] = axes[k]->bin_width(i);
+		h5_dataset(g, names[k][1], axis.data(), 1, d1, fn);
+	}
+	h5_attr(g, "n_jk_regions", H.NATIVE_INT, &DD->n_jk_regions, fn);
+	h5_dataset(g, "DD", DD->full_hist.bins, 2, dims, fn);
+	h5_dataset(g, "DR", DR->full_hist.bins, 2, dims, fn);
+	h5_dataset(g, "RR", RR->full_hist.bins, 2, dims, fn);
+	h5_dataset(g, "xi", xi_of(DD->full_hist.bins, DR->full_hist.bins, RR->full_hist.bins, nb).data(), 2, dims, fn);
+	H.Gclose(g);
+	for (int r = 0; r < DD->n_jk_regions; ++r) {
+		const size_t o = (size_t)r * nb;
+		g = h5_group(f, jk_group(r), fn);
+		h5_dataset(g, "DD", DD->sub_hist + o, 2, dims, fn);
+		h5_dataset(g, "DR", DR->sub_hist + o, 2, dims, fn);
+		h5_dataset(g, "RR", RR->sub_hist + o, 2, dims, fn);
+		h5_dataset(g, "xi", xi_of(DD->sub_hist + o, DR->sub_hist + o, RR->sub_hist + o, nb).data(), 2, dims, fn);
+		H.Gclose(g);
+	}
+	H.Fclose(f);
+}
+
+}  // namespace
+
+// io.cpp:741-834
 void output_xi_1D(std::string fn, std::string type, JKHists *DD, JKHists *DR, JKHists *RR, GalaxyCatalogue *,
                   GalaxyCatalogue *) {
+	if (type == "hdf5") {
+		xi_1d_hdf5(fn, DD, DR, RR);
+		return;
+	}
 	if (type != "ascii") {
 		std::cout << "Output file type not found\n";
 		return;
@@ -1080,9 +1342,13 @@ void output_xi_1D(std::string fn, std::string type, JKHists *DD, JKHists *DR, JK
 		          &RR->sub_hist[r * nb]);
 }
 
-// io.cpp:861-975 (ascii branch)
-void output_xi_2D(std::string fn, std::string type, std::string, std::string, JKHists2D *DD, JKHists2D *DR,
-                  JKHists2D *RR, GalaxyCatalogue *, GalaxyCatalogue *) {
+// io.cpp:861-975
+void output_xi_2D(std::string fn, std::string type, std::string name1, std::string name2, JKHists2D *DD,
+                  JKHists2D *DR, JKHists2D *RR, GalaxyCatalogue *, GalaxyCatalogue *) {
+	if (type == "hdf5") {
+		xi_2d_hdf5(fn, name1, name2, DD, DR, RR);
+		return;
+	}
 	if (type != "ascii") {
 		std::cout << "File type not found\n";
 		return;
@@ -1092,4 +1358,67 @@ void output_xi_2D(std::string fn, std::string type, std::string, std::string, JK
 	for (int r = 0; r < DD->n_jk_regions; ++r)
 		single_2d(generate_sub_filename(fn, r), &DD->full_hist.ax1, &DD->full_hist.ax2, &DD->sub_hist[r * nb],
 		          &DR->sub_hist[r * nb], &RR->sub_hist[r * nb]);
+}
+
+// hist.cpp:180-221: min, max and base are written as float attributes
+void JKHists::output_hdf5(std::string fn) {
+	const h5::Lib &H = h5::lib();
+	const int nb = full_hist.ax.n_bins;
+	const h5::hsize_t dims[1] = {(h5::hsize_t)nb};
+	h5::hid_t f = h5_create(fn);
+	h5::hid_t g = h5_group(f, "full_result", fn);
+	h5_attr(g, "n_jk_regions", H.NATIVE_INT, &n_jk_regions, fn);
+	h5_attr(g, "n_bins", H.NATIVE_INT, &full_hist.ax.n_bins, fn);
+	h5_attr(g, "min", H.NATIVE_FLOAT, &full_hist.ax.min, fn);
+	h5_attr(g, "max", H.NATIVE_FLOAT, &full_hist.ax.max, fn);
+	h5_attr(g, "base", H.NATIVE_FLOAT, &full_hist.ax.base, fn);
+	h5_dataset(g, "hist", full_hist.bins, 1, dims, fn);
+	std::vector<double> axis(nb);
+	for (int i = 0; i < nb; ++i) axis[i] = full_hist.ax.bin_centre(i);
+	h5_dataset(g, "Bin_centre", axis.data(), 1, dims, fn);
+	for (int i = 0; i < nb; ++i) axis[i] = full_hist.ax.bin_width(i);
+	h5_dataset(g, "Bin_width", axis.data(), 1, dims, fn);
+	H.Gclose(g);
+	for (int r = 0; r < n_jk_regions; ++r) {
+		g = h5_group(f, jk_group(r), fn);
+		h5_dataset(g, "hist", sub_hist + (size_t)r * nb, 1, dims, fn);
+		H.Gclose(g);
+	}
+	H.Fclose(f);
+}
+
+// hist.cpp:115-178: the attributes are read as int / double (the float
+// attributes convert exactly) and the Hist is built from them
+void read_hdf5_hist(std::string fn, JKHists **hist) {
+	std::cout << "\nReading file: " << fn << "\n";
+	const h5::Lib &H = h5::lib();
+	h5::hid_t f = H.Fopen(fn.c_str(), h5::ACC_RDONLY, h5::P_DEFAULT);
+	if (f < 0) h5_fail(fn, "open");
+	h5::hid_t g = H.Gopen2(f, "full_result", h5::P_DEFAULT);
+	if (g < 0) h5_fail(fn, "open of group 'full_result'");
+	auto attr = [&](const char *name, h5::hid_t type, void *v) {
+		h5::hid_t a = H.Aopen(g, name, h5::P_DEFAULT);
+		if (a < 0 || H.Aread(a, type, v) < 0) h5_fail(fn, std::string("read of attribute '") + name + "'");
+		H.Aclose(a);
+	};
+	int n_jk = 0, n_bins = 0;
+	double mn = 0, mx = 0, base = 0;
+	attr("n_jk_regions", H.NATIVE_INT, &n_jk);
+	attr("n_bins", H.NATIVE_INT, &n_bins);
+	attr("min", H.NATIVE_DOUBLE, &mn);
+	attr("max", H.NATIVE_DOUBLE, &mx);
+	attr("base", H.NATIVE_DOUBLE, &base);
+	if (n_bins <= 0 || n_jk < 0) h5_fail(fn, "read (bad n_bins or n_jk_regions)");
+	Hist t((float)mn, (float)mx, n_bins, (float)base);
+	JKHists *h = new JKHists(n_jk, t);
+	h5_read_column(g, fn, "hist", H.NATIVE_DOUBLE, h->full_hist.bins, n_bins);
+	H.Gclose(g);
+	for (int r = 0; r < n_jk; ++r) {
+		g = H.Gopen2(f, jk_group(r).c_str(), h5::P_DEFAULT);
+		if (g < 0) h5_fail(fn, "open of group '" + jk_group(r) + "'");
+		h5_read_column(g, fn, "hist", H.NATIVE_DOUBLE, h->sub_hist + (size_t)r * n_bins, n_bins);
+		H.Gclose(g);
+	}
+	H.Fclose(f);
+	*hist = h;
 }

@@ -10,17 +10,19 @@
 //     boxes.cpp:16-26 does, which is all normalise() relies on.
 //   * Params members are zero-initialised instead of left undefined when a
 //     key is missing (io.cpp:16-237 leaves them uninitialised).
-//   * Only ascii catalogues and ascii output are supported (the _HAVE_HDF5
-//     parts need libhdf5).  In the inverse-p build the ascii reader takes
-//     the bit masks as ceil(n_bitwise_runs / 64) extra integer columns (the
-//     reference reads them from the HDF5 datasets <name>0, <name>1, ...).
+//   * Catalogues and outputs are ascii or hdf5, as in a reference build with
+//     _HAVE_HDF5; libhdf5 is opened at run time on the first hdf5 file
+//     (h5lib.hpp), so ascii runs never need it.  In the inverse-p build the
+//     hdf5 reader takes the bit masks from the datasets <name>0, <name>1, ...
+//     as the reference does, and the ascii reader takes them as
+//     ceil(n_bitwise_runs / 64) extra integer columns.
 //   * The inverse-p build (_USE_INV_WEIGHTS) is a run-time switch,
 //     twopcf_enable_inverse_p(), set by the twopcf_invp driver, and so are
 //     dither weights (_USE_DITHER_WEIGHTS), twopcf_enable_dither_weights(),
-//     which the driver sets when TWOPCF_DITHER=1; the angular
-//     histograms
-//     calc_angular_upweight produces are kept in the process under their
-//     file names instead of HDF5 files (read_hdf5_hist, hist.cpp:115-178).
+//     which the driver sets when TWOPCF_DITHER=1.  The angular histograms
+//     calc_angular_upweight produces are written to their HDF5 files as in
+//     the reference and also kept in the process under their file names, so
+//     the 3-D counts of the same run do not read them back.
 //   * GalaxyPixels accepts healpix_order and ignores it: angular candidates
 //     come from a grid on the unit vectors, a superset of the pixel discs,
 //     so the counts are the same.
@@ -95,7 +97,15 @@ public:
 	void normalise(const GalaxyCatalogue *cat, int n_gals);
 	void normalise(const GalaxyCatalogue *cat_1, int n_gals_1, const GalaxyCatalogue *cat_2, int n_gals_2);
 	double interp(const double value);
+	// hist.cpp:180-221: groups full_result (attributes n_jk_regions, n_bins,
+	// min, max, base; datasets hist, Bin_centre, Bin_width) and jk_reg<k>
+	void output_hdf5(std::string filename);
 };
+
+// hist.cpp:115-178: a JKHists from a file JKHists::output_hdf5 wrote; *hist
+// is a new object (caller deletes).  Exits with a message if the file cannot
+// be opened.
+void read_hdf5_hist(std::string filename, JKHists **hist);
 
 class Hist2D {
 public:
@@ -188,6 +198,7 @@ public:
 };
 
 GalaxyCatalogue *read_data_ascii(std::string filename, const Params &params);
+GalaxyCatalogue *read_data_hdf5(std::string filename, const Params &params);
 GalaxyCatalogue *read_file(std::string filename, std::string file_type, const Params &params);
 double ls_estimator(double DD, double DR, double RR);
 std::string generate_sub_filename(std::string filename, long long sub_number);
@@ -251,10 +262,12 @@ void apply_bitwise_weights(GalaxyCatalogue *cat, const Params &params);
 void twopcf_enable_dither_weights(bool on);
 bool twopcf_dither_weights_enabled();
 
-// The angular histograms of calc_angular_upweight, by file name:
-// JKHists::output_hdf5 / read_hdf5_hist (hist.cpp:115-222) within one process.
+// The angular histograms of calc_angular_upweight, by file name, within one
+// process.  read_angular_hist returns a new object (caller deletes): the one
+// stored under the name in this process, else the HDF5 file of that name
+// (read_hdf5_hist), as the reference's *_invpweights correlators read it.
 void store_angular_hist(const std::string &filename, const JKHists &hist);
-JKHists *read_angular_hist(const std::string &filename);   // new object, caller deletes; exits if absent
+JKHists *read_angular_hist(const std::string &filename);
 
 // The context every entry point uses (device TWOPCF_DEVICE, default 0;
 // counting method TWOPCF_METHOD = fp32 (default) or fp64).  Aborts with a
