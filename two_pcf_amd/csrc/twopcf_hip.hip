@@ -91,7 +91,7 @@ struct JobDev {
 	float min_r_sq, max_r_sq;
 	AxisDev ax1, ax2;
 	int t_s2_off, t1_off, t2_off, tA_off, tB_off;   // offsets into the table pool (floats)
-	int tq_off;                                      // (T, A, B, 0) per s^2 threshold, 16-byte aligned
+	int tq_off;                                      // (A_k, B_k+1) per bin: bin k sure for A_k <= s^2 < B_k+1
 	float s_min, s_ilb_ln2, s_ils, s_inv_step;       // fast s-index estimate
 	int s_log;
 	float mu_h0, mu_c2;                              // fast mu test: 1/2 - delta = h0 - c2 * rsq(s2)
@@ -610,7 +610,7 @@ __device__ void process_item_fast(const JobDev &J, const WorkItem &W, const int4
 	// row blocks of 16 first-side objects this wave screens (uniform)
 	const int rb_n = min(4, max(0, W.i_count - ig * WAVE + 15) >> 4);
 
-	const float4 *tq = (const float4 *)(tab + J.tq_off);
+	const float2 *tq = (const float2 *)(tab + J.tq_off);
 
 	unsigned int n_acc = 0, n_rc = 0;
 
@@ -795,8 +795,8 @@ __device__ void process_item_fast(const JobDev &J, const WorkItem &W, const int4
 			               : KIND == K_S_MU    ? fmaf(s2, rs, -s_min)
 			                                   : __builtin_amdgcn_sqrtf(s2) - s_min;
 			const int k = bin_estimate(sv, s_inv, s_ilb, s_ils, n1_top);
-			const float4 q0 = tq[k], q1 = tq[k + 1];
-			sure = (s2 >= q0.y) && (s2 < q1.z);   // T_k + E <= s2 < T_k+1 - E
+			const float2 q = tq[k];
+			sure = (s2 >= q.x) && (s2 < q.y);   // T_k + E <= s2 < T_k+1 - E
 			if (KIND == K_S_MU) {
 				const float nmu = (float)n2;
 				// mu n2 = |pi| rsq(s2) n2: the midpoint pi times n2, or for the
@@ -1773,6 +1773,16 @@ int build_plan(twopcf_ctx *ctx, const twopcf_job *jobs, int n_jobs, int method, 
 		D.ax2 = AxisDev{H.ax2.n, H.ax2.log ? 1 : 0, H.ax2.minus_min, H.ax2.inv_step};
 		// identical tables (jobs on the same axes) are stored once; every
 		// table starts 16-byte aligned
+		// bin k is sure for A_k <= s2 < B_k+1 (A = T + E rounded up, B = T - E
+		// rounded down): the pairs (A_k, B_k+1), one 8-byte load per pair
+		auto sure_pairs = [](const std::vector<float> &A, const std::vector<float> &B) {
+			std::vector<float> P(2 * A.size());
+			for (size_t k = 0; k < A.size(); ++k) {
+				P[2 * k] = A[k];
+				P[2 * k + 1] = k + 1 < B.size() ? B[k + 1] : -INFINITY;
+			}
+			return P;
+		};
 		auto push = [&](const std::vector<float> &v) {
 			std::string k((const char *)v.data(), v.size() * sizeof(float));
 			auto f = table_at.find(k);
@@ -1810,13 +1820,7 @@ int build_plan(twopcf_ctx *ctx, const twopcf_job *jobs, int n_jobs, int method, 
 				D.t_s2_off = push(T2);
 				D.tA_off = push(A_);
 				D.tB_off = push(B_);
-				std::vector<float> Q(4 * nt, 0.0f);
-				for (size_t k = 0; k < nt; ++k) {
-					Q[4 * k] = T2[k];
-					Q[4 * k + 1] = A_[k];
-					Q[4 * k + 2] = B_[k];
-				}
-				D.tq_off = push(Q);
+				D.tq_off = push(sure_pairs(A_, B_));
 				D.s_min = H.ax1.min;
 				D.s_log = H.ax1.log ? 1 : 0;
 				D.s_ilb_ln2 = (float)((double)H.ax1.ilb * std::log(2.0));
@@ -1881,13 +1885,7 @@ int build_plan(twopcf_ctx *ctx, const twopcf_job *jobs, int n_jobs, int method, 
 			}
 			D.tA_off = push(A_);
 			D.tB_off = push(B_);
-			std::vector<float> Q(4 * H.t_s2.size(), 0.0f);
-			for (size_t k = 0; k < H.t_s2.size(); ++k) {
-				Q[4 * k] = H.t_s2[k];
-				Q[4 * k + 1] = A_[k];
-				Q[4 * k + 2] = B_[k];
-			}
-			D.tq_off = push(Q);
+			D.tq_off = push(sure_pairs(A_, B_));
 			D.s_min = H.ax1.min;
 			D.s_log = H.ax1.log ? 1 : 0;
 			D.s_ilb_ln2 = (float)((double)H.ax1.ilb * std::log(2.0));
