@@ -626,13 +626,22 @@ __device__ void process_item_fast(const JobDev &J, const WorkItem &W, const int4
 			else gadd(&J.jk_u[o], 1ull);
 		}
 	};
-	// lane's copy of the full histogram; bin b is f_stride << b bytes further
+	// lane's copy of the full histogram; bin b is f_stride << b bytes further.
+	// Unweighted jackknife counts keep in it only the pairs whose two
+	// regions differ: each such pair touches two regions, so the full count
+	// is the sum of the touching counts over all regions minus this cross
+	// count, which the host forms exactly in integers.  A pair within one
+	// region then takes one LDS atomic instead of two.
 	char *const F_lane = (char *)(F + f_lane);
 	const int f_bshift = f_shift + (WEIGHTED ? 3 : 2);
-	auto add = [&](int bin, float w, int si, int sj) {
+	const bool cross_only = !WEIGHTED && jk;
+	auto add_full = [&](int bin, float w) {
 		HT *const fp = (HT *)(F_lane + (bin << f_bshift));
 		if (WEIGHTED) atomicAdd((double *)fp, (double)w);
 		else atomicAdd((unsigned int *)fp, 1u);
+	};
+	auto add = [&](int bin, float w, int si, int sj) {
+		if (!cross_only) add_full(bin, w);
 		if (jk) {
 			if (__builtin_amdgcn_ballot_w64(si < 0 || sj < 0) == 0ull) {
 				// every active lane's regions have LDS slots
@@ -641,10 +650,14 @@ __device__ void process_item_fast(const JobDev &J, const WorkItem &W, const int4
 				if (sj != si) {
 					if (WEIGHTED) atomicAdd((double *)&H[sj + bin], (double)w);
 					else atomicAdd((unsigned int *)&H[sj + bin], 1u);
+					if (cross_only) add_full(bin, w);
 				}
 			} else {
 				add_region(si, bin, w);
-				if (sj != si) add_region(sj, bin, w);
+				if (sj != si) {
+					add_region(sj, bin, w);
+					if (cross_only) add_full(bin, w);
+				}
 			}
 		}
 	};
@@ -1141,6 +1154,9 @@ struct PlanLaunch {
 struct PlanOut {
 	bool weighted;
 	size_t full_off, jk_off, full_n, jk_n;
+	// the full pool holds the cross-region count (unweighted jackknife jobs
+	// of the FP32 kernel): full = sum over regions of touching - cross
+	bool cross;
 };
 
 // Everything one twopcf_count call needs on the device, kept for replays.
@@ -2131,7 +2147,8 @@ int build_plan(twopcf_ctx *ctx, const twopcf_job *jobs, int n_jobs, int method, 
 	plan.n_tables = (int)tables.size();
 	plan.n_items = (int)items.size();
 	for (int q = 0; q < n_jobs; ++q)
-		plan.outs.push_back(PlanOut{prep[q].weighted, prep[q].full_off, prep[q].jk_off, prep[q].full_n, prep[q].jk_n});
+		plan.outs.push_back(PlanOut{prep[q].weighted, prep[q].full_off, prep[q].jk_off, prep[q].full_n, prep[q].jk_n,
+		                            fast && !prep[q].weighted && prep[q].jk_n > 0});
 
 	// ---- launches ----
 	// LDS: tile + tables + slot regions + histograms
@@ -2361,8 +2378,17 @@ int twopcf_count(twopcf_ctx *ctx, const twopcf_job *jobs, int n_jobs, int method
 	for (int q = 0; q < n_jobs; ++q) {
 		const twopcf_job &J = jobs[q];
 		const PlanOut &P = plan->outs[q];
-		for (size_t b = 0; b < P.full_n; ++b)
-			J.full[b] = P.weighted ? hd[P.full_off + b] : (double)hu[P.full_off + b];
+		if (P.cross) {
+			// exact in u64: sum_r touching[r] = same + 2 cross
+			for (size_t b = 0; b < P.full_n; ++b) {
+				unsigned long long t = 0;
+				for (size_t e = b; e < P.jk_n; e += P.full_n) t += hu[P.jk_off + e];
+				J.full[b] = (double)(t - hu[P.full_off + b]);
+			}
+		} else {
+			for (size_t b = 0; b < P.full_n; ++b)
+				J.full[b] = P.weighted ? hd[P.full_off + b] : (double)hu[P.full_off + b];
+		}
 		if (J.jk) {
 			if (J.n_jk > 1) {
 				for (size_t e = 0; e < P.jk_n; ++e) J.jk[e] = P.weighted ? hd[P.jk_off + e] : (double)hu[P.jk_off + e];
