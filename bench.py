@@ -40,8 +40,8 @@ FLOP_BY_KIND = {0: 15, 1: 29, 2: 26, 3: 15}
 # 1, 100 regions) under the tuning the autotuner picks for it, from two
 # rocprofv3 --pmc runs: FETCH_SIZE x 2 + WRITE_SIZE x 1 (factors from a 1 GiB
 # copy), KiB -> B.  Source: profiles/k_count_fp32_jackknife_batch_hbm.md
-TRAFFIC_BYTES = {(1.0, 100, 2, 8, 256): (7.772e7 * 2 + 3.601e6) * 1024,
-                 (1.0, 100, 2, 9, 256): (7.890e7 * 2 + 2.876e6) * 1024}
+TRAFFIC_BYTES = {(1.0, 100, 2, 8, 256): (7.797e7 * 2 + 3.212e6) * 1024,
+                 (1.0, 100, 2, 9, 256): (7.897e7 * 2 + 2.532e6) * 1024}
 # region slots that fit in LDS on the default workload (2000-bin u32
 # histograms next to the 1024-entry tile, queues and tables)
 SLOTS_FIT = 9
