@@ -106,6 +106,7 @@ struct JobDev {
 	int row_off, n_rows;                             // stencil rows in the row pool
 	int wmode, n_runs, n_words;                      // pair-weight rule (TWOPCF_W_*)
 	int dither;                                      // inverse-p numerator from the dither masks
+	int mstage;                                      // fp32 path: first-side mask rows in LDS, words per row (0: none)
 	float scr_eps;                                   // fp32 screening: bound on |s^2_mfma - s^2_fp32|
 	int up_n;                                        // angular up-weight table
 	double up_min, up_max;
@@ -189,6 +190,7 @@ __global__ void k_fill_u64(unsigned long long *p, int64_t n, unsigned long long 
 // LDS layout of the counting kernel (bytes), see lds_layout().
 struct Lds {
 	int tile, tables, slots, hist, total;
+	int masks;   // first-side masks of the popcount rules (fp32 path; 0: not staged)
 };
 
 typedef float f32x4 __attribute__((ext_vector_type(4)));
@@ -245,25 +247,30 @@ __device__ double interp_up(const JobDev &J, double v) {
 	return b[seg] + (b[seg + 1] - b[seg]) * (v - c[seg]) / (c[seg + 1] - c[seg]);
 }
 
-// float(popcount(m1 & m2)) over n_words words at word offset off of each
-// object's masks (the bit mask at 0, the dither mask after it), 0 counting
-// as 1 (correlators_invpweight.cpp:55-59, 62-68).
-// CH words of each side per step (2 or 4); the batched s-mu kernel takes 2
-// to stay within its registers
-template <int CH = 4>
-__device__ __forceinline__ float popc_and(const JobDev &J, int gi, int gj, int off) {
-	const uint64_t GLOBAL_AS *m1 = gld(J.A.masks) + (size_t)gi * J.A.m_stride + off;
-	const uint64_t GLOBAL_AS *m2 = gld(J.B.masks) + (size_t)gj * J.B.m_stride + off;
-	const int nw = J.n_words;
+#define LDS_AS __attribute__((address_space(3)))
+template <class P>
+struct VecOf;
+template <>
+struct VecOf<const uint64_t GLOBAL_AS *> {
+	using type = const u32x4 GLOBAL_AS *;
+};
+template <>
+struct VecOf<const uint64_t LDS_AS *> {
+	using type = const u32x4 LDS_AS *;
+};
+
+// popcount(m1 & m2) over nw words; m1 is a row in global memory or in LDS.
+// With vec (both rows 16-byte aligned): CH words of each side per step in
+// 16-byte loads, all issued before any is used.
+template <int CH, class P1>
+__device__ __forceinline__ int popc_rows(P1 m1, const uint64_t GLOBAL_AS *m2, int nw, bool vec) {
 	int c = 0, k = 0;
-	// rows at even word offsets (16-byte aligned): CH words of each side per
-	// step in 16-byte loads, all issued before any is used
-	if (((J.A.m_stride | J.B.m_stride | off) & 1) == 0) {
+	if (vec) {
 		for (; k + CH <= nw; k += CH) {
 			u32x4 a[CH / 2], b[CH / 2];
 #pragma unroll
 			for (int t = 0; t < CH / 2; ++t) {
-				a[t] = ((const u32x4 GLOBAL_AS *)(m1 + k))[t];
+				a[t] = ((typename VecOf<P1>::type)(m1 + k))[t];
 				b[t] = ((const u32x4 GLOBAL_AS *)(m2 + k))[t];
 			}
 #pragma unroll
@@ -273,20 +280,40 @@ __device__ __forceinline__ float popc_and(const JobDev &J, int gi, int gj, int o
 		}
 	}
 	for (; k < nw; ++k) c += __popcll(m1[k] & m2[k]);
+	return c;
+}
+
+// float(popcount(m1 & m2)) over n_words words at word offset off of each
+// object's masks (the bit mask at 0, the dither mask after it), 0 counting
+// as 1 (correlators_invpweight.cpp:55-59, 62-68).  The first side's row is
+// read from global memory, or from its LDS copy m1l when use_l (rows of
+// J.mstage words).  CH words of each side per step (2 or 4); the batched
+// s-mu kernel takes 2 to stay within its registers.
+template <int CH = 4>
+__device__ __forceinline__ float popc_and(const JobDev &J, int gi, int gj, int off, const uint64_t LDS_AS *m1l = nullptr,
+                                          bool use_l = false) {
+	const uint64_t GLOBAL_AS *m2 = gld(J.B.masks) + (size_t)gj * J.B.m_stride + off;
+	const int c = use_l ? popc_rows<CH>(m1l + off, m2, J.n_words, ((J.mstage | J.B.m_stride | off) & 1) == 0)
+	                    : popc_rows<CH>(gld(J.A.masks) + (size_t)gi * J.A.m_stride + off, m2, J.n_words,
+	                                    ((J.A.m_stride | J.B.m_stride | off) & 1) == 0);
 	float pc = (float)c;
 	if (pc < 0.5f) pc += 1.0f;
 	return pc;
 }
 
 template <int CH = 4>
-__device__ __forceinline__ float popc_weight(const JobDev &J, int gi, int gj) { return popc_and<CH>(J, gi, gj, 0); }
+__device__ __forceinline__ float popc_weight(const JobDev &J, int gi, int gj, const uint64_t LDS_AS *m1l = nullptr,
+                                             bool use_l = false) {
+	return popc_and<CH>(J, gi, gj, 0, m1l, use_l);
+}
 
 // The numerator of the inverse-p rules: n_runs, or under dither weights the
 // dither popcount (correlators_invpweight.cpp:62-70, 749-757).  Both are
 // integers, exact in float.
 template <int CH = 4>
-__device__ __forceinline__ float invp_numerator(const JobDev &J, int gi, int gj) {
-	return J.dither ? popc_and<CH>(J, gi, gj, J.n_words) : (float)J.n_runs;
+__device__ __forceinline__ float invp_numerator(const JobDev &J, int gi, int gj, const uint64_t LDS_AS *m1l = nullptr,
+                                                bool use_l = false) {
+	return J.dither ? popc_and<CH>(J, gi, gj, J.n_words, m1l, use_l) : (float)J.n_runs;
 }
 
 // Pair weight under the job's rule, in the reference's arithmetic
@@ -321,19 +348,20 @@ __device__ float pair_weight(const JobDev &J, int gi, int gj, const double *ni, 
 // their float product wp (w1 * w2, or iw1 * iw2 for the input-weight rules):
 // the same operations in the same order.
 template <int CH>
-__device__ float pair_weight_staged(const JobDev &J, int gi, int gj, const double *ni, const double *nj, float wp) {
+__device__ float pair_weight_staged(const JobDev &J, int gi, int gj, const double *ni, const double *nj, float wp,
+                                    const uint64_t LDS_AS *m1l, bool use_l) {
 	switch (J.wmode) {
 	case TWOPCF_W_INVP_3D: {
-		const float pc = popc_weight<CH>(J, gi, gj);
-		double t = interp_up(J, ref_one_minus_cos(ni, nj)) * (double)invp_numerator<CH>(J, gi, gj);
+		const float pc = popc_weight<CH>(J, gi, gj, m1l, use_l);
+		double t = interp_up(J, ref_one_minus_cos(ni, nj)) * (double)invp_numerator<CH>(J, gi, gj, m1l, use_l);
 		t = t / (double)pc;
 		return wp * (float)t;
 	}
 	case TWOPCF_W_ANGUP_3D:
 		return (float)((double)wp * interp_up(J, ref_one_minus_cos(ni, nj)));
 	case TWOPCF_W_ANG_INVP: {
-		const float pc = popc_weight<CH>(J, gi, gj);
-		return wp * (invp_numerator<CH>(J, gi, gj) / pc);
+		const float pc = popc_weight<CH>(J, gi, gj, m1l, use_l);
+		return wp * (invp_numerator<CH>(J, gi, gj, m1l, use_l) / pc);
 	}
 	default:
 		return wp;
@@ -606,6 +634,23 @@ __device__ void process_item_fast(const JobDev &J, const WorkItem &W, const int4
 		cth[tid] = ok ? (s2_hi + J.scr_eps) - aa : -1e30f;
 		ctl[tid] = ok ? (s2_lo - J.scr_eps) - aa : 1e30f;
 	}
+	// the popcount rules read the first side's masks from an LDS copy when
+	// the launch has room for it: rows of J.mstage words (bit mask, then the
+	// dither mask), copied in 16-byte units.  Only the angular and monopole
+	// kernels (small histograms) have the room; the s-mu and sigma-pi ones
+	// keep their registers for binning.
+	constexpr bool can_stage = WEIGHTED && (KIND == K_ANGULAR || KIND == K_MONO);
+	const bool masks_in_lds = can_stage && L.masks != 0 && J.mstage != 0;
+	const uint64_t LDS_AS *m_lds = (const uint64_t LDS_AS *)(smem + L.masks);
+	if (masks_in_lds) {
+		const int ms = J.A.m_stride, upo = ms >> 1;
+		const u32x4 GLOBAL_AS *src = (const u32x4 GLOBAL_AS *)(gld(J.A.masks) + (size_t)W.i_begin * ms);
+		uint64_t LDS_AS *dst = (uint64_t LDS_AS *)(smem + L.masks);
+		for (int e = tid; e < W.i_count * upo; e += BLOCK) {
+			const int o = e / upo, u = e - o * upo;
+			*(u32x4 LDS_AS *)(dst + (size_t)o * J.mstage + 2 * u) = src[e];
+		}
+	}
 	__syncthreads();
 	// row blocks of 16 first-side objects this wave screens (uniform)
 	const int rb_n = min(4, max(0, W.i_count - ig * WAVE + 15) >> 4);
@@ -668,7 +713,8 @@ __device__ void process_item_fast(const JobDev &J, const WorkItem &W, const int4
 			const int g1 = W.i_begin + l2;
 			const double ni[3] = {gld(J.A.nx)[g1], gld(J.A.ny)[g1], gld(J.A.nz)[g1]};
 			const double nj[3] = {gld(J.B.nx)[g2], gld(J.B.ny)[g2], gld(J.B.nz)[g2]};
-			wp = pair_weight_staged<KIND == K_S_MU ? 2 : 4>(J, g1, g2, ni, nj, wp);
+			wp = pair_weight_staged<KIND == K_S_MU ? 2 : 4>(J, g1, g2, ni, nj, wp, m_lds + (size_t)l2 * J.mstage,
+			                                                 masks_in_lds);
 		}
 		return wp;
 	};
@@ -1111,7 +1157,7 @@ __global__ void __launch_bounds__(BLOCK) k_count(const JobDev *__restrict__ jobs
 	float *tab = (float *)(smem + L.tables);
 	for (int k = threadIdx.x; k < n_tables; k += BLOCK) tab[k] = tables[k];
 	HT *hist = (HT *)(smem + L.hist);
-	const int n_hist = (L.total - L.hist) / (int)sizeof(HT);
+	const int n_hist = ((L.masks ? L.masks : L.total) - L.hist) / (int)sizeof(HT);
 	for (int k = threadIdx.x; k < n_hist; k += BLOCK) hist[k] = (HT)0;
 	int it = blockIdx.x;
 	for (;;) {
@@ -1382,6 +1428,7 @@ struct Prepared {
 	size_t full_off, jk_off;   // offsets into the device output pools
 	bool weighted;
 	int f_elems = 0;           // LDS elements of the full histogram (lane copies included)
+	int mask_bytes = 0;        // LDS copy of the first side's masks (popcount rules, fp32 path; 0: none)
 };
 
 // Error model of the FP32 path.  Both objects are held as float offsets
@@ -1740,6 +1787,29 @@ int build_plan(twopcf_ctx *ctx, const twopcf_job *jobs, int n_jobs, int method, 
 	const int lds_fixed = (int)(((std::max(tile_bytes(true), tile_bytes(false)) + 15) & ~15) + ((tab_floats * 4 + 15) & ~15) +
 	                            ((MAX_SLOTS * 4 + 15) & ~15));
 	const size_t lds_budget = ctx->lds_max;
+	// The popcount rules of weighted angular and monopole jobs (small
+	// histograms) read the first side's masks from an LDS copy when it fits
+	// beside one region slot: rows of an odd number of 16-byte units, so
+	// that lanes reading different objects spread over the banks.  Every
+	// weighted job of such a kind shares its launch, so all of them keep at
+	// most 16 lane copies of the full histogram and leave room for the
+	// largest copy of their kind (stage_bytes).
+	auto mask_row = [](int ms) { return ((ms >> 1) & 1) ? ms : ms + 2; };
+	auto wants_copy = [&](const HostJob &H) {
+		const int wm = H.j->weight_mode, ms = H.a->m_stride;
+		return fast && H.weighted && (H.j->kind == TWOPCF_ANGULAR || H.j->kind == TWOPCF_MONOPOLE) &&
+		       (wm == TWOPCF_W_ANG_INVP || wm == TWOPCF_W_INVP_3D) && ms > 0 && (ms & 1) == 0;
+	};
+	int stage_bytes[4] = {0, 0, 0, 0};
+	for (int q = 0; q < n_jobs; ++q) {
+		const HostJob &H = hj[q];
+		if (!wants_copy(H)) continue;
+		const int bytes = ICHUNK * mask_row(H.a->m_stride) * 8;
+		const int hb = H.nb * 8;
+		const int fs = (size_t)H.nb * WAVE * 8 <= 32768 ? 4 : 0;
+		if ((long long)lds_fixed + (hb << fs) + hb + bytes + 16 <= (long long)ctx->lds_max)
+			stage_bytes[H.j->kind] = std::max(stage_bytes[H.j->kind], bytes);
+	}
 	for (int q = 0; q < n_jobs; ++q) {
 		HostJob &H = hj[q];
 		Prepared &P = prep[q];
@@ -1926,6 +1996,12 @@ int build_plan(twopcf_ctx *ctx, const twopcf_job *jobs, int n_jobs, int method, 
 		// small histograms of the fp32 path get one copy per lane
 		D.f_shift = (fast && (size_t)H.nb * WAVE * (H.weighted ? 8 : 4) <= 32768)
 		                ? 6 : 0;
+		// first-side mask copy in LDS (see stage_bytes)
+		if (fast && H.weighted && stage_bytes[H.j->kind] > 0) {
+			D.f_shift = std::min(D.f_shift, 4);
+			P.mask_bytes = stage_bytes[H.j->kind];
+			if (wants_copy(H)) D.mstage = mask_row(H.a->m_stride);
+		}
 		P.f_elems = H.nb << D.f_shift;
 		D.pi_def = H.j->pi_def;
 		D.wmode = H.j->weight_mode;
@@ -1971,8 +2047,10 @@ int build_plan(twopcf_ctx *ctx, const twopcf_job *jobs, int n_jobs, int method, 
 		// cap also keeps (1 + slots) histograms inside the LDS budget
 		const int hist_bytes = H.nb * (H.weighted ? 8 : 4);
 		const int f_bytes = hist_bytes << D.f_shift;
-		const int slot_cap =
-		    std::max(1, std::min(max_slots, (int)((lds_budget - lds_fixed - f_bytes) / std::max(hist_bytes, 1))));
+		const int slot_cap = std::max(
+		    1, std::min(max_slots,
+		                (int)((lds_budget - lds_fixed - f_bytes - (P.mask_bytes ? P.mask_bytes + 16 : 0)) /
+		                      std::max(hist_bytes, 1))));
 		if ((long long)lds_fixed + f_bytes + hist_bytes > (long long)ctx->lds_max)
 			return fail(ctx, TWOPCF_E_LIMIT, "histogram does not fit in LDS");
 		const int n_cell = H.g.dim[0] * H.g.dim[1] * H.g.dim[2];
@@ -2156,19 +2234,24 @@ int build_plan(twopcf_ctx *ctx, const twopcf_job *jobs, int n_jobs, int method, 
 	for (auto &H : hj) max_nb = std::max(max_nb, H.nb);
 	(void)max_tab_n1;
 	// f_elems: elements of the largest full histogram (with its lane copies)
-	auto layout = [&](bool fast_kernel, bool weighted, int nb, int slots, int f_elems) {
+	auto layout = [&](bool fast_kernel, bool weighted, int nb, int slots, int f_elems, int mask_bytes) {
 		Lds L;
 		L.tile = 0;
 		L.tables = (tile_bytes(fast_kernel) + 15) & ~15;
 		L.slots = L.tables + (((int)tables.size() * 4 + 15) & ~15);
 		L.hist = L.slots + ((MAX_SLOTS * 4 + 15) & ~15);
 		L.total = L.hist + (f_elems + nb * slots) * (weighted ? 8 : 4);
+		L.masks = 0;
+		if (mask_bytes > 0) {
+			L.masks = (L.total + 15) & ~15;
+			L.total = L.masks + mask_bytes;
+		}
 		return L;
 	};
 	auto launch = [&](int kind, bool weighted, bool mid, const WorkItem *it, int n_it, int nb, int slots,
-	                  unsigned *queue, int grid, int f_elems) -> int {
+	                  unsigned *queue, int grid, int f_elems, int mask_bytes) -> int {
 		const bool fast_kernel = fast;
-		Lds L = layout(fast_kernel, weighted, nb, slots, f_elems);
+		Lds L = layout(fast_kernel, weighted, nb, slots, f_elems, mask_bytes);
 		if ((size_t)L.total > ctx->lds_max)
 			return fail(ctx, TWOPCF_E_LIMIT, "histogram does not fit in LDS (" + std::to_string(L.total) + " B)");
 		void (*fn)(const JobDev *, const WorkItem *, int, unsigned *, const int4 *, const int *, const float *, int,
@@ -2202,7 +2285,7 @@ int build_plan(twopcf_ctx *ctx, const twopcf_job *jobs, int n_jobs, int method, 
 			int slots = 0;
 			for (size_t k = k0; k < k1; ++k) slots = std::max(slots, items[k].n_slots);
 			if ((rc = launch(hj[q].j->kind, prep[q].weighted, false, d_items + k0, (int)(k1 - k0), prep[q].nb, slots,
-			                 nullptr, (int)(k1 - k0), prep[q].f_elems))) {
+			                 nullptr, (int)(k1 - k0), prep[q].f_elems, 0))) {
 				release();
 				return rc;
 			}
@@ -2234,20 +2317,24 @@ int build_plan(twopcf_ctx *ctx, const twopcf_job *jobs, int n_jobs, int method, 
 			const int kind = std::get<0>(sp);
 			const bool weighted = std::get<1>(sp) != 0;
 			const int mid = std::get<4>(sp);
-			int nb = 1, f_elems = 1;
+			int nb = 1, f_elems = 1, mask_bytes = 0;
 			for (int q = 0; q < n_jobs; ++q)
 				if (hj[q].j->kind == kind && prep[q].weighted == weighted && mid_of(q) == mid) {
 					nb = std::max(nb, hj[q].nb);
 					f_elems = std::max(f_elems, prep[q].f_elems);
+					mask_bytes = std::max(mask_bytes, prep[q].mask_bytes);
 				}
 			int slots = 0;
 			for (size_t k = std::get<2>(sp); k < std::get<2>(sp) + std::get<3>(sp); ++k)
 				slots = std::max(slots, flat[k].n_slots);
-			Lds L = layout(true, weighted, nb, slots, f_elems);
+			// jobs of other rules in the same launch may leave no room for the
+			// mask copy: then every job of it reads its masks from global memory
+			if ((size_t)layout(true, weighted, nb, slots, f_elems, mask_bytes).total > ctx->lds_max) mask_bytes = 0;
+			Lds L = layout(true, weighted, nb, slots, f_elems, mask_bytes);
 			const int per_cu = std::max(1, std::min(8, (int)(ctx->lds_max / std::max(L.total, 1))));
 			const int grid = std::min<long long>((long long)ctx->n_cu * per_cu, (long long)std::get<3>(sp));
 			if ((rc = launch(kind, weighted, mid != 0, d_items + std::get<2>(sp), (int)std::get<3>(sp), nb, slots,
-			                 d_queue + qi, std::max(grid, 1), f_elems))) {
+			                 d_queue + qi, std::max(grid, 1), f_elems, mask_bytes))) {
 				release();
 				return rc;
 			}
