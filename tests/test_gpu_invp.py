@@ -167,14 +167,16 @@ def test_dither_weights(oracle_lib):
             ctx.count([dict(jobs[0], a=plain)], method=1)
 
 
-@pytest.mark.parametrize("words", [1, 2, 3, 4, 5])
-def test_popcount_rules_word_counts(oracle_lib, words):
+@pytest.mark.parametrize("words,dither", [(1, False), (2, False), (3, False), (4, False), (5, False), (3, True),
+                                          (2, True)])
+def test_popcount_rules_word_counts(oracle_lib, words, dither):
     """Angular DD with inverse-p weights and monopole DD with the 3-D
     inverse-p rule, masks of 1 to 5 words.  The batched kernels copy the
-    first side's masks into LDS for even word counts (rows padded to an odd
-    number of 16-byte units: 2 -> 2, 4 -> 6 words) and read both sides from
-    global memory for odd ones (the one-word tail loop); both methods equal
-    the oracle."""
+    first side's masks into LDS when a row (bit mask, then dither mask) has an
+    even word count (rows padded to an odd number of 16-byte units: 2 -> 2,
+    4 -> 6 words) and read both sides from global memory otherwise (the
+    one-word tail loop); with dither masks of 3 words the dither popcount
+    starts at an odd word of the LDS row.  Both methods equal the oracle."""
     from two_pcf_amd import lib as L
     o = oracle_lib
     n_runs = 64 * words
@@ -188,21 +190,29 @@ def test_popcount_rules_word_counts(oracle_lib, words):
     tot = np.array([sum(int(int(v)).bit_count() for v in row) for row in masks])
     tot[tot == 0] = 1
     w = np.where(tot != n_runs, (D["iw"].astype(np.float64) * (n_runs / tot)).astype(np.float32), D["iw"])
-    oc = o.Catalog(D["x"], D["y"], D["z"], D["nx"], D["ny"], D["nz"], w, D["jk"], iw=D["iw"], masks=masks)
+    dm = None
+    if dither:
+        dbits = rng.random((m, n_runs)) < rng.uniform(0.0, 0.3, m)[:, None]
+        dm = np.packbits(dbits.astype(np.uint8), axis=1, bitorder="little").view(np.uint64).reshape(m, words)
+    oc = o.Catalog(D["x"], D["y"], D["z"], D["nx"], D["ny"], D["nz"], w, D["jk"], iw=D["iw"], masks=masks, dither=dm)
     th = 0.03
     ang = (0.0, float(1.0 - np.cos(th)), 16, 1.6)
     oang = o.axis(*ang)
     up = np.ascontiguousarray(1.0 + 0.1 * rng.standard_normal(16))
     mono = [(0.0, 40.0, 15, 1.3)]
     with L.Context(0) as ctx:
-        dc = ctx.upload(D["x"], D["y"], D["z"], D["nx"], D["ny"], D["nz"], w, D["jk"], iw=D["iw"], masks=masks)
-        jobs = [dict(kind=L.ANGULAR, a=dc, axes=[ang], n_jk=2, weight_mode=L.W_ANG_INVP, n_runs=n_runs),
+        dc = ctx.upload(D["x"], D["y"], D["z"], D["nx"], D["ny"], D["nz"], w, D["jk"], iw=D["iw"], masks=masks,
+                        dither=dm)
+        jobs = [dict(kind=L.ANGULAR, a=dc, axes=[ang], n_jk=2, weight_mode=L.W_ANG_INVP, n_runs=n_runs,
+                     dither=dither),
                 dict(kind=L.MONOPOLE, a=dc, axes=mono, n_jk=2, weight_mode=L.W_INVP_3D, n_runs=n_runs, up_bins=up,
-                     up_axis=ang)]
-        want = [o.count(3, oc, None, oang, None, 2, wopt=o.WOpt(L.W_ANG_INVP, n_runs, None, o.Axis())),
-                o.count(0, oc, None, o.axis(*mono[0]), None, 2, wopt=o.WOpt(L.W_INVP_3D, n_runs, up.ctypes.data, oang))]
+                     up_axis=ang, dither=dither)]
+        d = int(dither)
+        want = [o.count(3, oc, None, oang, None, 2, wopt=o.WOpt(L.W_ANG_INVP, n_runs, None, o.Axis(), 0, d)),
+                o.count(0, oc, None, o.axis(*mono[0]), None, 2,
+                        wopt=o.WOpt(L.W_INVP_3D, n_runs, up.ctypes.data, oang, 0, d))]
         for method in (0, 1):
-            print(f"[gpu] popcount rules words={words} method={method}", flush=True)
+            print(f"[gpu] popcount rules words={words} dither={dither} method={method}", flush=True)
             res, _ = ctx.count(jobs, method=method)
             for r, (full, sub, acc) in zip(res, want):
                 assert r["n_pairs"] == acc
