@@ -229,8 +229,10 @@ def test_autotune_keeps_counts(ctx, oracle_lib):
 @pytest.mark.parametrize("method", [0, 1])
 def test_uneven_multi_catalogue_batch(ctx, oracle_lib, method):
     """Many independent cartesian mocks of uneven size, monopole and s-mu
-    DD/DR/RR of every mock in one call (one persistent launch per kind):
-    every job equals the oracle's count of that mock alone."""
+    DD/DR/RR of every mock in one call (the batched method bins each
+    monopole in the pass of the s-mu job on the same catalogues: one
+    persistent launch): every job equals the oracle's count of that mock
+    alone."""
     from two_pcf_amd import lib as L
     from two_pcf_amd import synth
     o = oracle_lib
@@ -257,7 +259,7 @@ def test_uneven_multi_catalogue_batch(ctx, oracle_lib, method):
     print(f"[gpu] multi-catalogue batch method={method}: {len(jobs)} jobs", flush=True)
     res, t = ctx.count(jobs, method=method)
     if method == 1:
-        assert t["n_launches"] == 2   # one persistent launch per kind
+        assert t["n_launches"] == 1   # each monopole binned in its s-mu job's pass: one persistent launch
     for r, (full, _, acc) in zip(res, want):
         np.testing.assert_array_equal(r["full"], full)
         assert r["n_pairs"] == acc
@@ -345,4 +347,49 @@ def test_long_column_z_window(ctx, oracle_lib, kind):
             np.testing.assert_array_equal(r["full"], full)
             np.testing.assert_array_equal(r["jk"], sub)
     for c in dc.values():
+        c.free()
+
+
+@pytest.mark.parametrize("mono", [(40, 1.3), (25, 1.6), (12, 1.0)], ids=["log40", "log25", "linear12"])
+@pytest.mark.parametrize("name", ["cart_uniform", "eq_clustered", "edge", "cart_midpoint", "eq_midpoint"])
+def test_fused_monopole_s_mu(ctx, oracle_lib, name, mono):
+    """A monopole over the same catalogues, weights and s range as an s-mu job
+    is binned in the s-mu job's pass (one launch for DD, DR and RR of both).
+    Its bins differ from the s-mu's s bins (other count, base, or linear), so
+    each pair is proved in both histograms or rechecked in both; the golden
+    cases bring clustered and equatorial catalogues, edge separations,
+    weights and the pair-midpoint line of sight.  Every histogram equals the
+    oracle's count of that job alone; the plain method takes no fusion."""
+    from two_pcf_amd import lib as L
+    o = oracle_lib
+    case = load_case(name)
+    cats = lib_cats(L, ctx, case.inputs, case.coord)
+    oc = {}
+    for tag, a in case.inputs.items():
+        if case.coord == "cartesian":
+            x, y, z = (np.asarray(a[k], np.float64) for k in "xyz")
+            _, nx, ny, nz = L.cart_norm(x, y, z)
+        else:
+            d = L.comoving_distance(0.25, 1.0, 0.0, 1.0, a["z"])
+            x, y, z, nx, ny, nz = L.eq_to_cart(a["ra"], a["dec"], d)
+        oc[tag] = o.Catalog(x, y, z, nx, ny, nz, a["w"], np.zeros(x.size, np.int32))
+    smu = case.axes("s_mu")
+    mono_ax = [(smu[0][0], smu[0][1], mono[0], mono[1])]
+    jobs, want = [], []
+    for a, b in (("data", None), ("data", "randoms"), ("randoms", None)):
+        for kind, axes in ((2, smu), (0, mono_ax)):
+            jobs.append(dict(kind=kind, a=cats[a], b=cats[b] if b else None, axes=axes, n_jk=1, pi_def=case.pi_def))
+            ax1 = o.axis(*axes[0])
+            ax2 = o.axis(*axes[1]) if len(axes) > 1 else None
+            want.append(o.count(kind, oc[a], oc[b] if b else None, ax1, ax2, 1,
+                                wopt=o.WOpt(0, 0, None, o.Axis(), case.pi_def, 0)))
+    for method in (0, 1):
+        print(f"[gpu] fused monopole {name} {mono} method={method}", flush=True)
+        res, t = ctx.count(jobs, method=method)
+        if method == 1:
+            assert t["n_launches"] == 1
+        for k, (r, (full, _, acc)) in enumerate(zip(res, want)):
+            assert r["n_pairs"] == acc, f"job {k}"
+            np.testing.assert_allclose(r["full"], full, rtol=1e-12, atol=0, err_msg=f"job {k}")
+    for c in cats.values():
         c.free()

@@ -115,6 +115,17 @@ struct JobDev {
 	unsigned long long *full_u, *jk_u;
 	double *full_d, *jk_d;
 	unsigned long long *counters;                    // [0] accepted, [1] rechecked
+	// fp32 path, s-mu jobs that also bin a monopole job of the same
+	// catalogues, weights and s minimum on the same pairs (no regions): its
+	// axis, tables, lane copies and outputs; its estimate terms are stored
+	// with the unused ones zero (inv_step on a log axis, ils and ilb on a
+	// linear one)
+	int m_nb, m_f_shift, m_tq_off, m_t_s2_off, m_tA_off, m_tB_off;
+	float m_s_ilb_ln2, m_s_ils, m_s_inv_step;
+	AxisDev m_ax1;
+	unsigned long long *m_full_u;
+	double *m_full_d;
+	unsigned long long *m_counters;
 };
 
 struct WorkItem {
@@ -533,9 +544,10 @@ __device__ void process_item(const JobDev &J, const WorkItem &W, const int4 *__r
 // the exact pipeline, away from the screening loop.
 // MID: the job's pi is the pair-midpoint definition (sigma-pi and s-mu),
 // chosen per item so that the per-pair code has no branch on it.
-template <int KIND, bool WEIGHTED, bool MID>
+template <int KIND, bool WEIGHTED, bool MID, bool FUSE>
 __device__ void process_item_fast(const JobDev &J, const WorkItem &W, const int4 *__restrict__ rows,
                                   const int *__restrict__ slot_regions, char *smem, const Lds &L) {
+	static_assert(!FUSE || KIND == K_S_MU, "a monopole is fused into s-mu jobs only");
 	using HT = typename HistT<WEIGHTED>::T;
 	constexpr int TILE = TILE_FAST;
 	constexpr int NP = np_of(KIND, WEIGHTED);   // pairs per lane in binning (MID: one)
@@ -604,8 +616,14 @@ __device__ void process_item_fast(const JobDev &J, const WorkItem &W, const int4
 		return code;
 	};
 	// below s2_lo: certainly out of range; at or above s2_hi: certainly out
-	const float s2_lo = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(tab[J.tB_off])));
-	const float s2_hi = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(tab[J.tA_off + n1])));
+	// (of both ranges, with a fused monopole)
+	float s2_lo_v = tab[J.tB_off], s2_hi_v = tab[J.tA_off + n1];
+	if constexpr (FUSE) {
+		s2_lo_v = fminf(s2_lo_v, tab[J.m_tB_off]);
+		s2_hi_v = fmaxf(s2_hi_v, tab[J.m_tA_off + J.m_nb]);
+	}
+	const float s2_lo = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(s2_lo_v)));
+	const float s2_hi = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(s2_hi_v)));
 	const bool check_lo = s2_lo > 0.0f;       // with s_min = 0 every s^2 >= 0 passes
 	const float s2_lo_eff = check_lo ? s2_lo : -INFINITY;
 	const int d1 = J.dim[1], d2 = J.dim[2];
@@ -657,7 +675,7 @@ __device__ void process_item_fast(const JobDev &J, const WorkItem &W, const int4
 
 	const float2 *tq = (const float2 *)(tab + J.tq_off);
 
-	unsigned int n_acc = 0, n_rc = 0;
+	unsigned int n_acc = 0, n_rc = 0, n_acc2 = 0;
 
 	// Region codes: s * nb for LDS slot s, -(region + 1) for a region without
 	// a slot in this item (its sums go straight to global memory).
@@ -706,6 +724,14 @@ __device__ void process_item_fast(const JobDev &J, const WorkItem &W, const int4
 			}
 		}
 	};
+	// the fused monopole's full histogram follows the s-mu one (no regions)
+	char *const F2_lane = (char *)(F + (nb << f_shift) + (lane & ((1 << J.m_f_shift) - 1)));
+	const int f2_bshift = J.m_f_shift + (WEIGHTED ? 3 : 2);
+	auto add2 = [&](int bin, float w) {
+		HT *const fp = (HT *)(F2_lane + (bin << f2_bshift));
+		if (WEIGHTED) atomicAdd((double *)fp, (double)w);
+		else atomicAdd((unsigned int *)fp, 1u);
+	};
 	// wp: the float product of the two staged weights (w, or iw for the
 	// rules built on the input weights); rules that need more read it here
 	auto weight = [&](int l2, int g2, float wp) {
@@ -742,6 +768,13 @@ __device__ void process_item_fast(const JobDev &J, const WorkItem &W, const int4
 					const float w = weight(e.x, g2, cia[e.x].w * (stage_iw ? gld(J.B.iw)[g2] : gld(J.B.w)[g2]));
 					add(bin, w, __float_as_int(cib[e.x].w), sj);
 				}
+				if constexpr (FUSE) {
+					const float *m_t = tab + J.m_t_s2_off;
+					if (exact_bin<K_MONO>(xi, ni, xj, nj, J.min_r_sq, J.max_r_sq, m_t, m_t, m_t, J.m_ax1, J.m_ax1, bin)) {
+						++n_acc2;
+						add2(bin, weight(e.x, g2, cia[e.x].w * (stage_iw ? gld(J.B.iw)[g2] : gld(J.B.w)[g2])));
+					}
+				}
 			}
 		}
 		rq_n = 0;
@@ -762,6 +795,11 @@ __device__ void process_item_fast(const JobDev &J, const WorkItem &W, const int4
 		return (int)__builtin_amdgcn_fmed3f(fmaf(inv, v, ilb * __builtin_amdgcn_logf(fmaf(ils, v, 1.0f))), 0.0f, top);
 	};
 	const float n1_top = (float)(n1 - 1), n2_top = (float)(n2 - 1);
+	// the fused monopole's estimate (its s minimum is the s-mu one) and its
+	// (A_k, B_k+1) pairs
+	const float m_ilb = FUSE ? J.m_s_ilb_ln2 : 0.0f, m_ils = FUSE ? J.m_s_ils : 0.0f;
+	const float m_inv = FUSE ? J.m_s_inv_step : 0.0f, m_top = FUSE ? (float)(J.m_nb - 1) : 0.0f;
+	const float2 *mtq = (const float2 *)(tab + (FUSE ? J.m_tq_off : 0));
 	const float mu_h0 = J.mu_h0, mu_c2 = J.mu_c2;
 	// mu_h0 held in a vector register: the margin's multiply-add then reads
 	// one scalar operand, with no per-pair copy
@@ -792,7 +830,7 @@ __device__ void process_item_fast(const JobDev &J, const WorkItem &W, const int4
 	int t_base = 0;
 	struct PairEval {
 		bool sure, in_range;
-		int bin, l2, q;
+		int bin, bin2, l2, q;   // bin2: the fused monopole's bin
 		float wp;     // product of the staged weights
 		int si, sj;   // region codes of the two objects
 	};
@@ -880,6 +918,15 @@ __device__ void process_item_fast(const JobDev &J, const WorkItem &W, const int4
 				const float fr = __builtin_amdgcn_fractf(xm);
 				sure = sure && (km < n2) && (fabsf(fr - 0.5f) <= hm);
 				bin = __mul24(k, n2) + km;
+				if constexpr (FUSE) {
+					// the monopole bin from the same s: a pair goes to the
+					// histograms only when both bins are proved, else both
+					// decisions are rechecked
+					const int k2 = bin_estimate(sv, m_inv, m_ilb, m_ils, m_top);
+					const float2 qm = mtq[k2];
+					sure = sure && s2 >= qm.x && s2 < qm.y;
+					r.bin2 = k2;
+				}
 			} else {
 				bin = k;
 			}
@@ -916,7 +963,12 @@ __device__ void process_item_fast(const JobDev &J, const WorkItem &W, const int4
 				const PairEval p = eval(mid, l, q, cia[l], cib[l], tfa[q], tfb[q]);
 				if (v && p.sure && p.in_range) {
 					++n_acc;
-					add(p.bin, weight(p.l2, t_base + p.q, p.wp), p.si, p.sj);
+					const float w = weight(p.l2, t_base + p.q, p.wp);
+					add(p.bin, w, p.si, p.sj);
+					if constexpr (FUSE) {
+						++n_acc2;
+						add2(p.bin2, w);
+					}
 				}
 				push(v && !p.sure, p);
 				__builtin_amdgcn_wave_barrier();
@@ -949,7 +1001,12 @@ __device__ void process_item_fast(const JobDev &J, const WorkItem &W, const int4
 			for (int k = 0; k < NP; ++k)
 				if (lane + k * WAVE < (int)n && p[k].sure && p[k].in_range) {
 					++n_acc;
-					add(p[k].bin, weight(p[k].l2, t_base + p[k].q, p[k].wp), p[k].si, p[k].sj);
+					const float w = weight(p[k].l2, t_base + p[k].q, p[k].wp);
+					add(p[k].bin, w, p[k].si, p[k].sj);
+					if constexpr (FUSE) {
+						++n_acc2;
+						add2(p[k].bin2, w);
+					}
 				}
 #pragma unroll
 			for (int k = 0; k < NP; ++k) push(lane + k * WAVE < (int)n && !p[k].sure, p[k]);
@@ -1131,13 +1188,31 @@ __device__ void process_item_fast(const JobDev &J, const WorkItem &W, const int4
 			}
 		}
 	}
+	if constexpr (FUSE) {
+		HT *F2 = F + (nb << f_shift);
+		const int mc = 1 << J.m_f_shift;
+		for (int b = tid; b < J.m_nb; b += BLOCK) {
+			HT v = (HT)0;
+			for (int r = 0; r < mc; ++r) {
+				v += F2[b * mc + r];
+				F2[b * mc + r] = (HT)0;
+			}
+			if (v != (HT)0) {
+				if (WEIGHTED) gadd(&J.m_full_d[b], (double)v);
+				else gadd(&J.m_full_u[b], (unsigned long long)v);
+			}
+		}
+	}
 	for (int off = 32; off; off >>= 1) {
 		n_acc += __shfl_xor(n_acc, off);
 		n_rc += __shfl_xor(n_rc, off);
+		if (FUSE) n_acc2 += __shfl_xor(n_acc2, off);
 	}
 	if (lane == 0) {
 		if (n_acc) gadd(&J.counters[0], (unsigned long long)n_acc);
 		if (n_rc) gadd(&J.counters[1], (unsigned long long)n_rc);
+		if (FUSE && n_acc2) gadd(&J.m_counters[0], (unsigned long long)n_acc2);
+		if (FUSE && n_rc) gadd(&J.m_counters[1], (unsigned long long)n_rc);
 	}
 	__syncthreads();
 }
@@ -1145,7 +1220,7 @@ __device__ void process_item_fast(const JobDev &J, const WorkItem &W, const int4
 // One launch: item = blockIdx.x (plain) or pulled from a global queue
 // (persistent, queue != nullptr).  Every block leaves when the queue is
 // drained, so the grid always finishes.
-template <int KIND, bool FAST, bool WEIGHTED, bool MID = false>
+template <int KIND, bool FAST, bool WEIGHTED, bool MID = false, bool FUSE = false>
 __global__ void __launch_bounds__(BLOCK) k_count(const JobDev *__restrict__ jobs, const WorkItem *__restrict__ items,
                                                  int n_items, unsigned int *queue, const int4 *__restrict__ rows,
                                                  const int *__restrict__ slot_regions, const float *__restrict__ tables,
@@ -1171,7 +1246,7 @@ __global__ void __launch_bounds__(BLOCK) k_count(const JobDev *__restrict__ jobs
 		const WorkItem W = items[it];
 		if constexpr (FAST) {
 			// MID: every job of this launch has the pair-midpoint pi
-			process_item_fast<KIND, WEIGHTED, MID>(jobs[W.job], W, rows, slot_regions, smem, L);
+			process_item_fast<KIND, WEIGHTED, MID, FUSE>(jobs[W.job], W, rows, slot_regions, smem, L);
 		} else {
 			process_item<KIND, WEIGHTED>(jobs[W.job], W, rows, slot_regions, smem, L);
 		}
@@ -1810,6 +1885,33 @@ int build_plan(twopcf_ctx *ctx, const twopcf_job *jobs, int n_jobs, int method, 
 		if ((long long)lds_fixed + (hb << fs) + hb + bytes + 16 <= (long long)ctx->lds_max)
 			stage_bytes[H.j->kind] = std::max(stage_bytes[H.j->kind], bytes);
 	}
+	// A monopole job over the same catalogues, weights and s minimum as an
+	// s-mu job, without regions, is binned in the s-mu job's pass (fp32
+	// path): the pairs are screened and compacted once.  fused[s] = m for the
+	// s-mu job s, absorbed[m] = true for the monopole job m, which gets no
+	// work items of its own.
+	std::vector<int> fused(n_jobs, -1);
+	std::vector<char> absorbed(n_jobs, 0);
+	auto copies_of = [](int nb, bool weighted) { return (size_t)nb * WAVE * (weighted ? 8 : 4) <= 32768 ? 6 : 0; };
+	for (int s_ = 0; s_ < n_jobs && fast; ++s_) {
+		const HostJob &S = hj[s_];
+		if (S.j->kind != TWOPCF_S_MU || S.j->n_jk > 1 || S.j->weight_mode != TWOPCF_W_PLAIN) continue;
+		for (int m = 0; m < n_jobs; ++m) {
+			const HostJob &M = hj[m];
+			if (absorbed[m] || M.j->kind != TWOPCF_MONOPOLE || M.j->n_jk > 1 || M.j->weight_mode != TWOPCF_W_PLAIN)
+				continue;
+			if (M.a != S.a || M.b != S.b || M.weighted != S.weighted || M.rmax != S.rmax) continue;
+			if (M.ax1.min != S.ax1.min) continue;
+			const int hs = S.nb * (S.weighted ? 8 : 4), hm = M.nb * (M.weighted ? 8 : 4);
+			if ((long long)lds_fixed + ((long long)hs << copies_of(S.nb, S.weighted)) +
+			        ((long long)hm << copies_of(M.nb, M.weighted)) + 16 >
+			    (long long)ctx->lds_max)
+				continue;
+			fused[s_] = m;
+			absorbed[m] = 1;
+			break;
+		}
+	}
 	for (int q = 0; q < n_jobs; ++q) {
 		HostJob &H = hj[q];
 		Prepared &P = prep[q];
@@ -2045,8 +2147,15 @@ int build_plan(twopcf_ctx *ctx, const twopcf_job *jobs, int n_jobs, int method, 
 		// work items: chunks of <= 64 first-side objects per cell, rows
 		// grouped so that each item sees at most slot_cap regions; the slot
 		// cap also keeps (1 + slots) histograms inside the LDS budget
+		if (absorbed[q]) continue;   // binned in its s-mu job's pass
+		if (fused[q] >= 0) {
+			const HostJob &M = hj[fused[q]];
+			D.m_nb = M.nb;
+			D.m_f_shift = copies_of(M.nb, M.weighted);
+			P.f_elems += M.nb << D.m_f_shift;
+		}
 		const int hist_bytes = H.nb * (H.weighted ? 8 : 4);
-		const int f_bytes = hist_bytes << D.f_shift;
+		const int f_bytes = (P.f_elems) * (H.weighted ? 8 : 4);
 		const int slot_cap = std::max(
 		    1, std::min(max_slots,
 		                (int)((lds_budget - lds_fixed - f_bytes - (P.mask_bytes ? P.mask_bytes + 16 : 0)) /
@@ -2180,6 +2289,24 @@ int build_plan(twopcf_ctx *ctx, const twopcf_job *jobs, int n_jobs, int method, 
 		}
 		P.jd.counters = d_cnt + 2 * q;
 	}
+	for (int q = 0; q < n_jobs; ++q) {
+		if (fused[q] < 0) continue;
+		JobDev &D = prep[q].jd;
+		const JobDev &M = prep[fused[q]].jd;
+		D.m_tq_off = M.tq_off;
+		D.m_t_s2_off = M.t_s2_off;
+		D.m_tA_off = M.tA_off;
+		D.m_tB_off = M.tB_off;
+		// the estimate's unused terms zero: inv_step on a log axis, ils and
+		// ilb on a linear one
+		D.m_s_ilb_ln2 = M.s_log ? M.s_ilb_ln2 : 0.0f;
+		D.m_s_ils = M.s_log ? M.s_ils : 0.0f;
+		D.m_s_inv_step = M.s_log ? 0.0f : M.s_inv_step;
+		D.m_ax1 = M.ax1;
+		D.m_full_u = M.full_u;
+		D.m_full_d = M.full_d;
+		D.m_counters = M.counters;
+	}
 	std::vector<JobDev> jd_h(n_jobs);
 	for (int q = 0; q < n_jobs; ++q) jd_h[q] = prep[q].jd;
 	// cost-sorted queue for the batched method
@@ -2248,7 +2375,7 @@ int build_plan(twopcf_ctx *ctx, const twopcf_job *jobs, int n_jobs, int method, 
 		}
 		return L;
 	};
-	auto launch = [&](int kind, bool weighted, bool mid, const WorkItem *it, int n_it, int nb, int slots,
+	auto launch = [&](int kind, bool weighted, bool mid, bool fz, const WorkItem *it, int n_it, int nb, int slots,
 	                  unsigned *queue, int grid, int f_elems, int mask_bytes) -> int {
 		const bool fast_kernel = fast;
 		Lds L = layout(fast_kernel, weighted, nb, slots, f_elems, mask_bytes);
@@ -2260,6 +2387,11 @@ int build_plan(twopcf_ctx *ctx, const twopcf_job *jobs, int n_jobs, int method, 
 #define PICK_PI(K, W) fn = mid ? k_count<K, true, W, true> : k_count<K, true, W, false>
 		if (fast) {
 			if (kind == K_MONO) { if (weighted) PICK(K_MONO, true, true); else PICK(K_MONO, true, false); }
+			else if (kind == K_S_MU && fz) {
+				// s-mu jobs with a fused monopole
+				if (weighted) fn = mid ? k_count<K_S_MU, true, true, true, true> : k_count<K_S_MU, true, true, false, true>;
+				else fn = mid ? k_count<K_S_MU, true, false, true, true> : k_count<K_S_MU, true, false, false, true>;
+			}
 			else if (kind == K_S_MU) { if (weighted) PICK_PI(K_S_MU, true); else PICK_PI(K_S_MU, false); }
 			else if (kind == K_SIGMA_PI) { if (weighted) PICK_PI(K_SIGMA_PI, true); else PICK_PI(K_SIGMA_PI, false); }
 			else { if (weighted) PICK(K_ANGULAR, true, true); else PICK(K_ANGULAR, true, false); }
@@ -2284,7 +2416,7 @@ int build_plan(twopcf_ctx *ctx, const twopcf_job *jobs, int n_jobs, int method, 
 			while (k1 < items.size() && items[k1].job == q) ++k1;
 			int slots = 0;
 			for (size_t k = k0; k < k1; ++k) slots = std::max(slots, items[k].n_slots);
-			if ((rc = launch(hj[q].j->kind, prep[q].weighted, false, d_items + k0, (int)(k1 - k0), prep[q].nb, slots,
+			if ((rc = launch(hj[q].j->kind, prep[q].weighted, false, false, d_items + k0, (int)(k1 - k0), prep[q].nb, slots,
 			                 nullptr, (int)(k1 - k0), prep[q].f_elems, 0))) {
 				release();
 				return rc;
@@ -2292,14 +2424,16 @@ int build_plan(twopcf_ctx *ctx, const twopcf_job *jobs, int n_jobs, int method, 
 			k0 = k1;
 		}
 	} else {
-		// batched: one persistent launch per (kind, weighted, pi definition)
-		// class over the cost-sorted queue of all jobs in that class
+		// batched: one persistent launch per (kind, weighted, pi definition,
+		// fused monopole) class over the cost-sorted queue of all jobs in
+		// that class
 		auto mid_of = [&](int q) {
 			const int k = hj[q].j->kind;
 			return (k == TWOPCF_SIGMA_PI || k == TWOPCF_S_MU) && hj[q].j->pi_def == TWOPCF_PI_MIDPOINT ? 1 : 0;
 		};
 		std::map<std::tuple<int, int, int>, std::vector<WorkItem>> cls;
-		for (auto &W : items) cls[{hj[W.job].j->kind, prep[W.job].weighted ? 1 : 0, mid_of(W.job)}].push_back(W);
+		for (auto &W : items)
+			cls[{hj[W.job].j->kind, prep[W.job].weighted ? 1 : 0, mid_of(W.job) + (fused[W.job] >= 0 ? 2 : 0)}].push_back(W);
 		size_t off = 0;
 		std::vector<WorkItem> flat;
 		std::vector<std::tuple<int, int, size_t, size_t, int>> spans;
@@ -2316,10 +2450,12 @@ int build_plan(twopcf_ctx *ctx, const twopcf_job *jobs, int n_jobs, int method, 
 		for (auto &sp : spans) {
 			const int kind = std::get<0>(sp);
 			const bool weighted = std::get<1>(sp) != 0;
-			const int mid = std::get<4>(sp);
+			const int mid = std::get<4>(sp) & 1;
+			const bool fz = (std::get<4>(sp) & 2) != 0;   // s-mu jobs with a fused monopole
 			int nb = 1, f_elems = 1, mask_bytes = 0;
 			for (int q = 0; q < n_jobs; ++q)
-				if (hj[q].j->kind == kind && prep[q].weighted == weighted && mid_of(q) == mid) {
+				if (!absorbed[q] && hj[q].j->kind == kind && prep[q].weighted == weighted && mid_of(q) == mid &&
+				    (fused[q] >= 0) == fz) {
 					nb = std::max(nb, hj[q].nb);
 					f_elems = std::max(f_elems, prep[q].f_elems);
 					mask_bytes = std::max(mask_bytes, prep[q].mask_bytes);
@@ -2333,7 +2469,7 @@ int build_plan(twopcf_ctx *ctx, const twopcf_job *jobs, int n_jobs, int method, 
 			Lds L = layout(true, weighted, nb, slots, f_elems, mask_bytes);
 			const int per_cu = std::max(1, std::min(8, (int)(ctx->lds_max / std::max(L.total, 1))));
 			const int grid = std::min<long long>((long long)ctx->n_cu * per_cu, (long long)std::get<3>(sp));
-			if ((rc = launch(kind, weighted, mid != 0, d_items + std::get<2>(sp), (int)std::get<3>(sp), nb, slots,
+			if ((rc = launch(kind, weighted, mid != 0, fz, d_items + std::get<2>(sp), (int)std::get<3>(sp), nb, slots,
 			                 d_queue + qi, std::max(grid, 1), f_elems, mask_bytes))) {
 				release();
 				return rc;
