@@ -157,8 +157,9 @@ def workload_multi(args, L, ctx):
     """64 independent cartesian mocks, sizes log-uniform in 20k-2M (scaled by
     --scale), Neyman-Scott at density 3e-4 (Mpc/h)^-3, 5x uniform randoms
     each; monopole (30 log bins base 1.3 to 100) and s-mu (40 x 50) DD, DR,
-    RR of every mock in one call: one persistent launch per correlator kind,
-    captured in one hipGraph."""
+    RR of every mock in one call: each monopole is binned in the pass of the
+    s-mu job on the same catalogues, one persistent launch captured in a
+    hipGraph."""
     from two_pcf_amd import synth
     t0 = time.time()
     mocks = synth.uneven_mocks(n_mocks=64, n_min=max(50, int(20_000 * args.scale)),
