@@ -58,11 +58,15 @@ enum {
 	 * per pair the reference's own mixed FP64/FP32 arithmetic
 	 * (correlators.cpp:309-343) evaluated exactly. */
 	TWOPCF_PLAIN_FP64 = 0,
-	/* Batched: every job of the call in one persistent launch over a
-	 * cost-sorted cell-pair work queue; cells subdivided; pair separations in
-	 * FP32 from cell-local offsets; pairs within the proven FP32 error band
-	 * of a range limit or bin edge are re-evaluated with the exact pipeline,
-	 * so counts are identical to TWOPCF_PLAIN_FP64. */
+	/* Batched: the jobs of the call in persistent launches (one per
+	 * correlator kind, weighting and line of sight) over a cost-sorted
+	 * cell-pair work queue; cells subdivided; pair separations in FP32 from
+	 * cell-local offsets; pairs within the proven FP32 error band of a range
+	 * limit or bin edge are re-evaluated with the exact pipeline, so counts
+	 * are identical to TWOPCF_PLAIN_FP64.  A monopole job over the same
+	 * catalogues, weights and s minimum as an s-mu job of the call, neither
+	 * with regions, is binned in the s-mu job's pass; its outputs are its
+	 * own. */
 	TWOPCF_FP32_RECHECK = 1
 };
 
