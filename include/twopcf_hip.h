@@ -147,7 +147,7 @@ typedef struct {
 } twopcf_timing;
 
 /* Tuning knobs of TWOPCF_FP32_RECHECK.  A zero field takes the default
- * (2, 8, 256).  A NULL tuning takes the choice twopcf_autotune made for the
+ * (2, 12, 256): as many of the 12 region slots as fit in LDS.  A NULL tuning takes the choice twopcf_autotune made for the
  * same job set, or the defaults.  No knob changes a result. */
 typedef struct {
 	int32_t cell_subdiv;     /* cells per rmax along each axis (1..3; larger values mean 3) */

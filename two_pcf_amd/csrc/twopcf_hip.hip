@@ -1694,7 +1694,8 @@ int build_plan(twopcf_ctx *ctx, const twopcf_job *jobs, int n_jobs, int method, 
                Plan &plan, int sample_stride = 1) {
 	const bool fast = method == TWOPCF_FP32_RECHECK;
 	int subdiv = fast ? 2 : 1;
-	int max_slots = 8;
+	// region slots: 12 requested on the batched path (as many as fit in LDS)
+	int max_slots = fast ? 12 : 8;
 	int i_chunk = ICHUNK;
 	if (tuning) {
 		if (fast && tuning->cell_subdiv > 0) subdiv = std::min(tuning->cell_subdiv, MAX_SUBDIV);
@@ -2667,7 +2668,7 @@ int twopcf_autotune(twopcf_ctx *ctx, const twopcf_job *jobs, int n_jobs, twopcf_
 		*ms = m;
 		return 0;
 	};
-	twopcf_tuning cur{2, 8, ICHUNK};
+	twopcf_tuning cur{2, 12, ICHUNK};
 	double cur_ms = INFINITY;
 	int rc = trial(cur, &cur_ms);
 	if (rc) return rc;
@@ -2690,7 +2691,7 @@ int twopcf_autotune(twopcf_ctx *ctx, const twopcf_job *jobs, int n_jobs, twopcf_
 	};
 	ctx->err.clear();
 	if ((rc = try_knob(&twopcf_tuning::cell_subdiv, {1, 3}))) return rc;
-	if (any_jk && (rc = try_knob(&twopcf_tuning::max_slots, {4, 12, 16}))) return rc;
+	if (any_jk && (rc = try_knob(&twopcf_tuning::max_slots, {4, 8, 16}))) return rc;
 	if ((rc = try_knob(&twopcf_tuning::i_chunk, {128}))) return rc;
 	ctx->tuned[job_set_key(jobs, n_jobs, method, nullptr)] = cur;
 	if (best) *best = cur;
