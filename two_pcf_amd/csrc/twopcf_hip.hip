@@ -1866,10 +1866,10 @@ int build_plan(twopcf_ctx *ctx, const twopcf_job *jobs, int n_jobs, int method, 
 	// The popcount rules of weighted angular and monopole jobs (small
 	// histograms) read the first side's masks from an LDS copy when it fits
 	// beside one region slot: rows of an odd number of 16-byte units, so
-	// that lanes reading different objects spread over the banks.  Every
-	// weighted job of such a kind shares its launch, so all of them keep at
-	// most 16 lane copies of the full histogram and leave room for the
-	// largest copy of their kind (stage_bytes).
+	// that lanes reading different objects spread over the banks.  These
+	// jobs form their own launch class; they keep at most 16 lane copies of
+	// the full histogram and leave room for the largest copy of their kind
+	// (stage_bytes).
 	auto mask_row = [](int ms) { return ((ms >> 1) & 1) ? ms : ms + 2; };
 	auto wants_copy = [&](const HostJob &H) {
 		const int wm = H.j->weight_mode, ms = H.a->m_stride;
@@ -2099,11 +2099,12 @@ int build_plan(twopcf_ctx *ctx, const twopcf_job *jobs, int n_jobs, int method, 
 		// small histograms of the fp32 path get one copy per lane
 		D.f_shift = (fast && (size_t)H.nb * WAVE * (H.weighted ? 8 : 4) <= 32768)
 		                ? 6 : 0;
-		// first-side mask copy in LDS (see stage_bytes)
-		if (fast && H.weighted && stage_bytes[H.j->kind] > 0) {
+		// first-side mask copy in LDS (see stage_bytes); the jobs that take
+		// it form their own launch class
+		if (wants_copy(H) && stage_bytes[H.j->kind] > 0) {
 			D.f_shift = std::min(D.f_shift, 4);
 			P.mask_bytes = stage_bytes[H.j->kind];
-			if (wants_copy(H)) D.mstage = mask_row(H.a->m_stride);
+			D.mstage = mask_row(H.a->m_stride);
 		}
 		P.f_elems = H.nb << D.f_shift;
 		D.pi_def = H.j->pi_def;
@@ -2434,7 +2435,9 @@ int build_plan(twopcf_ctx *ctx, const twopcf_job *jobs, int n_jobs, int method, 
 		};
 		std::map<std::tuple<int, int, int>, std::vector<WorkItem>> cls;
 		for (auto &W : items)
-			cls[{hj[W.job].j->kind, prep[W.job].weighted ? 1 : 0, mid_of(W.job) + (fused[W.job] >= 0 ? 2 : 0)}].push_back(W);
+			cls[{hj[W.job].j->kind, prep[W.job].weighted ? 1 : 0,
+			     mid_of(W.job) + (fused[W.job] >= 0 ? 2 : 0) + (prep[W.job].jd.mstage > 0 ? 4 : 0)}]
+			    .push_back(W);
 		size_t off = 0;
 		std::vector<WorkItem> flat;
 		std::vector<std::tuple<int, int, size_t, size_t, int>> spans;
@@ -2453,10 +2456,11 @@ int build_plan(twopcf_ctx *ctx, const twopcf_job *jobs, int n_jobs, int method, 
 			const bool weighted = std::get<1>(sp) != 0;
 			const int mid = std::get<4>(sp) & 1;
 			const bool fz = (std::get<4>(sp) & 2) != 0;   // s-mu jobs with a fused monopole
+			const bool st = (std::get<4>(sp) & 4) != 0;   // jobs with the LDS mask copy
 			int nb = 1, f_elems = 1, mask_bytes = 0;
 			for (int q = 0; q < n_jobs; ++q)
 				if (!absorbed[q] && hj[q].j->kind == kind && prep[q].weighted == weighted && mid_of(q) == mid &&
-				    (fused[q] >= 0) == fz) {
+				    (fused[q] >= 0) == fz && (prep[q].jd.mstage > 0) == st) {
 					nb = std::max(nb, hj[q].nb);
 					f_elems = std::max(f_elems, prep[q].f_elems);
 					mask_bytes = std::max(mask_bytes, prep[q].mask_bytes);
