@@ -2427,8 +2427,8 @@ int build_plan(twopcf_ctx *ctx, const twopcf_job *jobs, int n_jobs, int method, 
 		}
 	} else {
 		// batched: one persistent launch per (kind, weighted, pi definition,
-		// fused monopole) class over the cost-sorted queue of all jobs in
-		// that class
+		// fused monopole, LDS mask copy) class over the cost-sorted queue of
+		// all jobs in that class
 		auto mid_of = [&](int q) {
 			const int k = hj[q].j->kind;
 			return (k == TWOPCF_SIGMA_PI || k == TWOPCF_S_MU) && hj[q].j->pi_def == TWOPCF_PI_MIDPOINT ? 1 : 0;
