@@ -2169,15 +2169,18 @@ int build_plan(twopcf_ctx *ctx, const twopcf_job *jobs, int n_jobs, int method, 
 		const bool jk = H.j->n_jk > 1;
 		std::vector<int> rws;
 		for (int r = 0; r < D.n_rows; ++r) rws.push_back(r);
+		// scratch reused across cells and chunks (no allocation per row)
+		std::vector<int> regs_i, regs, add;
+		struct RowInfo {
+			int r, jb, je;
+		};
+		std::vector<RowInfo> ri;
 		for (int c = 0; c < n_cell; ++c) {
 			const int ib = A->start_h[c], ie = A->start_h[c + 1];
 			if (ib == ie) continue;
 			const int cx = c / (d1 * d2), cy = (c / d2) % d1, cz = c % d2;
 			// per row: j range, cost
-			struct RowInfo {
-				int r, jb, je;
-			};
-			std::vector<RowInfo> ri;
+			ri.clear();
 			for (int r : rws) {
 				const int4 row = row_pool[D.row_off + r];
 				const int qx = cx + row.x, qy = cy + row.y;
@@ -2195,24 +2198,21 @@ int build_plan(twopcf_ctx *ctx, const twopcf_job *jobs, int n_jobs, int method, 
 			for (int i0 = ib; i0 < ie; i0 += i_chunk) {
 				const int ni = std::min(i_chunk, ie - i0);
 				// group rows by region budget
-				std::vector<int> regs_i;
+				regs_i.clear();
 				if (jk) {
-					// regions of this chunk: those present in its cell
-					const int ro = A->reg_off[c], re = A->reg_off[c + 1];
-					(void)re;
-					std::vector<int> tmp;
-					for (int k = ro; k < re; ++k) tmp.push_back(A->reg_ids[k]);
-					regs_i = tmp;   // superset of the chunk's regions
+					// regions of this chunk: those present in its cell (a
+					// superset of the chunk's regions)
+					for (int k = A->reg_off[c]; k < A->reg_off[c + 1]; ++k) regs_i.push_back(A->reg_ids[k]);
 				}
 				size_t g0 = 0;
 				while (g0 < ri.size()) {
-					std::vector<int> regs = regs_i;
+					regs.assign(regs_i.begin(), regs_i.end());
 					unsigned long long mask = 0;
 					double cost = 0;
 					size_t g1 = g0;
 					for (; g1 < ri.size(); ++g1) {
 						if (jk) {
-							std::vector<int> add;
+							add.clear();
 							const int4 row = row_pool[D.row_off + ri[g1].r];
 							const int qx = cx + row.x, qy = cy + row.y;
 							const int base = (qx * d1 + qy) * d2;
