@@ -286,3 +286,25 @@ def test_layout_seen_by_h5dump(written):
     assert ang.returncode == 0, ang.stderr
     assert ang.stdout.count("H5T_IEEE_F32LE") == 3
     assert 'ATTRIBUTE "n_bins"' in ang.stdout
+
+
+def test_hdf5_library_missing_is_reported(harnesses, tmp_path):
+    """The hdf5 file type with a library that is not libhdf5 (TWOPCF_HDF5_LIB
+    names libm): the reader stops with a message naming what is missing
+    instead of calling into it; ascii catalogues never open the library."""
+    reader, _ = harnesses
+    cols, text = _catalogue(10, 9, False)
+    h5mini.write(str(tmp_path / "cat.h5"), cols)
+    (tmp_path / "cat.txt").write_text(text)
+    p = dict(BASE, coord_system="cartesian", use_weights=1, n_jk_regions=5)
+    env = dict(os.environ, TWOPCF_HDF5_LIB="libm.so.6")
+    for ft, fn, ok in (("hdf5", "cat.h5", False), ("ascii", "cat.txt", True)):
+        pf = tmp_path / f"p_{ft}.txt"
+        pf.write_text("".join(f"{k} = {v}\n" for k, v in dict(p, data_filename=tmp_path / fn,
+                                                                 data_file_type=ft).items()))
+        r = subprocess.run([reader, str(pf), "0"], capture_output=True, text=True, timeout=60, env=env)
+        if ok:
+            assert r.returncode == 0, r.stderr
+        else:
+            assert r.returncode != 0
+            assert "needs the HDF5 C library: symbol H5open missing" in r.stderr
