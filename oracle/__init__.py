@@ -50,7 +50,7 @@ def build():
     present in this container, oracle/_ref from them."""
     subprocess.run(["make", "-s", "-C", HERE, "oracle"], check=True)
     if os.path.isdir("/root/reference/src"):
-        subprocess.run(["make", "-s", "-C", HERE, "ref"], check=True)
+        subprocess.run(["make", "-s", "-j8", "-C", HERE, "ref"], check=True)
 
 
 _lib = None
