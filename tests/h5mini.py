@@ -198,3 +198,22 @@ class File:
         L.H5Aclose(hid_t(a))
         L.H5Gclose(hid_t(g))
         return size
+
+
+def tree(path):
+    """Everything an hdf5 file of the two_pcf writers holds, for comparing two
+    files: {member of the root: {"datasets": {name: (values as float64,
+    shape, stored element size)}, "attrs": {name: (stored bytes, size)}}},
+    members and names in name order; a root member that is not a group is
+    listed as a dataset of the root ("/")."""
+    out = {}
+    with File(path) as f:
+        for m in f.members("/"):
+            try:
+                names = f.members(m)
+            except KeyError:
+                out.setdefault("/", {"datasets": {}, "attrs": {}})["datasets"][m] = f.dataset("/", m)
+                continue
+            out[m] = {"datasets": {n: f.dataset(m, n) for n in names},
+                      "attrs": {a: f.attr(m, a, "str") for a in f.attr_names(m)}}
+    return out
