@@ -32,7 +32,7 @@
 extern "C" {
 #endif
 
-#define TWOPCF_ABI_VERSION 3
+#define TWOPCF_ABI_VERSION 4
 
 enum {
 	TWOPCF_OK = 0,
@@ -169,6 +169,24 @@ int64_t twopcf_catalog_size(const twopcf_cat *cat);
  * tuning may be NULL.  timing may be NULL. */
 int twopcf_count(twopcf_ctx *ctx, const twopcf_job *jobs, int n_jobs, int method,
                  const twopcf_tuning *tuning, twopcf_timing *timing);
+
+/* twopcf_count without device allocation.  No counterpart in the reference:
+ * it is for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  Same jobs,
+ * methods, tuning and outputs as twopcf_count, but it calls no hipMalloc,
+ * hipFree or allocating library routine: every per-call device buffer (job
+ * table, work items, stencil rows, region slots, tables, up-weight tables,
+ * output pools, counters) is carved from a library-owned device array of
+ * TWOPCF_NOALLOC_SCRATCH_MIB MiB, one per GPU, held until the process ends.
+ * A job set that needs more returns TWOPCF_E_LIMIT.  The catalogues must
+ * already be gridded for the job set's cell geometry: call twopcf_count once
+ * with the same jobs and tuning first (otherwise TWOPCF_E_ARG).  No graph is
+ * captured or replayed.  The outputs are cleared on the device at every
+ * call; calls that overlap on one GPU (from any context or thread) share the
+ * array and race. */
+#define TWOPCF_NOALLOC_SCRATCH_MIB 64
+int twopcf_count_noalloc(twopcf_ctx *ctx, const twopcf_job *jobs, int n_jobs, int method,
+                         const twopcf_tuning *tuning, twopcf_timing *timing);
 
 /* Autotuner of the batched method (no counterpart in the reference, whose
  * cell size is fixed at rmax): times trial runs of the job set on a sample of

@@ -33,13 +33,14 @@ def test_exports_every_declared_symbol(hip):
     for n in names:
         assert hasattr(L, n), n
     assert sorted(hip.EXPORTS) == names
-    assert L.twopcf_abi_version() == hip.ABI_VERSION == 3
+    assert L.twopcf_abi_version() == hip.ABI_VERSION == 4
 
 
 def test_null_arguments_are_rejected(hip):
     L = hip.lib()
     assert L.twopcf_ctx_create(0, None) == -1
     assert L.twopcf_count(None, None, 0, 0, None, None) == -1
+    assert L.twopcf_count_noalloc(None, None, 0, 0, None, None) == -1
     assert L.twopcf_catalog_free(None) == -1
     assert L.twopcf_s2_thresholds(0, None, None, None) == -1
 

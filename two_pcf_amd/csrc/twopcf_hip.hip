@@ -25,6 +25,7 @@
 #include <initializer_list>
 #include <map>
 #include <memory>
+#include <mutex>
 #include <string>
 #include <tuple>
 #include <type_traits>
@@ -133,6 +134,48 @@ struct WorkItem {
 	int i_begin, i_count, i_cell;
 	unsigned long long row_mask;
 	int slot_base, n_slots;
+};
+
+}  // namespace
+
+// Device memory of twopcf_count_noalloc: the per-call buffers (job table,
+// work items, stencil rows, region slots, float tables, up-weight tables,
+// output pools, counters, queue counters) are carved from this array instead
+// of hipMalloc.  Each GPU the library runs on holds it until the process
+// ends.  Host code never passes the array's name to a kernel or a copy: it
+// uses the address hipGetSymbolAddress gives on each device
+// (noalloc_scratch_ptr), resolved once per device in twopcf_ctx_create.
+constexpr size_t NOALLOC_SCRATCH_BYTES = size_t(64) << 20;
+static __device__ __align__(256) unsigned char scratch[NOALLOC_SCRATCH_BYTES];
+
+namespace {
+
+constexpr int NOALLOC_MAX_DEVICES = 64;
+void *g_scratch_ptr[NOALLOC_MAX_DEVICES];   // per device; null until resolved
+std::mutex g_scratch_mu;
+
+// Resolve the scratch array's address on the current device (once per
+// device).  Returns 0, or a HIP error from hipGetSymbolAddress.
+hipError_t resolve_scratch(int device) {
+	if (device < 0 || device >= NOALLOC_MAX_DEVICES) return hipSuccess;   // twopcf_count_noalloc then refuses
+	std::lock_guard<std::mutex> lock(g_scratch_mu);
+	if (g_scratch_ptr[device]) return hipSuccess;
+	void *p = nullptr;
+	hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+	if (e == hipSuccess) g_scratch_ptr[device] = p;
+	return e;
+}
+
+void *noalloc_scratch_ptr(int device) {
+	if (device < 0 || device >= NOALLOC_MAX_DEVICES) return nullptr;
+	std::lock_guard<std::mutex> lock(g_scratch_mu);
+	return g_scratch_ptr[device];
+}
+
+// Bump allocation of a plan's device buffers from the scratch array.
+struct Arena {
+	char *base;
+	size_t cap, used;
 };
 
 // -------------------------------------------------------------- kernels --
@@ -1391,13 +1434,18 @@ int dalloc(twopcf_ctx *ctx, T **p, size_t n) {
 	return 0;
 }
 
-// Sort one catalogue into the cell geometry g (cached per geometry).
-int grid_catalog(twopcf_ctx *ctx, twopcf_cat *c, const GridKey &g, SortedCat **out) {
+// Sort one catalogue into the cell geometry g (cached per geometry).  With
+// may_alloc false only a cached grid is taken (twopcf_count_noalloc).
+int grid_catalog(twopcf_ctx *ctx, twopcf_cat *c, const GridKey &g, SortedCat **out, bool may_alloc = true) {
 	auto it = c->grids.find(g);
 	if (it != c->grids.end()) {
 		*out = &it->second;
 		return 0;
 	}
+	if (!may_alloc)
+		return fail(ctx, TWOPCF_E_ARG,
+		            "catalogue not yet gridded for this job set's cell geometry (twopcf_count_noalloc allocates "
+		            "nothing): call twopcf_count once with the same jobs and tuning first");
 	SortedCat s;
 	const int64_t n = c->n;
 	const int n_cell = g.dim[0] * g.dim[1] * g.dim[2];
@@ -1551,6 +1599,14 @@ int twopcf_ctx_create(int device, twopcf_ctx **out) {
 	ctx->n_cu = prop.multiProcessorCount;
 	ctx->lds_max = std::max(prop.sharedMemPerBlock, prop.sharedMemPerBlockOptin);
 	ctx->name = prop.gcnArchName;
+	// the no-allocation entry point's scratch on this device (device current)
+	if (resolve_scratch(device) != hipSuccess) {
+		(void)hipEventDestroy(ctx->ev0);
+		(void)hipEventDestroy(ctx->ev1);
+		(void)hipStreamDestroy(ctx->stream);
+		delete ctx;
+		return TWOPCF_E_HIP;
+	}
 	*out = ctx;
 	return 0;
 }
@@ -1690,9 +1746,29 @@ namespace {
 // items, output pools and the launch list.  Cached per context.
 // sample_stride > 1 keeps every sample_stride-th item of the cost-sorted
 // queue (autotuning trials; the outputs are then partial).
+// arena non-null: every device buffer of the plan comes from it (the
+// scratch array of twopcf_count_noalloc) and no catalogue is gridded anew.
 int build_plan(twopcf_ctx *ctx, const twopcf_job *jobs, int n_jobs, int method, const twopcf_tuning *tuning,
-               Plan &plan, int sample_stride = 1) {
+               Plan &plan, int sample_stride = 1, Arena *arena = nullptr) {
 	const bool fast = method == TWOPCF_FP32_RECHECK;
+	// device buffer of the plan: from the arena, or hipMalloc owned by the plan
+	auto dev_alloc = [&](void **p, size_t bytes) -> int {
+		bytes = std::max<size_t>(bytes, 16);
+		if (arena) {
+			const size_t at = (arena->used + 255) & ~size_t(255);
+			if (at > arena->cap || bytes > arena->cap - at)
+				return fail(ctx, TWOPCF_E_LIMIT,
+				            "job set needs more than the " + std::to_string(arena->cap >> 20) +
+				                " MiB scratch of twopcf_count_noalloc");
+			*p = arena->base + at;
+			arena->used = at + bytes;
+			return 0;
+		}
+		hipError_t e = hipMalloc(p, bytes);
+		if (e != hipSuccess) return fail(ctx, TWOPCF_E_HIP, std::string("hipMalloc: ") + hipGetErrorString(e));
+		plan.owned.push_back(*p);
+		return 0;
+	};
 	int subdiv = fast ? 2 : 1;
 	// region slots: 12 requested on the batched path (as many as fit in LDS)
 	int max_slots = fast ? 12 : 8;
@@ -1918,9 +1994,9 @@ int build_plan(twopcf_ctx *ctx, const twopcf_job *jobs, int n_jobs, int method, 
 		Prepared &P = prep[q];
 		SortedCat *A = nullptr, *B = nullptr;
 		int rc;
-		if ((rc = grid_catalog(ctx, H.a, H.g, &A))) return rc;
+		if ((rc = grid_catalog(ctx, H.a, H.g, &A, !arena))) return rc;
 		if (H.b) {
-			if ((rc = grid_catalog(ctx, H.b, H.g, &B))) return rc;
+			if ((rc = grid_catalog(ctx, H.b, H.g, &B, !arena))) return rc;
 		} else {
 			B = A;
 		}
@@ -2117,12 +2193,10 @@ int build_plan(twopcf_ctx *ctx, const twopcf_job *jobs, int n_jobs, int method, 
 			float *ut = nullptr;
 			const size_t n = H.up_bins.size();
 			for (void **p : {(void **)&ub, (void **)&uc, (void **)&ut}) {
-				hipError_t e = hipMalloc(p, (n + 1) * 8);
-				if (e != hipSuccess) {
+				if (int rc = dev_alloc(p, (n + 1) * 8)) {
 					plan.release();
-					return fail(ctx, TWOPCF_E_HIP, std::string("hipMalloc: ") + hipGetErrorString(e));
+					return rc;
 				}
-				plan.owned.push_back(*p);
 			}
 			HIPCK(ctx, hipMemcpy(ub, H.up_bins.data(), n * 8, hipMemcpyHostToDevice));
 			HIPCK(ctx, hipMemcpy(uc, H.up_centre.data(), n * 8, hipMemcpyHostToDevice));
@@ -2261,12 +2335,7 @@ int build_plan(twopcf_ctx *ctx, const twopcf_job *jobs, int n_jobs, int method, 
 	double *d_pool_d = nullptr;
 	unsigned int *d_queue = nullptr;
 	auto release = [&]() { plan.release(); };
-	auto alloc = [&](void **p, size_t bytes) -> int {
-		hipError_t e = hipMalloc(p, std::max<size_t>(bytes, 16));
-		if (e != hipSuccess) return fail(ctx, TWOPCF_E_HIP, std::string("hipMalloc: ") + hipGetErrorString(e));
-		plan.owned.push_back(*p);
-		return 0;
-	};
+	auto alloc = [&](void **p, size_t bytes) -> int { return dev_alloc(p, bytes); };
 	int rc = 0;
 	if ((rc = alloc((void **)&d_jobs, sizeof(JobDev) * std::max(n_jobs, 1))) ||
 	    (rc = alloc((void **)&d_items, sizeof(WorkItem) * std::max<size_t>(items.size(), 1))) ||
@@ -2532,6 +2601,59 @@ int enqueue_plan(twopcf_ctx *ctx, Plan &plan) {
 	return 0;
 }
 
+// Copy a plan's pools and counters back into the jobs' host outputs (the
+// cross-region full histograms formed exactly in u64) and fill *timing.
+int read_results(twopcf_ctx *ctx, const Plan &plan_, const twopcf_job *jobs, int n_jobs, bool built, double t_start,
+                 twopcf_timing *timing) {
+	const Plan *plan = &plan_;
+	const size_t pool_u = plan->pool_u, pool_d = plan->pool_d;
+	std::vector<unsigned long long> hu(pool_u), hcnt(2 * (size_t)n_jobs);
+	std::vector<double> hd(pool_d);
+	if (pool_u) HIPCK(ctx, hipMemcpyAsync(hu.data(), plan->d_pool_u, pool_u * 8, hipMemcpyDeviceToHost, ctx->stream));
+	if (pool_d) HIPCK(ctx, hipMemcpyAsync(hd.data(), plan->d_pool_d, pool_d * 8, hipMemcpyDeviceToHost, ctx->stream));
+	if (n_jobs) HIPCK(ctx, hipMemcpyAsync(hcnt.data(), plan->d_cnt, hcnt.size() * 8, hipMemcpyDeviceToHost, ctx->stream));
+	HIPCK(ctx, hipStreamSynchronize(ctx->stream));
+	float kms = 0.0f;
+	HIPCK(ctx, hipEventElapsedTime(&kms, ctx->ev0, ctx->ev1));
+	uint64_t tot_acc = 0, tot_rc = 0;
+	for (int q = 0; q < n_jobs; ++q) {
+		const twopcf_job &J = jobs[q];
+		const PlanOut &P = plan->outs[q];
+		if (P.cross) {
+			// exact in u64: sum_r touching[r] = same + 2 cross
+			for (size_t b = 0; b < P.full_n; ++b) {
+				unsigned long long t = 0;
+				for (size_t e = b; e < P.jk_n; e += P.full_n) t += hu[P.jk_off + e];
+				J.full[b] = (double)(t - hu[P.full_off + b]);
+			}
+		} else {
+			for (size_t b = 0; b < P.full_n; ++b)
+				J.full[b] = P.weighted ? hd[P.full_off + b] : (double)hu[P.full_off + b];
+		}
+		if (J.jk) {
+			if (J.n_jk > 1) {
+				for (size_t e = 0; e < P.jk_n; ++e) J.jk[e] = P.weighted ? hd[P.jk_off + e] : (double)hu[P.jk_off + e];
+			} else {
+				for (size_t b = 0; b < P.full_n; ++b) J.jk[b] = J.full[b];   // every pair touches region 0
+			}
+		}
+		if (J.n_pairs) *J.n_pairs = hcnt[2 * q];
+		if (J.n_rechecked) *J.n_rechecked = hcnt[2 * q + 1];
+		tot_acc += hcnt[2 * q];
+		tot_rc += hcnt[2 * q + 1];
+	}
+	if (timing) {
+		timing->kernel_ms = kms;
+		timing->prep_ms = built ? plan->prep_ms : 0.0;
+		timing->total_ms = now_ms() - t_start;
+		timing->n_launches = (int)plan->launches.size();
+		timing->n_work_items = plan->n_items;
+		timing->n_pairs = tot_acc;
+		timing->n_rechecked = tot_rc;
+	}
+	return 0;
+}
+
 }  // namespace
 
 extern "C" {
@@ -2591,54 +2713,42 @@ int twopcf_count(twopcf_ctx *ctx, const twopcf_job *jobs, int n_jobs, int method
 		if (rc) return rc;
 	}
 	HIPCK(ctx, hipEventRecord(ctx->ev1, ctx->stream));
+	return read_results(ctx, *plan, jobs, n_jobs, built, t_start, timing);
+}
 
-	// ---- results ----
-	const size_t pool_u = plan->pool_u, pool_d = plan->pool_d;
-	std::vector<unsigned long long> hu(pool_u), hcnt(2 * (size_t)n_jobs);
-	std::vector<double> hd(pool_d);
-	if (pool_u) HIPCK(ctx, hipMemcpyAsync(hu.data(), plan->d_pool_u, pool_u * 8, hipMemcpyDeviceToHost, ctx->stream));
-	if (pool_d) HIPCK(ctx, hipMemcpyAsync(hd.data(), plan->d_pool_d, pool_d * 8, hipMemcpyDeviceToHost, ctx->stream));
-	if (n_jobs) HIPCK(ctx, hipMemcpyAsync(hcnt.data(), plan->d_cnt, hcnt.size() * 8, hipMemcpyDeviceToHost, ctx->stream));
-	HIPCK(ctx, hipStreamSynchronize(ctx->stream));
-	float kms = 0.0f;
-	HIPCK(ctx, hipEventElapsedTime(&kms, ctx->ev0, ctx->ev1));
-	uint64_t tot_acc = 0, tot_rc = 0;
-	for (int q = 0; q < n_jobs; ++q) {
-		const twopcf_job &J = jobs[q];
-		const PlanOut &P = plan->outs[q];
-		if (P.cross) {
-			// exact in u64: sum_r touching[r] = same + 2 cross
-			for (size_t b = 0; b < P.full_n; ++b) {
-				unsigned long long t = 0;
-				for (size_t e = b; e < P.jk_n; e += P.full_n) t += hu[P.jk_off + e];
-				J.full[b] = (double)(t - hu[P.full_off + b]);
-			}
-		} else {
-			for (size_t b = 0; b < P.full_n; ++b)
-				J.full[b] = P.weighted ? hd[P.full_off + b] : (double)hu[P.full_off + b];
-		}
-		if (J.jk) {
-			if (J.n_jk > 1) {
-				for (size_t e = 0; e < P.jk_n; ++e) J.jk[e] = P.weighted ? hd[P.jk_off + e] : (double)hu[P.jk_off + e];
-			} else {
-				for (size_t b = 0; b < P.full_n; ++b) J.jk[b] = J.full[b];   // every pair touches region 0
-			}
-		}
-		if (J.n_pairs) *J.n_pairs = hcnt[2 * q];
-		if (J.n_rechecked) *J.n_rechecked = hcnt[2 * q + 1];
-		tot_acc += hcnt[2 * q];
-		tot_rc += hcnt[2 * q + 1];
+// twopcf_count without device allocation: the plan is built for this call
+// with its device buffers in the scratch array (through the address
+// resolved at context creation, never the array's name), launched without a
+// graph, and its outputs read back before returning.  The catalogues must
+// already be gridded for the job set's geometry (cached by an earlier
+// twopcf_count of the same jobs and tuning).  The output pools and counters
+// are cleared by fill kernels on every call; calls that overlap on one GPU
+// share the array and race.
+int twopcf_count_noalloc(twopcf_ctx *ctx, const twopcf_job *jobs, int n_jobs, int method,
+                         const twopcf_tuning *tuning, twopcf_timing *timing) {
+	const double t_start = now_ms();
+	if (!ctx || (!jobs && n_jobs > 0) || n_jobs < 0) return fail(ctx, TWOPCF_E_ARG, "null argument");
+	if (method != TWOPCF_PLAIN_FP64 && method != TWOPCF_FP32_RECHECK) return fail(ctx, TWOPCF_E_ARG, "bad method");
+	for (int q = 0; q < n_jobs; ++q)
+		if (!jobs[q].a || !jobs[q].full) return fail(ctx, TWOPCF_E_ARG, "job needs a catalogue and a full[] output");
+	HIPCK(ctx, hipSetDevice(ctx->device));
+	void *scr = noalloc_scratch_ptr(ctx->device);
+	if (!scr) return fail(ctx, TWOPCF_E_ARG, "no scratch array on this device");
+	twopcf_tuning tu{};
+	if (tuning) {
+		tu = *tuning;
+	} else if (method == TWOPCF_FP32_RECHECK) {
+		auto t = ctx->tuned.find(job_set_key(jobs, n_jobs, method, nullptr));
+		if (t != ctx->tuned.end()) tu = t->second;
 	}
-	if (timing) {
-		timing->kernel_ms = kms;
-		timing->prep_ms = built ? plan->prep_ms : 0.0;
-		timing->total_ms = now_ms() - t_start;
-		timing->n_launches = (int)plan->launches.size();
-		timing->n_work_items = plan->n_items;
-		timing->n_pairs = tot_acc;
-		timing->n_rechecked = tot_rc;
-	}
-	return 0;
+	Arena arena{(char *)scr, NOALLOC_SCRATCH_BYTES, 0};
+	Plan plan;   // owns no device memory and no graph
+	int rc = build_plan(ctx, jobs, n_jobs, method, &tu, plan, 1, &arena);
+	if (rc) return rc;
+	HIPCK(ctx, hipEventRecord(ctx->ev0, ctx->stream));
+	if ((rc = enqueue_plan(ctx, plan))) return rc;
+	HIPCK(ctx, hipEventRecord(ctx->ev1, ctx->stream));
+	return read_results(ctx, plan, jobs, n_jobs, true, t_start, timing);
 }
 
 // Autotuning of the batched method: trials on every 8th item of the

@@ -14,13 +14,13 @@ MONOPOLE, SIGMA_PI, S_MU, ANGULAR = 0, 1, 2, 3
 PLAIN_FP64, FP32_RECHECK = 0, 1
 W_PLAIN, W_INVP_3D, W_ANGUP_3D, W_ANG_IW, W_ANG_INVP, W_ANG_W = range(6)
 PI_ANGULAR, PI_MIDPOINT = 0, 1
-ABI_VERSION = 3   # TWOPCF_ABI_VERSION of the structures below
+ABI_VERSION = 4   # TWOPCF_ABI_VERSION of the structures below
 
 EXPORTS = [
     "twopcf_abi_version", "twopcf_ctx_create", "twopcf_ctx_destroy", "twopcf_last_error",
     "twopcf_device_info", "twopcf_catalog_upload", "twopcf_catalog_free", "twopcf_catalog_size",
     "twopcf_count", "twopcf_s2_thresholds", "twopcf_axis_thresholds", "twopcf_comoving_distance",
-    "twopcf_eq_to_cart", "twopcf_autotune",
+    "twopcf_eq_to_cart", "twopcf_autotune", "twopcf_count_noalloc",
 ]
 
 
@@ -82,6 +82,7 @@ def lib():
         L.twopcf_catalog_size.restype = C.c_int64
         L.twopcf_count.argtypes = [C.c_void_p, C.POINTER(JobDesc), C.c_int, C.c_int, C.POINTER(Tuning),
                                    C.POINTER(Timing)]
+        L.twopcf_count_noalloc.argtypes = L.twopcf_count.argtypes
         L.twopcf_s2_thresholds.argtypes = [C.c_int, C.POINTER(AxisDesc), C.POINTER(AxisDesc), C.c_void_p]
         L.twopcf_axis_thresholds.argtypes = [C.POINTER(AxisDesc), C.c_void_p]
         L.twopcf_comoving_distance.argtypes = [C.c_double, C.c_double, C.c_double, C.c_double, C.c_int64,
@@ -203,19 +204,22 @@ class Context:
         self.cats.append(cat)
         return cat
 
-    def count(self, jobs, method=FP32_RECHECK, tuning=None):
+    def count(self, jobs, method=FP32_RECHECK, tuning=None, noalloc=False):
         """jobs: list of dicts(kind, a, b, axes=[(min,max,n,base), ...], n_jk).
         Fills and returns, per job, dict(full, jk, n_pairs, n_rechecked);
         plus the timing record.  tuning None: the autotuned choice for this
-        job set if autotune() ran, else the defaults."""
+        job set if autotune() ran, else the defaults.  noalloc: through
+        twopcf_count_noalloc (no device allocation; the catalogues must be
+        gridded for the job set by an earlier count)."""
         descs, outs = self._descs(jobs)
         t = Timing()
         tu = None
         if tuning is not None:
             tu = Tuning(int(tuning.get("cell_subdiv", 0)), int(tuning.get("max_slots", 0)),
                         int(tuning.get("i_chunk", 0)))
-        self.check(lib().twopcf_count(self.h, descs, len(jobs), method,
-                                      C.byref(tu) if tu is not None else None, C.byref(t)), "count")
+        fn = lib().twopcf_count_noalloc if noalloc else lib().twopcf_count
+        self.check(fn(self.h, descs, len(jobs), method, C.byref(tu) if tu is not None else None, C.byref(t)),
+                   "count_noalloc" if noalloc else "count")
         res = []
         for o in outs:
             o.pop("_up", None)
