@@ -393,3 +393,76 @@ def test_fused_monopole_s_mu(ctx, oracle_lib, name, mono):
             np.testing.assert_allclose(r["full"], full, rtol=1e-12, atol=0, err_msg=f"job {k}")
     for c in cats.values():
         c.free()
+
+
+def _eq_cats(L, ctx, o, case, unit_w=False, regions=True):
+    """Library and oracle catalogues of an equatorial golden case: unit
+    weights when unit_w; the case's regions, or all objects in region 0."""
+    cats, oc = {}, {}
+    for tag, a in case.inputs.items():
+        d = L.comoving_distance(0.25, 1.0, 0.0, 1.0, a["z"])
+        x, y, z, nx, ny, nz = L.eq_to_cart(a["ra"], a["dec"], d)
+        w = np.ones(x.size, np.float32) if unit_w else a["w"]
+        jk = a["jk"] if regions else np.zeros(x.size, np.int32)
+        cats[tag] = ctx.upload(x, y, z, nx, ny, nz, w, jk)
+        oc[tag] = o.Catalog(x, y, z, nx, ny, nz, w, jk)
+    return cats, oc
+
+
+def test_fused_monopole_near_lds_limit(ctx, oracle_lib):
+    """Unweighted s-mu 120 x 50 with a 120-bin monopole on the same
+    catalogues: the pair fuses (its two histograms with their lane copies
+    fit in LDS) and, fused or not, the call succeeds in both methods; every
+    histogram equals the oracle's count of that job alone."""
+    from two_pcf_amd import lib as L
+    o = oracle_lib
+    case = load_case("eq_clustered")
+    cats, oc = _eq_cats(L, ctx, o, case, unit_w=True, regions=False)
+    smu = [(0.0, 60.0, 120, 1.05), (0.0, 1.0, 50, 0.0)]
+    mono = [(0.0, 60.0, 120, 1.05)]
+    jobs, want = [], []
+    for a, b in (("data", None), ("data", "randoms"), ("randoms", None)):
+        for kind, axes in ((2, smu), (0, mono)):
+            jobs.append(dict(kind=kind, a=cats[a], b=cats[b] if b else None, axes=axes, n_jk=1))
+            ax1 = o.axis(*axes[0])
+            ax2 = o.axis(*axes[1]) if len(axes) > 1 else None
+            want.append(o.count(kind, oc[a], oc[b] if b else None, ax1, ax2, 1))
+    for method in (0, 1):
+        print(f"[gpu] fused monopole near the LDS limit method={method}", flush=True)
+        res, t = ctx.count(jobs, method=method)
+        for k, (r, (full, _, acc)) in enumerate(zip(res, want)):
+            assert r["n_pairs"] == acc, f"job {k}"
+            np.testing.assert_array_equal(r["full"], full, err_msg=f"job {k}")
+    for c in cats.values():
+        c.free()
+
+
+@pytest.mark.parametrize("kind,axes", [
+    (2, [(0.0, 60.0, 100, 1.05), (0.0, 1.0, 50, 0.0)]),        # weighted s-mu, 5000 bins
+    (1, [(0.0, 40.0, 200, 0.0), (0.0, 40.0, 100, 0.0)]),       # weighted sigma-pi, 20000 bins
+], ids=["s_mu_5000", "sigma_pi_20000"])
+def test_large_weighted_histograms(ctx, oracle_lib, kind, axes):
+    """Weighted histograms with 9 regions too large to keep beside region
+    slots in LDS (5000 bins x 8 B, and 20000 bins x 8 B = 160 KB, which no
+    LDS holds): the library drops lane copies, then keeps the full histogram
+    in the global pool and as many region slots as fit (none for the second),
+    the other regions' sums going to the global pool.  DD, DR and RR equal
+    the oracle's in both methods."""
+    from two_pcf_amd import lib as L
+    o = oracle_lib
+    case = load_case("eq_clustered")
+    cats, oc = _eq_cats(L, ctx, o, case)
+    ax1 = o.axis(*axes[0])
+    ax2 = o.axis(*axes[1])
+    spec = (("data", None), ("data", "randoms"), ("randoms", None))
+    jobs = [dict(kind=kind, a=cats[a], b=cats[b] if b else None, axes=axes, n_jk=case.n_jk) for a, b in spec]
+    want = [o.count(kind, oc[a], oc[b] if b else None, ax1, ax2, case.n_jk) for a, b in spec]
+    for method in (0, 1):
+        print(f"[gpu] large weighted histogram kind={kind} nb={axes[0][2] * axes[1][2]} method={method}", flush=True)
+        res, _ = ctx.count(jobs, method=method)
+        for k, (r, (full, sub, acc)) in enumerate(zip(res, want)):
+            assert r["n_pairs"] == acc, f"job {k}"
+            np.testing.assert_allclose(r["full"], full, rtol=1e-12, atol=0, err_msg=f"job {k}")
+            np.testing.assert_allclose(r["jk"], sub, rtol=1e-12, atol=0, err_msg=f"job {k}")
+    for c in cats.values():
+        c.free()

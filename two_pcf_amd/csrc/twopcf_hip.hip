@@ -97,6 +97,7 @@ struct JobDev {
 	int s_log;
 	float mu_h0, mu_c2;                              // fast mu test: 1/2 - delta = h0 - c2 * rsq(s2)
 	int f_shift;                                     // fp32 path: full histogram in 2^f_shift lane copies
+	int full_global;                                 // full histogram too large for LDS: added to the pool
 	int x1_off;                                      // sigma-pi fp32: sqrtf-domain thresholds of the sigma axis
 	float p_min, p_ilb_ln2, p_ils, p_inv_step;       // sigma-pi fp32: pi-axis index estimate
 	int p_log;
@@ -440,8 +441,11 @@ __device__ void process_item(const JobDev &J, const WorkItem &W, const int4 *__r
 	int *tslot = tgj + TILE;
 	float *tab = (float *)(smem + L.tables);
 	int *sreg = (int *)(smem + L.slots);
+	// the full histogram (none in LDS when it is kept in the global pool),
+	// then the region slots
+	const bool full_g = J.full_global != 0;
 	HT *F = (HT *)(smem + L.hist);
-	HT *H = F + nb;
+	HT *H = F + (full_g ? 0 : nb);
 
 	// tables and slot regions
 	const int n_tab = L.slots - L.tables;
@@ -490,8 +494,14 @@ __device__ void process_item(const JobDev &J, const WorkItem &W, const int4 *__r
 		}
 	};
 	auto add = [&](int bin, float w, int slot_j) {
-		if (WEIGHTED) atomicAdd((double *)&F[bin], (double)w);
-		else atomicAdd((unsigned int *)&F[bin], 1u);
+		if (full_g) {
+			if (WEIGHTED) gadd(&J.full_d[bin], (double)w);
+			else gadd(&J.full_u[bin], 1ull);
+		} else if (WEIGHTED) {
+			atomicAdd((double *)&F[bin], (double)w);
+		} else {
+			atomicAdd((unsigned int *)&F[bin], 1u);
+		}
 		if (jk) {
 			add_region(slot_i, bin, w);
 			if (slot_j != slot_i) add_region(slot_j, bin, w);
@@ -551,7 +561,7 @@ __device__ void process_item(const JobDev &J, const WorkItem &W, const int4 *__r
 	}
 
 	// flush and clear the LDS histograms
-	for (int b = tid; b < nb; b += BLOCK) {
+	for (int b = tid; b < (full_g ? 0 : nb); b += BLOCK) {
 		HT v = F[b];
 		if (v != (HT)0) {
 			if (WEIGHTED) gadd(&J.full_d[b], (double)v);
@@ -637,10 +647,12 @@ __device__ void process_item_fast(const JobDev &J, const WorkItem &W, const int4
 	int *sreg = (int *)(smem + L.slots);
 	// the full histogram: 2^f_shift copies, bin-major, lane l adding to copy
 	// l mod 2^f_shift (small histograms: lanes of one atomic never collide)
+	// (none in LDS when the histogram is kept in the global pool)
 	const int f_shift = J.f_shift;
 	const int f_lane = lane & ((1 << f_shift) - 1);
+	const bool full_g = J.full_global != 0;
 	HT *F = (HT *)(smem + L.hist);
-	HT *H = F + (nb << f_shift);
+	HT *H = F + (full_g ? 0 : (nb << f_shift));
 
 	for (int k = tid; k < n_slots; k += BLOCK) sreg[k] = slot_regions[W.slot_base + k];
 	const bool use_map = jk && J.n_jk <= REGION_MAP;
@@ -742,6 +754,11 @@ __device__ void process_item_fast(const JobDev &J, const WorkItem &W, const int4
 	const int f_bshift = f_shift + (WEIGHTED ? 3 : 2);
 	const bool cross_only = !WEIGHTED && jk;
 	auto add_full = [&](int bin, float w) {
+		if (full_g) {
+			if (WEIGHTED) gadd(&J.full_d[bin], (double)w);
+			else gadd(&J.full_u[bin], 1ull);
+			return;
+		}
 		HT *const fp = (HT *)(F_lane + (bin << f_bshift));
 		if (WEIGHTED) atomicAdd((double *)fp, (double)w);
 		else atomicAdd((unsigned int *)fp, 1u);
@@ -1208,7 +1225,7 @@ __device__ void process_item_fast(const JobDev &J, const WorkItem &W, const int4
 	if (rq_n) flush_rq();
 	__syncthreads();   // every wave's rechecked pairs are in the LDS histograms
 
-	for (int b = tid; b < nb; b += BLOCK) {
+	for (int b = tid; b < (full_g ? 0 : nb); b += BLOCK) {
 		HT v = (HT)0;
 		for (int r = 0; r < (1 << f_shift); ++r) {
 			v += F[(b << f_shift) + r];
@@ -1936,7 +1953,10 @@ int build_plan(twopcf_ctx *ctx, const twopcf_job *jobs, int n_jobs, int method, 
 			tab_floats += 9 * (H.ax1.n + 1) + (H.ax2.n + 1) + 32;
 		}
 	}
-	const int lds_fixed = (int)(((std::max(tile_bytes(true), tile_bytes(false)) + 15) & ~15) + ((tab_floats * 4 + 15) & ~15) +
+	// LDS besides the histograms: the method's tile (with the fp32 path's
+	// first-side chunk and pair queues), the tables and the slot list, as
+	// the launch layout places them
+	const int lds_fixed = (int)(((tile_bytes(fast) + 15) & ~15) + ((tab_floats * 4 + 15) & ~15) +
 	                            ((MAX_SLOTS * 4 + 15) & ~15));
 	const size_t lds_budget = ctx->lds_max;
 	// The popcount rules of weighted angular and monopole jobs (small
@@ -2230,17 +2250,33 @@ int build_plan(twopcf_ctx *ctx, const twopcf_job *jobs, int n_jobs, int method, 
 			D.m_f_shift = copies_of(M.nb, M.weighted);
 			P.f_elems += M.nb << D.m_f_shift;
 		}
+		// LDS for the histograms: the full one (with its lane copies) and, with
+		// regions, at least one slot histogram.  When they do not fit, the
+		// lane copies go first, then the full histogram moves to the global
+		// pool (its pairs are added there with atomics); regions without a
+		// slot are added to the global pool as well, and with no room for a
+		// slot every region is.
+		const bool jk = H.j->n_jk > 1;
 		const int hist_bytes = H.nb * (H.weighted ? 8 : 4);
-		const int f_bytes = (P.f_elems) * (H.weighted ? 8 : 4);
-		const int slot_cap = std::max(
-		    1, std::min(max_slots,
-		                (int)((lds_budget - lds_fixed - f_bytes - (P.mask_bytes ? P.mask_bytes + 16 : 0)) /
-		                      std::max(hist_bytes, 1))));
-		if ((long long)lds_fixed + f_bytes + hist_bytes > (long long)ctx->lds_max)
-			return fail(ctx, TWOPCF_E_LIMIT, "histogram does not fit in LDS");
+		const long long avail = (long long)lds_budget - lds_fixed - (P.mask_bytes ? P.mask_bytes + 16 : 0);
+		long long f_bytes = (long long)P.f_elems * (H.weighted ? 8 : 4);
+		if (fused[q] < 0 && f_bytes + (jk ? hist_bytes : 0) > avail && D.f_shift > 0) {
+			D.f_shift = 0;
+			P.f_elems = H.nb;
+			f_bytes = hist_bytes;
+		}
+		if (f_bytes + (jk ? hist_bytes : 0) > avail) {
+			if (fused[q] >= 0) return fail(ctx, TWOPCF_E_LIMIT, "fused histograms do not fit in LDS");
+			D.full_global = 1;
+			P.f_elems = 0;
+			f_bytes = 0;
+		}
+		if (avail < f_bytes) return fail(ctx, TWOPCF_E_LIMIT, "tables do not fit in LDS");
+		const int slot_cap = jk ? (int)std::min<long long>(max_slots, (avail - f_bytes) / std::max(hist_bytes, 1)) : 0;
+		// group rows by the regions they bring only when there are slots to fill
+		const bool group_regions = jk && slot_cap > 0;
 		const int n_cell = H.g.dim[0] * H.g.dim[1] * H.g.dim[2];
 		const int d1 = H.g.dim[1], d2 = H.g.dim[2];
-		const bool jk = H.j->n_jk > 1;
 		std::vector<int> rws;
 		for (int r = 0; r < D.n_rows; ++r) rws.push_back(r);
 		// scratch reused across cells and chunks (no allocation per row)
@@ -2273,7 +2309,7 @@ int build_plan(twopcf_ctx *ctx, const twopcf_job *jobs, int n_jobs, int method, 
 				const int ni = std::min(i_chunk, ie - i0);
 				// group rows by region budget
 				regs_i.clear();
-				if (jk) {
+				if (group_regions) {
 					// regions of this chunk: those present in its cell (a
 					// superset of the chunk's regions)
 					for (int k = A->reg_off[c]; k < A->reg_off[c + 1]; ++k) regs_i.push_back(A->reg_ids[k]);
@@ -2285,7 +2321,7 @@ int build_plan(twopcf_ctx *ctx, const twopcf_job *jobs, int n_jobs, int method, 
 					double cost = 0;
 					size_t g1 = g0;
 					for (; g1 < ri.size(); ++g1) {
-						if (jk) {
+						if (group_regions) {
 							add.clear();
 							const int4 row = row_pool[D.row_off + ri[g1].r];
 							const int qx = cx + row.x, qy = cy + row.y;
@@ -2312,8 +2348,8 @@ int build_plan(twopcf_ctx *ctx, const twopcf_job *jobs, int n_jobs, int method, 
 					W.row_mask = mask;
 					W.slot_base = (int)slot_pool.size();
 					if ((int)regs.size() > slot_cap) regs.resize(slot_cap);   // the rest: global atomics
-					W.n_slots = jk ? (int)regs.size() : 0;
-					if (jk) slot_pool.insert(slot_pool.end(), regs.begin(), regs.end());
+					W.n_slots = group_regions ? (int)regs.size() : 0;
+					if (group_regions) slot_pool.insert(slot_pool.end(), regs.begin(), regs.end());
 					items.push_back(W);
 					item_cost.push_back(cost);
 					g0 = g1;
@@ -2431,14 +2467,16 @@ int build_plan(twopcf_ctx *ctx, const twopcf_job *jobs, int n_jobs, int method, 
 	int max_nb = 1, max_tab = (int)tables.size();
 	for (auto &H : hj) max_nb = std::max(max_nb, H.nb);
 	(void)max_tab_n1;
-	// f_elems: elements of the largest full histogram (with its lane copies)
-	auto layout = [&](bool fast_kernel, bool weighted, int nb, int slots, int f_elems, int mask_bytes) {
+	// h_elems: histogram elements of the launch's largest job: its full
+	// histogram in LDS (with lane copies; none when kept in the global pool)
+	// and its region slots
+	auto layout = [&](bool fast_kernel, bool weighted, int h_elems, int mask_bytes) {
 		Lds L;
 		L.tile = 0;
 		L.tables = (tile_bytes(fast_kernel) + 15) & ~15;
 		L.slots = L.tables + (((int)tables.size() * 4 + 15) & ~15);
 		L.hist = L.slots + ((MAX_SLOTS * 4 + 15) & ~15);
-		L.total = L.hist + (f_elems + nb * slots) * (weighted ? 8 : 4);
+		L.total = L.hist + h_elems * (weighted ? 8 : 4);
 		L.masks = 0;
 		if (mask_bytes > 0) {
 			L.masks = (L.total + 15) & ~15;
@@ -2446,10 +2484,10 @@ int build_plan(twopcf_ctx *ctx, const twopcf_job *jobs, int n_jobs, int method, 
 		}
 		return L;
 	};
-	auto launch = [&](int kind, bool weighted, bool mid, bool fz, const WorkItem *it, int n_it, int nb, int slots,
-	                  unsigned *queue, int grid, int f_elems, int mask_bytes) -> int {
+	auto launch = [&](int kind, bool weighted, bool mid, bool fz, const WorkItem *it, int n_it, int h_elems,
+	                  unsigned *queue, int grid, int mask_bytes) -> int {
 		const bool fast_kernel = fast;
-		Lds L = layout(fast_kernel, weighted, nb, slots, f_elems, mask_bytes);
+		Lds L = layout(fast_kernel, weighted, h_elems, mask_bytes);
 		if ((size_t)L.total > ctx->lds_max)
 			return fail(ctx, TWOPCF_E_LIMIT, "histogram does not fit in LDS (" + std::to_string(L.total) + " B)");
 		void (*fn)(const JobDev *, const WorkItem *, int, unsigned *, const int4 *, const int *, const float *, int,
@@ -2487,8 +2525,8 @@ int build_plan(twopcf_ctx *ctx, const twopcf_job *jobs, int n_jobs, int method, 
 			while (k1 < items.size() && items[k1].job == q) ++k1;
 			int slots = 0;
 			for (size_t k = k0; k < k1; ++k) slots = std::max(slots, items[k].n_slots);
-			if ((rc = launch(hj[q].j->kind, prep[q].weighted, false, false, d_items + k0, (int)(k1 - k0), prep[q].nb, slots,
-			                 nullptr, (int)(k1 - k0), prep[q].f_elems, 0))) {
+			if ((rc = launch(hj[q].j->kind, prep[q].weighted, false, false, d_items + k0, (int)(k1 - k0),
+			                 prep[q].f_elems + prep[q].nb * slots, nullptr, (int)(k1 - k0), 0))) {
 				release();
 				return rc;
 			}
@@ -2526,25 +2564,25 @@ int build_plan(twopcf_ctx *ctx, const twopcf_job *jobs, int n_jobs, int method, 
 			const int mid = std::get<4>(sp) & 1;
 			const bool fz = (std::get<4>(sp) & 2) != 0;   // s-mu jobs with a fused monopole
 			const bool st = (std::get<4>(sp) & 4) != 0;   // jobs with the LDS mask copy
-			int nb = 1, f_elems = 1, mask_bytes = 0;
+			// per job: its largest slot count over the launch's items
+			std::vector<int> slots_of(n_jobs, 0);
+			for (size_t k = std::get<2>(sp); k < std::get<2>(sp) + std::get<3>(sp); ++k)
+				slots_of[flat[k].job] = std::max(slots_of[flat[k].job], flat[k].n_slots);
+			int h_elems = 1, mask_bytes = 0;
 			for (int q = 0; q < n_jobs; ++q)
 				if (!absorbed[q] && hj[q].j->kind == kind && prep[q].weighted == weighted && mid_of(q) == mid &&
 				    (fused[q] >= 0) == fz && (prep[q].jd.mstage > 0) == st) {
-					nb = std::max(nb, hj[q].nb);
-					f_elems = std::max(f_elems, prep[q].f_elems);
+					h_elems = std::max(h_elems, prep[q].f_elems + hj[q].nb * slots_of[q]);
 					mask_bytes = std::max(mask_bytes, prep[q].mask_bytes);
 				}
-			int slots = 0;
-			for (size_t k = std::get<2>(sp); k < std::get<2>(sp) + std::get<3>(sp); ++k)
-				slots = std::max(slots, flat[k].n_slots);
 			// jobs of other rules in the same launch may leave no room for the
 			// mask copy: then every job of it reads its masks from global memory
-			if ((size_t)layout(true, weighted, nb, slots, f_elems, mask_bytes).total > ctx->lds_max) mask_bytes = 0;
-			Lds L = layout(true, weighted, nb, slots, f_elems, mask_bytes);
+			if ((size_t)layout(true, weighted, h_elems, mask_bytes).total > ctx->lds_max) mask_bytes = 0;
+			Lds L = layout(true, weighted, h_elems, mask_bytes);
 			const int per_cu = std::max(1, std::min(8, (int)(ctx->lds_max / std::max(L.total, 1))));
 			const int grid = std::min<long long>((long long)ctx->n_cu * per_cu, (long long)std::get<3>(sp));
-			if ((rc = launch(kind, weighted, mid != 0, fz, d_items + std::get<2>(sp), (int)std::get<3>(sp), nb, slots,
-			                 d_queue + qi, std::max(grid, 1), f_elems, mask_bytes))) {
+			if ((rc = launch(kind, weighted, mid != 0, fz, d_items + std::get<2>(sp), (int)std::get<3>(sp), h_elems,
+			                 d_queue + qi, std::max(grid, 1), mask_bytes))) {
 				release();
 				return rc;
 			}
