@@ -70,47 +70,137 @@ def make_catalogs(scale):
     return out, c["n_jk"]
 
 
-def cpu_baseline(cats, n_jk, frac, seed=5):
-    """The reference's OpenMP s-mu path (oracle/_ref/ref_counts, compiled
-    from the reference sources) on a seeded random subsample of both
-    catalogues; rate = accepted pairs / counting seconds."""
-    ref = os.path.join(ROOT, "oracle", "_ref", "ref_counts")
-    if not os.path.exists(ref):
-        return None
-    rng = np.random.Generator(np.random.PCG64(seed))
-    nthreads = min(os.cpu_count() or 1, 16)
+def cpu_threads():
+    """Host threads for the reference's OpenMP path: every core this process
+    may run on, within OMP_NUM_THREADS when that is set (a GPU host's share)."""
+    n = len(os.sched_getaffinity(0))
+    omp = os.environ.get("OMP_NUM_THREADS", "")
+    return min(n, int(omp)) if omp.isdigit() and int(omp) > 0 else n
+
+
+def host_cpu():
+    """lscpu's model name, sockets, cores per socket and threads per core."""
+    want = {"Model name": "model", "Socket(s)": "sockets", "Core(s) per socket": "cores_per_socket",
+            "Thread(s) per core": "threads_per_core", "CPU(s)": "cpus"}
+    out = {}
+    try:
+        text = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=30).stdout
+    except (OSError, subprocess.SubprocessError):
+        return out
+    for line in text.splitlines():
+        k, _, v = line.partition(":")
+        if k.strip() in want:
+            out[want[k.strip()]] = v.strip()
+    out["affinity_cpus"] = len(os.sched_getaffinity(0))
+    out["omp_num_threads_env"] = os.environ.get("OMP_NUM_THREADS")
+    return out
+
+
+REF_PARAMS = dict(data_file_type="ascii", random_file_type="ascii", coord_system="equatorial",
+                  ra_x_dataset_name="x", dec_y_dataset_name="y", z_z_dataset_name="z", weight_dataset_name="w",
+                  jk_dataset_name="jk", use_weights=0, n_jk_regions=0, omega_m=0.25, h=1.0, z_min=0.0, z_max=1.0,
+                  plot_monopole=0, monopole_filename="mono.txt", monopole_output_type="ascii", monopole_log_base=1.3,
+                  monopole_min=0.0, monopole_max=100.0, monopole_n_bins=30,
+                  plot_sigma_pi=0, sigma_pi_filename="sp.txt", sigma_pi_output_type="ascii", sigma_log_base=1.0,
+                  sigma_min=0.0, sigma_max=50.0, sigma_n_bins=50, pi_log_base=1.0, pi_min=0.0, pi_max=50.0,
+                  pi_n_bins=50, plot_s_mu=0, s_mu_filename="smu.txt", s_mu_output_type="ascii", s_log_base=1.3,
+                  s_min=0.0, s_max=100.0, s_n_bins=40, mu_n_bins=50,
+                  calculate_angular_dd=0, calculate_angular_dd_invpweights=0, calculate_angular_dr=0,
+                  calculate_angular_dr_invpweights=0, angular_dd_filename="add.h5",
+                  angular_dd_invpweights_filename="addinv.h5", angular_dr_filename="adr.h5",
+                  angular_dr_invpweights_filename="adrinv.h5", theta_max=0.175, theta_n_bins=40,
+                  theta_log_base=1.6, healpix_order=5, n_bitwise_runs=2048, bitwise_weight_dataset_name="BITWEIGHT",
+                  dither_weight_dataset_name="DITHER")
+
+
+def run_reference(binary, params, cols, td, extra_args=("0", "0")):
+    """Run a reference-linked driver (oracle/_ref, compiled from the
+    reference's sources) on catalogues written as ascii (cols: tag -> list of
+    columns; formats as the reference's reader takes them) or given as
+    files; returns (its JSON, counting seconds, end-to-end seconds)."""
+    p = dict(REF_PARAMS)
+    p.update(params, n_threads=cpu_threads())
+    for tag, key in (("data", "data_filename"), ("randoms", "random_filename")):
+        c = cols[tag]
+        if isinstance(c, str):
+            p[key] = c
+            continue
+        fn = os.path.join(td, tag + ".txt")
+        fmt = " ".join("%d" if np.issubdtype(np.asarray(v).dtype, np.integer) else
+                       ("%.9g" if np.asarray(v).dtype == np.float32 else "%.17g") for v in c) + "\n"
+        with open(fn, "w") as f:
+            for row in zip(*[np.asarray(v).tolist() for v in c]):
+                f.write(fmt % row)
+        p[key] = fn
+    pf = os.path.join(td, "params.txt")
+    with open(pf, "w") as f:
+        f.write("".join(f"{k} = {v}\n" for k, v in p.items()))
+    js = os.path.join(td, "out.json")
+    t0 = time.perf_counter()
+    subprocess.run([binary, pf, js, *extra_args], check=True, stdout=subprocess.DEVNULL, cwd=td)
+    wall = time.perf_counter() - t0
+    with open(js) as f:
+        r = json.load(f)
+    secs = sum(sum(v["seconds"]) for k, v in r.items() if isinstance(v, dict) and "seconds" in v)
+    secs += r.get("angular_seconds", 0.0)
+    return r, secs, wall
+
+
+def ref_binary(name):
+    b = os.path.join(ROOT, "oracle", "_ref", name)
+    if not os.path.exists(b):
+        raise FileNotFoundError(f"{b} is not built")
+    return b
+
+
+def subsample(n, frac, rng):
+    return rng.random(n) < frac
+
+
+def baseline_record(pairs, secs, wall, what):
+    return dict(value=pairs / secs, unit="pairs/s", cores=cpu_threads(), kind="reference", sample=what,
+                counting_s=secs, end_to_end_s=wall, host=host_cpu())
+
+
+def uniform_monopole_full():
+    """The uniform monopole case in full on the reference: 50k data + 100k
+    randoms uniform in [0, 500)^3, 30 log bins base 1.3 to 100, DD, DR, RR."""
+    from two_pcf_amd import synth
+    c = synth.uniform_monopole()
     with tempfile.TemporaryDirectory() as td:
-        files = {}
-        for tag in ("data", "randoms"):
-            a = cats[tag]
-            keep = rng.random(a["x"].size) < frac
-            arr = np.stack([a["ra"][keep], a["dec"][keep], a["zz"][keep]], axis=1)
-            fn = os.path.join(td, tag + ".txt")
-            with open(fn, "w") as f:
-                for (r, d, z), j in zip(arr.tolist(), a["jk"][keep].tolist()):
-                    f.write("%.17g %.17g %.17g %d\n" % (r, d, z, j))
-            files[tag] = (fn, int(keep.sum()))
-        params = dict(data_filename=files["data"][0], data_file_type="ascii",
-                      random_filename=files["randoms"][0], random_file_type="ascii",
-                      coord_system="equatorial", ra_x_dataset_name="x", dec_y_dataset_name="y",
-                      z_z_dataset_name="z", weight_dataset_name="w", jk_dataset_name="jk",
-                      use_weights=0, n_threads=nthreads, n_jk_regions=n_jk, omega_m=0.25, h=1.0,
-                      z_min=0.0, z_max=1.0, plot_monopole=0, plot_sigma_pi=0, plot_s_mu=1,
-                      s_mu_filename="smu.txt", s_mu_output_type="ascii", s_log_base=1.3, s_min=0.0,
-                      s_max=100.0, s_n_bins=40, mu_n_bins=50)
-        pf = os.path.join(td, "params.txt")
-        with open(pf, "w") as f:
-            f.write("".join(f"{k} = {v}\n" for k, v in params.items()))
-        js = os.path.join(td, "out.json")
-        subprocess.run([ref, pf, js, "0", "0"], check=True, stdout=subprocess.DEVNULL, cwd=td)
-        with open(js) as f:
-            r = json.load(f)["s_mu"]
-    pairs = sum(r["DD"]["full"]) / 2 + sum(r["DR"]["full"]) + sum(r["RR"]["full"]) / 2
-    secs = sum(r["seconds"])
-    return dict(value=pairs / secs, unit="pairs/s", cores=nthreads, kind="reference",
-                sample=f"seeded {frac:.0%} subsample of data and randoms "
-                       f"({files['data'][1]} + {files['randoms'][1]} objects, {pairs:.3g} pairs, "
-                       f"{secs:.1f} s of counting)")
+        r, secs, wall = run_reference(ref_binary("ref_counts"), dict(coord_system="cartesian", plot_monopole=1),
+                                      {t: [c[t][:, 0], c[t][:, 1], c[t][:, 2]] for t in ("data", "randoms")}, td)
+    m = r["monopole"]
+    pairs = sum(m["DD"]["full"]) / 2 + sum(m["DR"]["full"]) + sum(m["RR"]["full"]) / 2
+    return dict(workload="uniform monopole 50k + 100k in [0,500)^3, 30 log bins to 100, DD+DR+RR, full run",
+                pairs=pairs, counting_s=secs, end_to_end_s=wall, pairs_per_s=pairs / secs, cores=cpu_threads())
+
+
+def gpu_pairs(L, ctx, jobs_of_sample):
+    """Accepted pairs of the sample, counted by the library (the reference's
+    weighted sums do not give them)."""
+    res, t = ctx.count(jobs_of_sample, method=L.FP32_RECHECK)
+    return t["n_pairs"]
+
+
+def cpu_baseline_jackknife(cats, n_jk, frac, seed=5):
+    """The reference's OpenMP s-mu path (oracle/_ref/ref_counts) on a seeded
+    subsample of both catalogues (frac of the objects, frac^2 of the pairs);
+    rate = accepted pairs / counting seconds.  Also the uniform monopole case
+    in full."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    keep = {t: subsample(cats[t]["x"].size, frac, rng) for t in ("data", "randoms")}
+    cols = {t: [cats[t]["ra"][keep[t]], cats[t]["dec"][keep[t]], cats[t]["zz"][keep[t]], cats[t]["jk"][keep[t]]]
+            for t in ("data", "randoms")}
+    with tempfile.TemporaryDirectory() as td:
+        r, secs, wall = run_reference(ref_binary("ref_counts"), dict(plot_s_mu=1, n_jk_regions=n_jk), cols, td)
+    m = r["s_mu"]
+    pairs = sum(m["DD"]["full"]) / 2 + sum(m["DR"]["full"]) + sum(m["RR"]["full"]) / 2
+    out = baseline_record(pairs, secs, wall, f"seeded {frac:.0%} subsample of data and randoms "
+                          f"({int(keep['data'].sum())} + {int(keep['randoms'].sum())} objects, {pairs:.3g} pairs, "
+                          f"{frac * frac:.2%} of the pairs)")
+    out["uniform_monopole_full"] = uniform_monopole_full()
+    return out
 
 
 def copy_calibration(mib):
@@ -147,7 +237,7 @@ def workload_jackknife(args, L, ctx):
               "n_data": int(cats["data"]["x"].size), "n_randoms": int(cats["randoms"]["x"].size),
               "scale": args.scale, "n_jk": n_jk}
     return dict(jobs=jobs, upload_ms=upload_ms, config=config, traffic_key=(args.scale, n_jk),
-                cpu=lambda: cpu_baseline(cats, n_jk, args.cpu_frac))
+                cpu=lambda: cpu_baseline_jackknife(cats, n_jk, args.cpu_frac))
 
 
 MONO_AXES = [(0.0, 100.0, 30, 1.3)]
@@ -186,7 +276,38 @@ def workload_multi(args, L, ctx):
                           "in 20k-2M, density 3e-4, 5x randoms; monopole (30 log bins) + s-mu (40x50) "
                           "DD+DR+RR of every mock in one call",
               "n_data": nd, "n_randoms": nr, "scale": args.scale, "n_jobs": len(jobs)}
-    return dict(jobs=jobs, upload_ms=up_ms, config=config, traffic_key=None, cpu=None)
+
+    def cpu(frac=args.cpu_frac, seed=6):
+        """The reference (ref_counts: monopole then s-mu, DD, DR, RR) on every
+        8th mock by size, a seeded frac of each mock's objects."""
+        rng = np.random.Generator(np.random.PCG64(seed))
+        order = sorted(range(len(mocks)), key=lambda k: mocks[k]["data"].shape[0])
+        pick = order[::8]
+        secs = wall = 0.0
+        sample_jobs, nobj = [], 0
+        for k in pick:
+            cols, dc = {}, {}
+            for tag in ("data", "randoms"):
+                p = mocks[k][tag][subsample(mocks[k][tag].shape[0], frac, rng)]
+                nobj += p.shape[0]
+                cols[tag] = [p[:, 0], p[:, 1], p[:, 2]]
+                x, y, z = (np.ascontiguousarray(p[:, i]) for i in range(3))
+                _, nx, ny, nz = L.cart_norm(x, y, z)
+                dc[tag] = ctx.upload(x, y, z, nx, ny, nz)
+            with tempfile.TemporaryDirectory() as td:
+                _, s_, w_ = run_reference(ref_binary("ref_counts"),
+                                          dict(coord_system="cartesian", plot_monopole=1, plot_s_mu=1), cols, td)
+            secs += s_
+            wall += w_
+            for kind, axes in ((L.MONOPOLE, MONO_AXES), (L.S_MU, AXES)):
+                sample_jobs += [dict(kind=kind, a=dc["data"], b=None, axes=axes, n_jk=1),
+                                dict(kind=kind, a=dc["data"], b=dc["randoms"], axes=axes, n_jk=1),
+                                dict(kind=kind, a=dc["randoms"], b=None, axes=axes, n_jk=1)]
+        pairs = gpu_pairs(L, ctx, sample_jobs)
+        return baseline_record(pairs, secs, wall, f"{len(pick)} of the 64 mocks (every 8th by size), seeded "
+                               f"{frac:.0%} of their objects ({nobj} objects, {pairs:.3g} monopole + s-mu pairs)")
+
+    return dict(jobs=jobs, upload_ms=up_ms, config=config, traffic_key=None, cpu=cpu)
 
 
 def _eq_upload(L, ctx, a, c, masks=False):
@@ -216,7 +337,32 @@ def workload_equatorial(args, L, ctx):
     config = {"workload": "equatorial s-mu + sigma-pi: 1M clustered galaxies + 5M randoms, weights "
                           "U[0.5,1.5]; s-mu 40 log bins x 50 mu, sigma-pi 50x50 linear to 50; DD+DR+RR in one call",
               "n_data": int(c["data"]["ra"].size), "n_randoms": int(c["randoms"]["ra"].size), "scale": args.scale}
-    return dict(jobs=jobs, upload_ms=upload_ms, config=config, traffic_key=None, cpu=None)
+
+    def cpu(frac=args.cpu_frac, seed=7):
+        """The reference (ref_counts: sigma-pi then s-mu, weighted) on a seeded
+        frac of the objects."""
+        rng = np.random.Generator(np.random.PCG64(seed))
+        sub, cols = {}, {}
+        for tag in ("data", "randoms"):
+            a = c[tag]
+            k = subsample(a["ra"].size, frac, rng)
+            sub[tag] = {f: a[f][k] for f in ("ra", "dec", "z", "w", "jk")}
+            cols[tag] = [sub[tag]["ra"], sub[tag]["dec"], sub[tag]["z"], sub[tag]["w"]]
+        with tempfile.TemporaryDirectory() as td:
+            _, secs, wall = run_reference(ref_binary("ref_counts"),
+                                          dict(use_weights=1, plot_sigma_pi=1, plot_s_mu=1), cols, td)
+        sd = {t: _eq_upload(L, ctx, sub[t], c) for t in ("data", "randoms")}
+        sample_jobs = []
+        for kind, axes in ((L.S_MU, AXES), (L.SIGMA_PI, SP_AXES)):
+            sample_jobs += [dict(kind=kind, a=sd["data"], b=None, axes=axes, n_jk=1),
+                            dict(kind=kind, a=sd["data"], b=sd["randoms"], axes=axes, n_jk=1),
+                            dict(kind=kind, a=sd["randoms"], b=None, axes=axes, n_jk=1)]
+        pairs = gpu_pairs(L, ctx, sample_jobs)
+        return baseline_record(pairs, secs, wall, f"seeded {frac:.0%} subsample of data and randoms "
+                               f"({sub['data']['ra'].size} + {sub['randoms']['ra'].size} objects, {pairs:.3g} "
+                               f"s-mu + sigma-pi pairs, {frac * frac:.2%} of the pairs)")
+
+    return dict(jobs=jobs, upload_ms=upload_ms, config=config, traffic_key=None, cpu=cpu)
 
 
 ANG_AXIS = (0.0, float(1.0 - np.cos(0.175)), 40, 1.6)
@@ -237,20 +383,26 @@ def workload_invp(args, L, ctx):
     t_up = time.perf_counter()
     dc = {t: _eq_upload(L, ctx, c[t], c, masks=True) for t in ("data", "randoms")}
     upload_ms = 1000.0 * (time.perf_counter() - t_up)
-    D, R = dc["data"], dc["randoms"]
-    ang = [dict(kind=L.ANGULAR, a=D, b=None, axes=[ANG_AXIS], weight_mode=L.W_ANG_IW, n_runs=n_runs),
-           dict(kind=L.ANGULAR, a=D, b=None, axes=[ANG_AXIS], weight_mode=L.W_ANG_INVP, n_runs=n_runs),
-           dict(kind=L.ANGULAR, a=D, b=R, axes=[ANG_AXIS], weight_mode=L.W_ANG_IW, n_runs=n_runs),
-           dict(kind=L.ANGULAR, a=D, b=R, axes=[ANG_AXIS], weight_mode=L.W_ANG_W, n_runs=n_runs)]
+    def angular(D, R):
+        return [dict(kind=L.ANGULAR, a=D, b=None, axes=[ANG_AXIS], weight_mode=L.W_ANG_IW, n_runs=n_runs),
+                dict(kind=L.ANGULAR, a=D, b=None, axes=[ANG_AXIS], weight_mode=L.W_ANG_INVP, n_runs=n_runs),
+                dict(kind=L.ANGULAR, a=D, b=R, axes=[ANG_AXIS], weight_mode=L.W_ANG_IW, n_runs=n_runs),
+                dict(kind=L.ANGULAR, a=D, b=R, axes=[ANG_AXIS], weight_mode=L.W_ANG_W, n_runs=n_runs)]
 
-    def threed(up_dd, up_dr):
+    def threed_of(D, R, up_dd, up_dr):
         return [dict(kind=L.S_MU, a=D, b=None, axes=AXES, weight_mode=L.W_INVP_3D, n_runs=n_runs,
                      up_bins=up_dd, up_axis=ANG_AXIS),
                 dict(kind=L.S_MU, a=D, b=R, axes=AXES, weight_mode=L.W_ANGUP_3D, n_runs=n_runs,
                      up_bins=up_dr, up_axis=ANG_AXIS),
                 dict(kind=L.S_MU, a=R, b=None, axes=AXES)]
 
-    def run(method, tu):
+    D, R = dc["data"], dc["randoms"]
+    ang = angular(D, R)
+
+    def threed(up_dd, up_dr):
+        return threed_of(D, R, up_dd, up_dr)
+
+    def step(ang, threed, method, tu):
         r1, t1 = ctx.count(ang, method=method, tuning=tu)
         fin = [L.finish(j.get("b") is None, r["full"], r["jk"])[0] for r, j in zip(r1, ang)]
         with np.errstate(invalid="ignore", divide="ignore"):
@@ -262,13 +414,52 @@ def workload_invp(args, L, ctx):
             f"({t2['n_pairs']} pairs)")
         return r1 + r2, t
 
+    def run(method, tu):
+        return step(ang, threed, method, tu)
+
+    def cpu(frac=min(args.cpu_frac, 0.1), seed=8):
+        """The reference's inverse-p build (ref_invp32: _USE_INV_WEIGHTS,
+        _N_MASK_INTS=32, hdf5 catalogues with BITWEIGHT0..31) on a seeded frac
+        of the objects: calc_angular_upweight, then s-mu DD (inverse-p), DR
+        (up-weight) and RR."""
+        sys.path.insert(0, os.path.join(ROOT, "tests"))
+        import h5mini
+        rng = np.random.Generator(np.random.PCG64(seed))
+        sub = {}
+        with tempfile.TemporaryDirectory() as td:
+            files = {}
+            for tag in ("data", "randoms"):
+                a = c[tag]
+                k = subsample(a["ra"].size, frac, rng)
+                sub[tag] = {f: a[f][k] for f in ("ra", "dec", "z", "w", "iw", "jk", "masks")}
+                cols = dict(RA=sub[tag]["ra"], DEC=sub[tag]["dec"], Z=sub[tag]["z"],
+                            W=sub[tag]["iw"].astype(np.float64))
+                for w_ in range(sub[tag]["masks"].shape[1]):
+                    cols[f"BITWEIGHT{w_}"] = np.ascontiguousarray(sub[tag]["masks"][:, w_]).view(np.int64)
+                files[tag] = os.path.join(td, tag + ".h5")
+                h5mini.write(files[tag], cols)
+            _, secs, wall = run_reference(
+                ref_binary("ref_invp32"),
+                dict(data_file_type="hdf5", random_file_type="hdf5", ra_x_dataset_name="RA", dec_y_dataset_name="DEC",
+                     z_z_dataset_name="Z", weight_dataset_name="W", use_weights=1, n_bitwise_runs=n_runs,
+                     calculate_angular_dd=1, calculate_angular_dd_invpweights=1, calculate_angular_dr=1,
+                     calculate_angular_dr_invpweights=1, plot_s_mu=1),
+                files, td, extra_args=("none", "0"))
+        sd = {t: _eq_upload(L, ctx, sub[t], c, masks=True) for t in ("data", "randoms")}
+        _, t = step(angular(sd["data"], sd["randoms"]),
+                    lambda u, v: threed_of(sd["data"], sd["randoms"], u, v), L.FP32_RECHECK, None)
+        pairs = t["n_pairs"]
+        return baseline_record(pairs, secs, wall, f"seeded {frac:.0%} subsample of data and randoms "
+                               f"({sub['data']['ra'].size} + {sub['randoms']['ra'].size} objects, {pairs:.3g} "
+                               f"angular + s-mu pairs, {frac * frac:.2%} of the pairs)")
+
     config = {"workload": "bitwise inverse-p: 500k clustered galaxies with 2048-bit masks + 2.5M randoms; "
                           "angular DD/DR (parent and inverse-p weights, theta<0.175, 40 log bins) then s-mu "
                           "DD (inverse-p) / DR (up-weight) / RR, as two calls per step",
               "n_data": int(c["data"]["ra"].size), "n_randoms": int(c["randoms"]["ra"].size), "scale": args.scale,
               "n_runs": n_runs}
     all_jobs = ang + threed(None, None)
-    return dict(jobs=ang, all_jobs=all_jobs, upload_ms=upload_ms, config=config, traffic_key=None, cpu=None,
+    return dict(jobs=ang, all_jobs=all_jobs, upload_ms=upload_ms, config=config, traffic_key=None, cpu=cpu,
                 run=run, tunable=False)
 
 

@@ -19,7 +19,9 @@
 //     before normalise(), then the normalised ones.
 //
 // With "--main" instead of an output file it runs main.cpp:49-62's sequence
-// and writes the output files the parameter file names.
+// and writes the output files the parameter file names.  A probe file of
+// "none" evaluates no probes; a fourth argument 0 leaves the catalogues out
+// of the JSON (timing runs).
 //
 // TEST INFRASTRUCTURE ONLY.  Used in this container to write golden fixtures
 // (tests/golden/) and, on the GPU host, as the reference CPU baseline of the
@@ -102,7 +104,7 @@ static JKHists *read_angular(bool asked, const std::string &fn) {
 
 int main(int argc, char **argv) {
 	if (argc < 3) {
-		fprintf(stderr, "usage: ref_invp_counts <param-file> <out.json | --main> [probe-file]\n");
+		fprintf(stderr, "usage: ref_invp_counts <param-file> <out.json | --main> [probe-file|none] [dump_catalog=1|0]\n");
 		return 1;
 	}
 	Params params(argv[1]);
@@ -130,8 +132,10 @@ int main(int argc, char **argv) {
 	fprintf(f, "{\"n_threads\": %d, \"n_data\": %d, \"n_randoms\": %d, \"n_mask_ints\": %d",
 		params.n_threads, data->n_gals, randoms->n_gals, (int)_N_MASK_INTS);
 	// catalogues as read, before GalaxyPixels / GalaxyBoxes reorder them
-	put_cat(f, "data", data);
-	put_cat(f, "randoms", randoms);
+	if (argc <= 4 || std::atoi(argv[4]) != 0) {
+		put_cat(f, "data", data);
+		put_cat(f, "randoms", randoms);
+	}
 
 	double t0 = now_s();
 	calc_angular_upweight(data, randoms, params);
@@ -154,7 +158,7 @@ int main(int argc, char **argv) {
 
 	// up-weights and their interpolation (as the 3-D correlators form them)
 	std::vector<double> probes;
-	if (argc > 3) {
+	if (argc > 3 && std::strcmp(argv[3], "none") != 0) {
 		FILE *pf = fopen(argv[3], "r");
 		char line[128];
 		while (pf && fgets(line, sizeof line, pf)) probes.push_back(strtod(line, NULL));

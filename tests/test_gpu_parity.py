@@ -100,8 +100,9 @@ def test_jackknife_batch_vs_oracle(ctx, oracle_lib, method, slots):
 
 
 def test_methods_agree_weighted_sigma_pi(ctx, oracle_lib):
-    """Weighted sigma-pi and s-mu (weights U[0.5,1.5]): both GPU methods equal
-    the oracle bit for bit (partial sums are exactly representable)."""
+    """Weighted sigma-pi, s-mu and monopole (weights U[0.5,1.5]), DD, DR and
+    RR: both GPU methods equal the oracle bit for bit (partial sums are
+    exactly representable)."""
     from two_pcf_amd import lib as L
     from two_pcf_amd import synth
     o = oracle_lib
@@ -118,11 +119,15 @@ def test_methods_agree_weighted_sigma_pi(ctx, oracle_lib):
                        (0, [(0.0, 100.0, 30, 1.3)])):
         ax1 = o.axis(*axes[0])
         ax2 = o.axis(*axes[1]) if len(axes) > 1 else None
-        want = o.count(kind, oc["data"], oc["randoms"], ax1, ax2, 1)
+        spec = (("data", None), ("data", "randoms"), ("randoms", None))
+        want = [o.count(kind, oc[a], oc[b] if b else None, ax1, ax2, 1) for a, b in spec]
         for method in (0, 1):
             print(f"[gpu] weighted kind={kind} method={method}", flush=True)
-            res, _ = ctx.count([dict(kind=kind, a=dc["data"], b=dc["randoms"], axes=axes, n_jk=1)], method=method)
-            np.testing.assert_array_equal(res[0]["full"], want[0])
+            res, _ = ctx.count([dict(kind=kind, a=dc[a], b=dc[b] if b else None, axes=axes, n_jk=1) for a, b in spec],
+                               method=method)
+            for r, w, (a, b) in zip(res, want, spec):
+                np.testing.assert_array_equal(r["full"], w[0], err_msg=f"kind {kind} {a}-{b or a}")
+                assert r["n_pairs"] == w[2]
     for c_ in dc.values():
         c_.free()
 
