@@ -3,9 +3,10 @@
 //
 // Layout on the device (per catalogue and cell geometry):
 //   sorted SoA   double x,y,z,nx,ny,nz; float w, iw; int jk   (cell-major,
-//                sorted along z within a cell: hipCUB radix sort on the key
-//                cell << ZQ_BITS | z quantised within the cell, so that a
-//                row of cells along z is sorted along z as a whole)
+//                sorted along z within a cell: a counting sort by cell, then
+//                by 2^ZB_BITS z buckets of the cell and catalogue index, so
+//                that a row of cells along z is sorted along z as a whole up
+//                to ties within one bucket)
 //   cell_start   int[n_cell + 1]
 // Work: a work item is (job, up to 256 first-side objects of one cell, a set
 // of neighbour-cell rows, up to S region slots).  A 1024-thread block (16
@@ -15,7 +16,6 @@
 // is shared by all 16 waves.  Per-item histograms live in LDS (u32 counts or
 // f64 sums) and are flushed once per item with global atomics.
 #include <hip/hip_runtime.h>
-#include <hipcub/hipcub.hpp>
 
 #include <algorithm>
 #include <chrono>
@@ -181,14 +181,21 @@ struct Arena {
 
 // -------------------------------------------------------------- kernels --
 
-// Sort key: the cell, then the position along z within the cell in 2^ZQ_BITS
-// steps (monotone in z, so objects of a row of cells along z come out sorted
-// along z up to ties within one step of h 2^-ZQ_BITS).
-constexpr int ZQ_BITS = 16;
+// Gridding: a counting sort by cell, then by z within each cell.  Objects
+// of a cell are ordered by ZB_BITS z buckets of the cell's height and, inside
+// a bucket, by their index in the catalogue; a row of cells along z is then
+// sorted along z up to ties within one bucket (h 2^-ZB_BITS).
+constexpr int ZB_BITS = 10;
+constexpr int ZB = 1 << ZB_BITS;
+constexpr int SORT_BLOCK = 1024;            // threads of the scan and per-cell kernels (one per z bucket)
+static_assert(SORT_BLOCK == ZB, "one thread per z bucket");
+static_assert(ZB_BITS >= 9, "the counting kernel's z window slack, h 2^-8, covers two z buckets");
 
-__global__ void k_cell_keys(const double *__restrict__ x, const double *__restrict__ y,
-                            const double *__restrict__ z, int64_t n, double lo0, double lo1, double lo2, double h,
-                            int d0, int d1, int d2, unsigned long long *__restrict__ keys, int *__restrict__ vals) {
+// cell and z bucket of every object; count[cell] += 1
+__global__ void k_cell_bucket(const double *__restrict__ x, const double *__restrict__ y,
+                              const double *__restrict__ z, int64_t n, double lo0, double lo1, double lo2, double h,
+                              int d0, int d1, int d2, int *__restrict__ cell, unsigned short *__restrict__ zb,
+                              unsigned *__restrict__ count) {
 	int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
 	if (i >= n) return;
 	const double fz = (z[i] - lo2) / h;
@@ -196,11 +203,123 @@ __global__ void k_cell_keys(const double *__restrict__ x, const double *__restri
 	c0 = min(max(c0, 0), d0 - 1);
 	c1 = min(max(c1, 0), d1 - 1);
 	c2 = min(max(c2, 0), d2 - 1);
-	const double q = floor((fz - c2) * (double)(1 << ZQ_BITS));
-	const unsigned long long zq = (unsigned long long)min(max(q, 0.0), (double)((1 << ZQ_BITS) - 1));
-	const unsigned long long cell = ((unsigned long long)c0 * d1 + c1) * d2 + c2;
-	keys[i] = (cell << ZQ_BITS) | zq;
-	vals[i] = (int)i;
+	const double q = floor((fz - c2) * (double)ZB);
+	const int c = (c0 * d1 + c1) * d2 + c2;
+	cell[i] = c;
+	zb[i] = (unsigned short)min(max(q, 0.0), (double)(ZB - 1));
+	atomicAdd(&count[c], 1u);
+}
+
+// Exclusive scan of v over a block of SORT_BLOCK threads (16 waves); *total
+// gets the block's sum.  wsum: 16 LDS words.
+__device__ unsigned block_exclusive_scan(unsigned v, unsigned *wsum, unsigned *total) {
+	constexpr int NW = SORT_BLOCK / WAVE;
+	const int lane = threadIdx.x & (WAVE - 1), w = threadIdx.x / WAVE;
+	unsigned x = v;
+	for (int d = 1; d < WAVE; d <<= 1) {
+		const unsigned y = __shfl_up(x, d);
+		if (lane >= d) x += y;
+	}
+	if (lane == WAVE - 1) wsum[w] = x;
+	__syncthreads();
+	if (w == 0) {
+		unsigned t = lane < NW ? wsum[lane] : 0u;
+		for (int d = 1; d < NW; d <<= 1) {
+			const unsigned y = __shfl_up(t, d);
+			if (lane >= d) t += y;
+		}
+		if (lane < NW) wsum[lane] = t;
+	}
+	__syncthreads();
+	const unsigned before = w ? wsum[w - 1] : 0u;
+	*total = wsum[NW - 1];
+	__syncthreads();
+	return before + x - v;
+}
+
+// scan in tiles of SORT_BLOCK: out = exclusive prefix within the tile,
+// sums[tile] = the tile's total
+__global__ void __launch_bounds__(SORT_BLOCK) k_scan_tiles(const unsigned *__restrict__ in, unsigned *__restrict__ out,
+                                                          int64_t n, unsigned *__restrict__ sums) {
+	__shared__ unsigned wsum[SORT_BLOCK / WAVE];
+	const int64_t i = (int64_t)blockIdx.x * SORT_BLOCK + threadIdx.x;
+	unsigned total;
+	const unsigned ex = block_exclusive_scan(i < n ? in[i] : 0u, wsum, &total);
+	if (i < n) out[i] = ex;
+	if (threadIdx.x == 0) sums[blockIdx.x] = total;
+}
+
+// exclusive scan of the tile sums in place (one block, carried across chunks)
+__global__ void __launch_bounds__(SORT_BLOCK) k_scan_sums(unsigned *__restrict__ sums, int n_tiles) {
+	__shared__ unsigned wsum[SORT_BLOCK / WAVE];
+	unsigned carry = 0;
+	for (int b = 0; b < n_tiles; b += SORT_BLOCK) {
+		const int k = b + (int)threadIdx.x;
+		unsigned total;
+		const unsigned ex = block_exclusive_scan(k < n_tiles ? sums[k] : 0u, wsum, &total);
+		if (k < n_tiles) sums[k] = carry + ex;
+		carry += total;
+	}
+}
+
+__global__ void __launch_bounds__(SORT_BLOCK) k_scan_add(unsigned *__restrict__ out, int64_t n,
+                                                        const unsigned *__restrict__ sums) {
+	const int64_t i = (int64_t)blockIdx.x * SORT_BLOCK + threadIdx.x;
+	if (i < n) out[i] += sums[blockIdx.x];
+}
+
+// objects to their cell's segment (in any order inside it)
+__global__ void k_scatter_cells(const int *__restrict__ cell, const unsigned short *__restrict__ zb, int64_t n,
+                                const unsigned *__restrict__ start, unsigned *__restrict__ cursor,
+                                int *__restrict__ tperm, unsigned short *__restrict__ tzb) {
+	int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+	if (i >= n) return;
+	const int c = cell[i];
+	const unsigned pos = start[c] + atomicAdd(&cursor[c], 1u);
+	tperm[pos] = (int)i;
+	tzb[pos] = zb[i];
+}
+
+// Each cell's segment by z bucket (a counting sort in LDS, one thread per
+// bucket), then each bucket by catalogue index (an insertion sort by the
+// bucket's thread), so the order is deterministic.  Blocks stride over the
+// cells; every loop bound is uniform over the block.
+__global__ void __launch_bounds__(SORT_BLOCK) k_zsort_cells(const unsigned *__restrict__ start, int n_cell,
+                                                           const int *__restrict__ tperm,
+                                                           const unsigned short *__restrict__ tzb,
+                                                           int *__restrict__ perm) {
+	__shared__ unsigned hist[ZB];
+	__shared__ unsigned wsum[SORT_BLOCK / WAVE];
+	const int tid = threadIdx.x;
+	for (int c = blockIdx.x; c < n_cell; c += gridDim.x) {
+		const unsigned b = start[c], e = start[c + 1];
+		if (e - b <= 1u) {
+			if (tid == 0 && e > b) perm[b] = tperm[b];
+			continue;
+		}
+		hist[tid] = 0u;
+		__syncthreads();
+		for (unsigned k = b + tid; k < e; k += SORT_BLOCK) atomicAdd(&hist[tzb[k]], 1u);
+		__syncthreads();
+		const unsigned cnt = hist[tid];
+		unsigned total;
+		const unsigned off = block_exclusive_scan(cnt, wsum, &total);
+		hist[tid] = off;
+		__syncthreads();
+		for (unsigned k = b + tid; k < e; k += SORT_BLOCK) perm[b + atomicAdd(&hist[tzb[k]], 1u)] = tperm[k];
+		__syncthreads();   // the block's global writes are visible to the block
+		const unsigned lo = b + off, hi = lo + cnt;
+		for (unsigned i = lo + 1; i < hi; ++i) {
+			const int key = perm[i];
+			unsigned j = i;
+			while (j > lo && perm[j - 1] > key) {
+				perm[j] = perm[j - 1];
+				--j;
+			}
+			perm[j] = key;
+		}
+		__syncthreads();
+	}
 }
 
 __global__ void k_gather(const int *__restrict__ perm, int64_t n, const double *x, const double *y, const double *z,
@@ -222,19 +341,9 @@ __global__ void k_gather(const int *__restrict__ perm, int64_t n, const double *
 	for (int k = 0; k < n_words; ++k) omasks[i * n_words + k] = masks[(int64_t)s * n_words + k];
 }
 
-// start[c] = first sorted index whose key >= c << ZQ_BITS.
-__global__ void k_cell_start(const unsigned long long *__restrict__ keys, int64_t n, int n_cell,
-                             int *__restrict__ start) {
-	int c = blockIdx.x * blockDim.x + threadIdx.x;
-	if (c > n_cell) return;
-	unsigned long long target = (unsigned long long)c << ZQ_BITS;
-	int64_t lo = 0, hi = n;
-	while (lo < hi) {
-		int64_t mid = (lo + hi) >> 1;
-		if (keys[mid] < target) lo = mid + 1;
-		else hi = mid;
-	}
-	start[c] = (int)lo;
+__global__ void k_fill_u32(unsigned *p, int64_t n, unsigned v) {
+	int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+	if (i < n) p[i] = v;
 }
 
 __global__ void k_fill_u64(unsigned long long *p, int64_t n, unsigned long long v) {
@@ -1093,8 +1202,9 @@ __device__ void process_item_fast(const JobDev &J, const WorkItem &W, const int4
 	const unsigned lane_code = ((unsigned)lrow << 6) | (unsigned)(4 * lk);
 	const int lane_d = lrow - 4 * lk;   // (column - row) within the block, for the order test
 	const float *tff = (const float *)tfa;
-	// z window slack: float offsets, sort ties within one key step (h 2^-16)
-	const float zpad = (float)(J.h * 0x1p-12);
+	// z window slack: float offsets, and sort ties within one z bucket of the
+	// gridding (h 2^-ZB_BITS) on both sides of a block's first and last entry
+	const float zpad = (float)(J.h * 0x1p-8);
 
 	for (int r = 0; r < J.n_rows; ++r) {
 		if (!((W.row_mask >> r) & 1ull)) continue;
@@ -1355,6 +1465,7 @@ struct Plan {
 	int n_jobs = 0, n_tables = 0, n_items = 0;
 	double prep_ms = 0;
 	std::vector<PlanLaunch> launches;
+	bool concurrent = false;   // launches forked over the context's streams (batched method)
 	std::vector<PlanOut> outs;
 	hipGraph_t graph = nullptr;
 	hipGraphExec_t exec = nullptr;
@@ -1371,10 +1482,18 @@ struct Plan {
 
 }  // namespace
 
+// side streams of a context: with the context stream, as many hardware queues
+// as a process gets by default
+constexpr int N_SIDE = 3;
+
 struct twopcf_ctx {
 	int device = 0;
 	hipStream_t stream = nullptr;
 	hipEvent_t ev0 = nullptr, ev1 = nullptr;
+	// the batched method's launch classes run concurrently: forked from the
+	// context stream onto the side streams and joined back
+	hipStream_t side[N_SIDE] = {};
+	hipEvent_t fork = nullptr, join[N_SIDE] = {};
 	std::string err;
 	int n_cu = 0;
 	size_t lds_max = 65536;
@@ -1466,53 +1585,67 @@ int grid_catalog(twopcf_ctx *ctx, twopcf_cat *c, const GridKey &g, SortedCat **o
 	SortedCat s;
 	const int64_t n = c->n;
 	const int n_cell = g.dim[0] * g.dim[1] * g.dim[2];
-	unsigned long long *keys = nullptr, *keys_s = nullptr;
-	int *vals = nullptr, *perm = nullptr;
-	void *tmp = nullptr;
-	size_t tmp_bytes = 0;
-	int rc = 0;
-	auto cleanup = [&]() {
-		for (void *p : {(void *)keys, (void *)keys_s, (void *)vals, (void *)perm, tmp})
-			if (p) (void)hipFree(p);
+	// temporaries of the sort, freed on every return
+	struct Temps {
+		std::vector<void *> p;
+		~Temps() {
+			for (void *q : p) (void)hipFree(q);
+		}
+	} tmp;
+	auto talloc = [&](auto **q, size_t count) -> int {
+		int rc_ = dalloc(ctx, q, count);
+		if (!rc_) tmp.p.push_back((void *)*q);
+		return rc_;
 	};
-	if ((rc = dalloc(ctx, &keys, n)) || (rc = dalloc(ctx, &keys_s, n)) || (rc = dalloc(ctx, &vals, n)) ||
-	    (rc = dalloc(ctx, &perm, n))) {
-		cleanup();
+	int *cell = nullptr, *tperm = nullptr, *perm = nullptr;
+	unsigned short *zb = nullptr, *tzb = nullptr;
+	unsigned *count = nullptr, *sums = nullptr;
+	const int64_t n_scan = (int64_t)n_cell + 1;
+	const int n_tiles = (int)((n_scan + SORT_BLOCK - 1) / SORT_BLOCK);
+	int rc = 0;
+	if ((rc = talloc(&cell, n)) || (rc = talloc(&zb, n)) || (rc = talloc(&tperm, n)) || (rc = talloc(&tzb, n)) ||
+	    (rc = talloc(&perm, n)) || (rc = talloc(&count, n_scan)) || (rc = talloc(&sums, n_tiles)))
 		return rc;
-	}
-	const unsigned long long max_key = (unsigned long long)n_cell << ZQ_BITS;
-	int end_bit = 1;
-	while (end_bit < 64 && (1ull << end_bit) <= max_key) ++end_bit;
-	const int nblk = (int)((n + 255) / 256);
-	if (n > 0) {
-		k_cell_keys<<<nblk, 256, 0, ctx->stream>>>(g.on_norm ? c->nx : c->x, g.on_norm ? c->ny : c->y,
-		                                           g.on_norm ? c->nz : c->z, n, g.lo[0], g.lo[1], g.lo[2], g.h,
-		                                           g.dim[0], g.dim[1], g.dim[2], keys, vals);
-		HIPCK(ctx, hipGetLastError());
-		HIPCK(ctx, hipcub::DeviceRadixSort::SortPairs(nullptr, tmp_bytes, keys, keys_s, vals, perm, (int)n, 0,
-		                                               end_bit, ctx->stream));
-		HIPCK(ctx, hipMalloc(&tmp, std::max<size_t>(tmp_bytes, 1)));
-		HIPCK(ctx, hipcub::DeviceRadixSort::SortPairs(tmp, tmp_bytes, keys, keys_s, vals, perm, (int)n, 0, end_bit,
-		                                               ctx->stream));
-	}
 	if ((rc = dalloc(ctx, &s.x, n)) || (rc = dalloc(ctx, &s.y, n)) || (rc = dalloc(ctx, &s.z, n)) ||
 	    (rc = dalloc(ctx, &s.nx, n)) || (rc = dalloc(ctx, &s.ny, n)) || (rc = dalloc(ctx, &s.nz, n)) ||
 	    (rc = dalloc(ctx, &s.w, n)) || (rc = dalloc(ctx, &s.iw, n)) || (rc = dalloc(ctx, &s.jk, n)) ||
 	    (rc = dalloc(ctx, &s.masks, (size_t)n * c->m_stride)) ||
 	    (rc = dalloc(ctx, &s.start, (size_t)n_cell + 1))) {
-		cleanup();
 		s.release();
 		return rc;
 	}
 	s.m_stride = c->m_stride;
+	// counting sort by cell (count, scan into the cell starts, scatter), then
+	// by z bucket and catalogue index within each cell
+	unsigned *start = (unsigned *)s.start;
+	const int nblk = (int)((n + 255) / 256);
+	auto check = [&]() -> int {
+		hipError_t e = hipGetLastError();
+		if (e != hipSuccess) {
+			s.release();
+			return fail(ctx, TWOPCF_E_HIP, std::string("gridding: ") + hipGetErrorString(e));
+		}
+		return 0;
+	};
+	k_fill_u32<<<(unsigned)((n_scan + 255) / 256), 256, 0, ctx->stream>>>(count, n_scan, 0u);
+	if (n > 0)
+		k_cell_bucket<<<nblk, 256, 0, ctx->stream>>>(g.on_norm ? c->nx : c->x, g.on_norm ? c->ny : c->y,
+		                                             g.on_norm ? c->nz : c->z, n, g.lo[0], g.lo[1], g.lo[2], g.h,
+		                                             g.dim[0], g.dim[1], g.dim[2], cell, zb, count);
+	k_scan_tiles<<<n_tiles, SORT_BLOCK, 0, ctx->stream>>>(count, start, n_scan, sums);
+	k_scan_sums<<<1, SORT_BLOCK, 0, ctx->stream>>>(sums, n_tiles);
+	k_scan_add<<<n_tiles, SORT_BLOCK, 0, ctx->stream>>>(start, n_scan, sums);
+	if ((rc = check())) return rc;
 	if (n > 0) {
+		k_fill_u32<<<(unsigned)((n_scan + 255) / 256), 256, 0, ctx->stream>>>(count, n_scan, 0u);   // cursors
+		k_scatter_cells<<<nblk, 256, 0, ctx->stream>>>(cell, zb, n, start, count, tperm, tzb);
+		k_zsort_cells<<<std::max(1, std::min(n_cell, ctx->n_cu * 8)), SORT_BLOCK, 0, ctx->stream>>>(start, n_cell, tperm,
+		                                                                                           tzb, perm);
 		k_gather<<<nblk, 256, 0, ctx->stream>>>(perm, n, c->x, c->y, c->z, c->nx, c->ny, c->nz, c->w, c->iw, c->jk,
 		                                        c->masks, c->m_stride, s.x, s.y, s.z, s.nx, s.ny, s.nz, s.w, s.iw,
 		                                        s.jk, s.masks);
-		HIPCK(ctx, hipGetLastError());
+		if ((rc = check())) return rc;
 	}
-	k_cell_start<<<(n_cell + 1 + 255) / 256, 256, 0, ctx->stream>>>(keys_s, n, n_cell, s.start);
-	HIPCK(ctx, hipGetLastError());
 	s.start_h.resize((size_t)n_cell + 1);
 	HIPCK(ctx, hipMemcpyAsync(s.start_h.data(), s.start, ((size_t)n_cell + 1) * sizeof(int), hipMemcpyDeviceToHost,
 	                          ctx->stream));
@@ -1522,7 +1655,6 @@ int grid_catalog(twopcf_ctx *ctx, twopcf_cat *c, const GridKey &g, SortedCat **o
 		HIPCK(ctx, hipMemcpyAsync(jk_h.data(), s.jk, n * sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
 	}
 	HIPCK(ctx, hipStreamSynchronize(ctx->stream));
-	cleanup();
 	if (g.n_region > 1) {
 		s.reg_off.assign((size_t)n_cell + 1, 0);
 		std::vector<int> seen((size_t)g.n_region, -1);   // last cell that listed the region
@@ -1607,10 +1739,15 @@ int twopcf_ctx_create(int device, twopcf_ctx **out) {
 	auto *ctx = new twopcf_ctx();
 	ctx->device = device;
 	hipDeviceProp_t prop;
-	if (hipSetDevice(device) != hipSuccess || hipGetDeviceProperties(&prop, device) != hipSuccess ||
-	    hipStreamCreate(&ctx->stream) != hipSuccess || hipEventCreate(&ctx->ev0) != hipSuccess ||
-	    hipEventCreate(&ctx->ev1) != hipSuccess) {
-		delete ctx;
+	bool ok = hipSetDevice(device) == hipSuccess && hipGetDeviceProperties(&prop, device) == hipSuccess &&
+	          hipStreamCreate(&ctx->stream) == hipSuccess && hipEventCreate(&ctx->ev0) == hipSuccess &&
+	          hipEventCreate(&ctx->ev1) == hipSuccess &&
+	          hipEventCreateWithFlags(&ctx->fork, hipEventDisableTiming) == hipSuccess;
+	for (int k = 0; k < N_SIDE && ok; ++k)
+		ok = hipStreamCreate(&ctx->side[k]) == hipSuccess &&
+		     hipEventCreateWithFlags(&ctx->join[k], hipEventDisableTiming) == hipSuccess;
+	if (!ok) {
+		twopcf_ctx_destroy(ctx);
 		return TWOPCF_E_HIP;
 	}
 	ctx->n_cu = prop.multiProcessorCount;
@@ -1618,10 +1755,7 @@ int twopcf_ctx_create(int device, twopcf_ctx **out) {
 	ctx->name = prop.gcnArchName;
 	// the no-allocation entry point's scratch on this device (device current)
 	if (resolve_scratch(device) != hipSuccess) {
-		(void)hipEventDestroy(ctx->ev0);
-		(void)hipEventDestroy(ctx->ev1);
-		(void)hipStreamDestroy(ctx->stream);
-		delete ctx;
+		twopcf_ctx_destroy(ctx);
 		return TWOPCF_E_HIP;
 	}
 	*out = ctx;
@@ -1635,6 +1769,11 @@ int twopcf_ctx_destroy(twopcf_ctx *ctx) {
 	while (!ctx->cats.empty()) twopcf_catalog_free(ctx->cats.back());
 	if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
 	if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
+	if (ctx->fork) (void)hipEventDestroy(ctx->fork);
+	for (int k = 0; k < N_SIDE; ++k) {
+		if (ctx->join[k]) (void)hipEventDestroy(ctx->join[k]);
+		if (ctx->side[k]) (void)hipStreamDestroy(ctx->side[k]);
+	}
 	if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
 	delete ctx;
 	return 0;
@@ -2535,7 +2674,9 @@ int build_plan(twopcf_ctx *ctx, const twopcf_job *jobs, int n_jobs, int method, 
 	} else {
 		// batched: one persistent launch per (kind, weighted, pi definition,
 		// fused monopole, LDS mask copy) class over the cost-sorted queue of
-		// all jobs in that class
+		// all jobs in that class; the classes run concurrently unless
+		// TWOPCF_SERIAL_CLASSES is set
+		plan.concurrent = getenv("TWOPCF_SERIAL_CLASSES") == nullptr;
 		auto mid_of = [&](int q) {
 			const int k = hj[q].j->kind;
 			return (k == TWOPCF_SIGMA_PI || k == TWOPCF_S_MU) && hj[q].j->pi_def == TWOPCF_PI_MIDPOINT ? 1 : 0;
@@ -2622,6 +2763,12 @@ std::string job_set_key(const twopcf_job *jobs, int n_jobs, int method, std::vec
 
 // Enqueue one execution of a plan on the context stream: clear the outputs
 // with fill kernels (a graph may hold them; no memset nodes) and launch.
+// The batched method's launch classes write disjoint outputs and pull from
+// their own queues, so they run concurrently: launch k goes to the context
+// stream (k = 0) or side stream (k - 1) mod N_SIDE, each side stream waits
+// for the fills, and the context stream waits for every side stream; the
+// tail of one persistent launch then overlaps the next class.  Captured into
+// a graph, the fork and join become its dependencies.
 int enqueue_plan(twopcf_ctx *ctx, Plan &plan) {
 	auto fill = [&](unsigned long long *p, size_t n) {
 		if (n) k_fill_u64<<<(unsigned)((n + 255) / 256), 256, 0, ctx->stream>>>(p, (int64_t)n, 0ull);
@@ -2631,10 +2778,20 @@ int enqueue_plan(twopcf_ctx *ctx, Plan &plan) {
 	fill(plan.d_cnt, 2 * (size_t)plan.n_jobs);
 	fill((unsigned long long *)plan.d_queue, 32);
 	HIPCK(ctx, hipGetLastError());
-	for (auto &l : plan.launches) {
-		hipLaunchKernelGGL(l.fn, dim3(l.grid), dim3(BLOCK), l.L.total, ctx->stream, plan.d_jobs, l.items, l.n_items,
-		                   l.queue, plan.d_rows, plan.d_slots, plan.d_tab, plan.n_tables, l.L);
+	const int n = (int)plan.launches.size();
+	const int n_side = plan.concurrent ? std::min(N_SIDE, n - 1) : 0;
+	if (n_side > 0) HIPCK(ctx, hipEventRecord(ctx->fork, ctx->stream));
+	for (int k = 0; k < n_side; ++k) HIPCK(ctx, hipStreamWaitEvent(ctx->side[k], ctx->fork, 0));
+	for (int k = 0; k < n; ++k) {
+		const PlanLaunch &l = plan.launches[k];
+		hipStream_t s = (n_side > 0 && k > 0) ? ctx->side[(k - 1) % n_side] : ctx->stream;
+		hipLaunchKernelGGL(l.fn, dim3(l.grid), dim3(BLOCK), l.L.total, s, plan.d_jobs, l.items, l.n_items, l.queue,
+		                   plan.d_rows, plan.d_slots, plan.d_tab, plan.n_tables, l.L);
 		HIPCK(ctx, hipGetLastError());
+	}
+	for (int k = 0; k < n_side; ++k) {
+		HIPCK(ctx, hipEventRecord(ctx->join[k], ctx->side[k]));
+		HIPCK(ctx, hipStreamWaitEvent(ctx->stream, ctx->join[k], 0));
 	}
 	return 0;
 }
