@@ -1121,7 +1121,11 @@ __device__ void process_item_fast(const JobDev &J, const WorkItem &W, const int4
 		}
 	};
 	const std::integral_constant<bool, MID> mid;
+	// Binning raises the wave's priority: a wave whose pairs' operands are in
+	// flight is issued ahead of the waves that screen, which shortens the
+	// dependent LDS chains of binning.
 	auto consume = [&](unsigned head, unsigned n) {
+		__builtin_amdgcn_s_setprio(2);
 		__builtin_amdgcn_wave_barrier();
 		if constexpr (MID) {
 			// the midpoint's longer per-pair code: one pair per lane at a time
@@ -1181,6 +1185,7 @@ __device__ void process_item_fast(const JobDev &J, const WorkItem &W, const int4
 			for (int k = 0; k < NP; ++k) push(lane + k * WAVE < (int)n && !p[k].sure, p[k]);
 		}
 		__builtin_amdgcn_wave_barrier();
+		__builtin_amdgcn_s_setprio(0);
 		if (rq_n >= WAVE) flush_rq();
 	};
 

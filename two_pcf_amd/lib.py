@@ -8,7 +8,8 @@ import os
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "libtwopcf_hip.so")
+# the in-tree build, or another build of the same ABI named by $TWOPCF_LIB
+LIB_PATH = os.environ.get("TWOPCF_LIB") or os.path.join(HERE, "libtwopcf_hip.so")
 
 MONOPOLE, SIGMA_PI, S_MU, ANGULAR = 0, 1, 2, 3
 PLAIN_FP64, FP32_RECHECK = 0, 1
