@@ -37,11 +37,14 @@ FLOP_PER_PAIR = 26            # FP32 flop per accepted s-mu pair
 # log index 3, weight 1)
 FLOP_BY_KIND = {0: 15, 1: 29, 2: 26, 3: 15}
 # HBM bytes per launch of the counting kernel on the default workload (scale
-# 1, 100 regions) under the tuning the autotuner picks for it, from two
+# 1, 100 regions, tuning (2, 12, 256) with 9 region slots fitting), from two
 # rocprofv3 --pmc runs: FETCH_SIZE x 2 + WRITE_SIZE x 1 (factors from a 1 GiB
-# copy), KiB -> B.  Source: profiles/k_count_fp32_jackknife_batch_hbm.md
-TRAFFIC_BYTES = {(1.0, 100, 2, 8, 256): (7.797e7 * 2 + 3.212e6) * 1024,
-                 (1.0, 100, 2, 9, 256): (7.897e7 * 2 + 2.532e6) * 1024}
+# copy), KiB -> B.  Keyed by the first 16 hex digits of the SHA-256 of the
+# kernel source they were measured on: for any other source the line reports
+# no traffic.  Source: profiles/k_count_fp32_jackknife_batch_hbm.md
+KERNEL_SRC = os.path.join(ROOT, "two_pcf_amd", "csrc", "twopcf_hip.hip")
+TRAFFIC_BYTES = {("6fd9a79add253f89", 1.0, 100, 2, 9, 256): (7.902e7 * 2 + 2.532e6) * 1024}
+ALGO_BYTES_PER_PAIR = 0.5     # algorithmic HBM bytes per accepted pair (upper bound)
 # region slots that fit in LDS on the default workload (2000-bin u32
 # histograms next to the 1024-entry tile, queues and tables)
 SLOTS_FIT = 9
@@ -571,9 +574,12 @@ def main():
                                                    max_slots=tuning.get("max_slots", 8), i_chunk=256)
     # region slots in use: the LDS holds at most SLOTS_FIT per item on the
     # default workload, so larger settings run the same plan
-    traffic_key = (W["traffic_key"] + (tu_used["cell_subdiv"], min(tu_used["max_slots"], SLOTS_FIT),
-                                       tu_used["i_chunk"])
+    import hashlib
+    src_sha = hashlib.sha256(open(KERNEL_SRC, "rb").read()).hexdigest()[:16] if os.path.exists(KERNEL_SRC) else None
+    traffic_key = ((src_sha,) + W["traffic_key"] + (tu_used["cell_subdiv"], min(tu_used["max_slots"], SLOTS_FIT),
+                                                   tu_used["i_chunk"])
                    if W["traffic_key"] else None)
+    traffic = TRAFFIC_BYTES.get(traffic_key)
     config = dict(W["config"])
     config.update({"parallelism": "single GPU", "tuning": tuning or (tuned if tuned else "default")})
     line = {
@@ -594,7 +600,12 @@ def main():
         "speedup_vs_plain_fp64": (plain["kernel_ms"] / float(np.mean(kms))) if plain else None,
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / PEAK_FP32_TFLOPS,
-                     "traffic": TRAFFIC_BYTES.get(traffic_key),
+                     "traffic": traffic,
+                     "traffic_source": (f"rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE of this kernel source (sha256 "
+                                        f"{src_sha}), profiles/k_count_fp32_jackknife_batch_hbm.md" if traffic else
+                                        f"not measured for this kernel source (sha256 {src_sha}) and workload"),
+                     "traffic_bytes_per_pair": traffic / pairs if traffic else None,
+                     "algorithmic_bytes_per_pair": ALGO_BYTES_PER_PAIR,
                      "note": "FP32 compute bound (pair screening on the matrix cores, binning on the vector "
                              "ALUs; both FP32 peaks are 157.3 TFLOP/s); algorithmic flop per accepted pair: "
                              "s-mu 26, sigma-pi 29, monopole and angular 15"},
