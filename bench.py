@@ -43,7 +43,7 @@ FLOP_BY_KIND = {0: 15, 1: 29, 2: 26, 3: 15}
 # kernel source they were measured on: for any other source the line reports
 # no traffic.  Source: profiles/k_count_fp32_jackknife_batch_hbm.md
 KERNEL_SRC = os.path.join(ROOT, "two_pcf_amd", "csrc", "twopcf_hip.hip")
-TRAFFIC_BYTES = {("6fd9a79add253f89", 1.0, 100, 2, 9, 256): (7.902e7 * 2 + 2.532e6) * 1024}
+TRAFFIC_BYTES = {("7154385d9f8d9581", 1.0, 100, 2, 9, 256): (3.454e7 * 2 + 2.532e6) * 1024}
 ALGO_BYTES_PER_PAIR = 0.5     # algorithmic HBM bytes per accepted pair (upper bound)
 # region slots that fit in LDS on the default workload (2000-bin u32
 # histograms next to the 1024-entry tile, queues and tables)

@@ -59,7 +59,7 @@ enum {
 	 * (correlators.cpp:309-343) evaluated exactly. */
 	TWOPCF_PLAIN_FP64 = 0,
 	/* Batched: the jobs of the call in persistent launches (one per
-	 * correlator kind, weighting, line of sight and LDS use) over a cost-sorted
+	 * correlator kind, weighting, line of sight and LDS use) over a
 	 * cell-pair work queue; cells subdivided; pair separations in FP32 from
 	 * cell-local offsets; pairs within the proven FP32 error band of a range
 	 * limit or bin edge are re-evaluated with the exact pipeline, so counts
