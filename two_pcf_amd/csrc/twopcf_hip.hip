@@ -723,6 +723,14 @@ __device__ void process_item_fast(const JobDev &J, const WorkItem &W, const int4
 	const int ig_n = W.i_count > 2 * WAVE ? IG : (W.i_count > WAVE ? 2 : 1);
 	const int jg_n = (BLOCK / WAVE) / ig_n;
 	const int ig = wave % ig_n, jg = wave / ig_n;
+	// The item's row blocks of 16 first-side objects are dealt to the groups
+	// in turn (row block b to group b mod ig_n, as its (b / ig_n)-th), so that
+	// the groups of a partial item differ by one row block at most.  The LDS
+	// chunk keeps each group's row blocks together: object i of the item is
+	// in slot 64 g + 16 k + (i mod 16), and ring entries name slots.
+	const int n_rbk = (W.i_count + 15) >> 4;
+	auto slot_of = [&](int i) { const int b = i >> 4; return ((b % ig_n) << 6) + ((b / ig_n) << 4) + (i & 15); };
+	auto item_of = [&](int s) { return ((((s >> 4) & 3) * ig_n + (s >> 6)) << 4) + (s & 15); };
 	const int nb = J.nb, n1 = J.n1, n2 = J.n2;
 	const bool is_auto = J.is_auto != 0;
 	const bool jk = J.n_jk > 1;
@@ -797,24 +805,25 @@ __device__ void process_item_fast(const JobDev &J, const WorkItem &W, const int4
 	// mu n2 = |d.(n1 + n2)| rsq(s2) takes no scale multiply per pair
 	const float nscale = (KIND == K_S_MU && !MID) ? 0.5f * (float)n2 : 1.0f;
 	__syncthreads();
-	if (tid < ICHUNK) {
+	if (tid < n_rbk * 16) {
 		const int g = W.i_begin + min(tid, W.i_count - 1);
+		const int s = slot_of(tid);
 		const int code = jk ? code_of(gld(J.A.jk)[g]) : 0;
 		// angular counts work on the unit vectors (1 - n1.n2 = |n1 - n2|^2 / 2)
 		const double GLOBAL_AS *px = gld(KIND == K_ANGULAR ? J.A.nx : J.A.x);
 		const double GLOBAL_AS *py = gld(KIND == K_ANGULAR ? J.A.ny : J.A.y);
 		const double GLOBAL_AS *pz = gld(KIND == K_ANGULAR ? J.A.nz : J.A.z);
 		const float ox = (float)(px[g] - c0), oy = (float)(py[g] - c1), oz = (float)(pz[g] - c2);
-		cia[tid] = make_float4(ox, oy, oz, stage_iw ? gld(J.A.iw)[g] : gld(J.A.w)[g]);
-		cib[tid] = make_float4(nscale * (float)gld(J.A.nx)[g], nscale * (float)gld(J.A.ny)[g],
+		cia[s] = make_float4(ox, oy, oz, stage_iw ? gld(J.A.iw)[g] : gld(J.A.w)[g]);
+		cib[s] = make_float4(nscale * (float)gld(J.A.nx)[g], nscale * (float)gld(J.A.ny)[g],
 		                       nscale * (float)gld(J.A.nz)[g], __int_as_float(code));
 		// s^2 = |a|^2 + (|b|^2 - 2 a.b) may be in range iff the bracket lies
 		// within [s2_lo - eps - |a|^2, s2_hi + eps - |a|^2); objects past the
 		// end of the item never pass
 		const float aa = fmaf(oz, oz, fmaf(oy, oy, ox * ox));
 		const bool ok = tid < W.i_count;
-		cth[tid] = ok ? (s2_hi + J.scr_eps) - aa : -1e30f;
-		ctl[tid] = ok ? (s2_lo - J.scr_eps) - aa : 1e30f;
+		cth[s] = ok ? (s2_hi + J.scr_eps) - aa : -1e30f;
+		ctl[s] = ok ? (s2_lo - J.scr_eps) - aa : 1e30f;
 	}
 	// the popcount rules read the first side's masks from an LDS copy when
 	// the launch has room for it: rows of J.mstage words (bit mask, then the
@@ -830,12 +839,12 @@ __device__ void process_item_fast(const JobDev &J, const WorkItem &W, const int4
 		uint64_t LDS_AS *dst = (uint64_t LDS_AS *)(smem + L.masks);
 		for (int e = tid; e < W.i_count * upo; e += BLOCK) {
 			const int o = e / upo, u = e - o * upo;
-			*(u32x4 LDS_AS *)(dst + (size_t)o * J.mstage + 2 * u) = src[e];
+			*(u32x4 LDS_AS *)(dst + (size_t)slot_of(o) * J.mstage + 2 * u) = src[e];
 		}
 	}
 	__syncthreads();
 	// row blocks of 16 first-side objects this wave screens (uniform)
-	const int rb_n = min(4, max(0, W.i_count - ig * WAVE + 15) >> 4);
+	const int rb_n = ig < n_rbk ? min(4, (n_rbk - ig + ig_n - 1) / ig_n) : 0;
 
 	const float2 *tq = (const float2 *)(tab + J.tq_off);
 
@@ -905,7 +914,7 @@ __device__ void process_item_fast(const JobDev &J, const WorkItem &W, const int4
 	// rules built on the input weights); rules that need more read it here
 	auto weight = [&](int l2, int g2, float wp) {
 		if (WEIGHTED && needs_more) {
-			const int g1 = W.i_begin + l2;
+			const int g1 = W.i_begin + item_of(l2);
 			const double ni[3] = {gld(J.A.nx)[g1], gld(J.A.ny)[g1], gld(J.A.nz)[g1]};
 			const double nj[3] = {gld(J.B.nx)[g2], gld(J.B.ny)[g2], gld(J.B.nz)[g2]};
 			wp = pair_weight_staged<KIND == K_S_MU ? 2 : 4>(J, g1, g2, ni, nj, wp, m_lds + (size_t)l2 * J.mstage,
@@ -922,7 +931,7 @@ __device__ void process_item_fast(const JobDev &J, const WorkItem &W, const int4
 			if (b + lane < rq_n) {
 				++n_rc;
 				const int2 e = make_int2(rq_l[b + lane], rq_g[b + lane]);
-				const int g1 = W.i_begin + e.x, g2 = e.y;
+				const int g1 = W.i_begin + item_of(e.x), g2 = e.y;
 				const double xi[3] = {gld(J.A.x)[g1], gld(J.A.y)[g1], gld(J.A.z)[g1]};
 				const double ni[3] = {gld(J.A.nx)[g1], gld(J.A.ny)[g1], gld(J.A.nz)[g1]};
 				const double xj[3] = {gld(J.B.x)[g2], gld(J.B.y)[g2], gld(J.B.z)[g2]};
@@ -1285,7 +1294,7 @@ __device__ void process_item_fast(const JobDev &J, const WorkItem &W, const int4
 					const float4 th = *(const float4 *)&cth[o + 4 * lk];
 					const float4 tl = *(const float4 *)&ctl[o + 4 * lk];
 					// gj > gi  <=>  lane_d > i_begin + o + r - t0 - qb
-					const int ord0 = W.i_begin + o - t0;
+					const int ord0 = W.i_begin + ((rb * ig_n + ig) << 4) - t0;
 					// Software pipeline: the next block's operand is loaded and
 					// its product started while this block's masks are
 					// appended.  The tile's last block is padded with entries
