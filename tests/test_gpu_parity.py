@@ -231,6 +231,32 @@ def test_autotune_keeps_counts(ctx, oracle_lib):
         c.free()
 
 
+@pytest.mark.parametrize("subdiv", [1, 2, 3])
+def test_first_side_chunk_sizes(ctx, subdiv):
+    """Work items of at most 64, 128, 192 and 256 first-side objects: the
+    batched method deals an item's row blocks of 16 objects to 1, 2 or 4
+    wave groups, partial items included (cell occupancies vary with the
+    cell subdivision); its counts, with 100 regions, equal the plain
+    method's for DD, DR and RR."""
+    from two_pcf_amd import lib as L
+    cats, n_jk = _jk_batch(0.01, 17)
+    axes = [(0.0, 100.0, 40, 1.3), (0.0, 1.0, 50, 0.0)]
+    dc = {t: ctx.upload(a["x"], a["y"], a["z"], a["nx"], a["ny"], a["nz"], a["w"], a["jk"]) for t, a in cats.items()}
+    jobs = [dict(kind=L.S_MU, a=dc["data"], b=None, axes=axes, n_jk=n_jk),
+            dict(kind=L.S_MU, a=dc["data"], b=dc["randoms"], axes=axes, n_jk=n_jk),
+            dict(kind=L.S_MU, a=dc["randoms"], b=None, axes=axes, n_jk=n_jk)]
+    plain, tp = ctx.count(jobs, method=L.PLAIN_FP64)
+    for chunk in (64, 128, 192, 256):
+        print(f"[gpu] first-side chunk {chunk}, cell subdivision {subdiv}", flush=True)
+        res, t = ctx.count(jobs, method=L.FP32_RECHECK, tuning={"cell_subdiv": subdiv, "i_chunk": chunk})
+        assert t["n_pairs"] == tp["n_pairs"]
+        for k, (a, b) in enumerate(zip(res, plain)):
+            np.testing.assert_array_equal(a["full"], b["full"], err_msg=f"chunk {chunk} job {k}")
+            np.testing.assert_array_equal(a["jk"], b["jk"], err_msg=f"chunk {chunk} job {k}")
+    for c in dc.values():
+        c.free()
+
+
 @pytest.mark.parametrize("method", [0, 1])
 def test_uneven_multi_catalogue_batch(ctx, oracle_lib, method):
     """Many independent cartesian mocks of uneven size, monopole and s-mu

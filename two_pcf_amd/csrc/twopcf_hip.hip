@@ -68,6 +68,8 @@ __host__ __device__ constexpr int tile_bytes(bool fast) {
 	            : TILE_EXACT * (6 * 8 + 4 + 4 + 4);
 }
 constexpr int MAX_SLOTS = 16;
+// histogram layouts of the fp32 kernel's jobs (its RG template parameter)
+constexpr int RG_ANY = 0, RG_NONE = 1, RG_JK = 2;
 constexpr int MAX_ROWS = 64;                // stencil rows per work item (a u64 row mask)
 constexpr int MAX_SUBDIV = 3;               // cells per rmax; 4 needs more than MAX_ROWS rows
 constexpr int MAX_AXIS_BINS = 4096;
@@ -706,7 +708,11 @@ __device__ void process_item(const JobDev &J, const WorkItem &W, const int4 *__r
 // the exact pipeline, away from the screening loop.
 // MID: the job's pi is the pair-midpoint definition (sigma-pi and s-mu),
 // chosen per item so that the per-pair code has no branch on it.
-template <int KIND, bool WEIGHTED, bool MID, bool FUSE>
+// RG: the jobs' histogram layout, fixed per launch so that the per-pair code
+// has no branch on it either: RG_NONE no regions, RG_JK regions, both with
+// the full histogram in LDS; RG_ANY read from the job (any layout, the full
+// histogram possibly in the global pool).
+template <int KIND, bool WEIGHTED, bool MID, bool FUSE, int RG>
 __device__ void process_item_fast(const JobDev &J, const WorkItem &W, const int4 *__restrict__ rows,
                                   const int *__restrict__ slot_regions, char *smem, const Lds &L) {
 	static_assert(!FUSE || KIND == K_S_MU, "a monopole is fused into s-mu jobs only");
@@ -733,7 +739,7 @@ __device__ void process_item_fast(const JobDev &J, const WorkItem &W, const int4
 	auto item_of = [&](int s) { return ((((s >> 4) & 3) * ig_n + (s >> 6)) << 4) + (s & 15); };
 	const int nb = J.nb, n1 = J.n1, n2 = J.n2;
 	const bool is_auto = J.is_auto != 0;
-	const bool jk = J.n_jk > 1;
+	const bool jk = RG == RG_ANY ? J.n_jk > 1 : RG == RG_JK;
 	const int n_slots = W.n_slots;
 	// weights staged with the objects: the input weights for the rules built
 	// on them, else the weights
@@ -767,7 +773,7 @@ __device__ void process_item_fast(const JobDev &J, const WorkItem &W, const int4
 	// (none in LDS when the histogram is kept in the global pool)
 	const int f_shift = J.f_shift;
 	const int f_lane = lane & ((1 << f_shift) - 1);
-	const bool full_g = J.full_global != 0;
+	const bool full_g = RG == RG_ANY && J.full_global != 0;
 	HT *F = (HT *)(smem + L.hist);
 	HT *H = F + (full_g ? 0 : (nb << f_shift));
 
@@ -1404,7 +1410,7 @@ __device__ void process_item_fast(const JobDev &J, const WorkItem &W, const int4
 // One launch: item = blockIdx.x (plain) or pulled from a global queue
 // (persistent, queue != nullptr).  Every block leaves when the queue is
 // drained, so the grid always finishes.
-template <int KIND, bool FAST, bool WEIGHTED, bool MID = false, bool FUSE = false>
+template <int KIND, bool FAST, bool WEIGHTED, bool MID = false, bool FUSE = false, int RG = 0>
 __global__ void __launch_bounds__(BLOCK) k_count(const JobDev *__restrict__ jobs, const WorkItem *__restrict__ items,
                                                  int n_items, unsigned int *queue, const int4 *__restrict__ rows,
                                                  const int *__restrict__ slot_regions, const float *__restrict__ tables,
@@ -1430,7 +1436,7 @@ __global__ void __launch_bounds__(BLOCK) k_count(const JobDev *__restrict__ jobs
 		const WorkItem W = items[it];
 		if constexpr (FAST) {
 			// MID: every job of this launch has the pair-midpoint pi
-			process_item_fast<KIND, WEIGHTED, MID, FUSE>(jobs[W.job], W, rows, slot_regions, smem, L);
+			process_item_fast<KIND, WEIGHTED, MID, FUSE, RG>(jobs[W.job], W, rows, slot_regions, smem, L);
 		} else {
 			process_item<KIND, WEIGHTED>(jobs[W.job], W, rows, slot_regions, smem, L);
 		}
@@ -1446,6 +1452,16 @@ double now_ms() {
 
 using KernelFn = void (*)(const JobDev *, const WorkItem *, int, unsigned *, const int4 *, const int *, const float *,
                           int, Lds);
+
+// the fp32 kernel of one histogram layout; a fused monopole comes only with
+// jobs without regions whose histograms fit in LDS
+template <int KIND, bool WEIGHTED, bool MID, bool FUSE>
+KernelFn fast_kernel_of(int rg) {
+	if constexpr (FUSE) return k_count<KIND, true, WEIGHTED, MID, true, RG_NONE>;
+	if (rg == RG_NONE) return k_count<KIND, true, WEIGHTED, MID, false, RG_NONE>;
+	if (rg == RG_JK) return k_count<KIND, true, WEIGHTED, MID, false, RG_JK>;
+	return k_count<KIND, true, WEIGHTED, MID, false, RG_ANY>;
+}
 
 struct PlanLaunch {
 	KernelFn fn;
@@ -2646,26 +2662,26 @@ int build_plan(twopcf_ctx *ctx, const twopcf_job *jobs, int n_jobs, int method, 
 		}
 		return L;
 	};
-	auto launch = [&](int kind, bool weighted, bool mid, bool fz, const WorkItem *it, int n_it, int h_elems,
+	auto launch = [&](int kind, bool weighted, bool mid, bool fz, int rg, const WorkItem *it, int n_it, int h_elems,
 	                  unsigned *queue, int grid, int mask_bytes) -> int {
 		const bool fast_kernel = fast;
 		Lds L = layout(fast_kernel, weighted, h_elems, mask_bytes);
 		if ((size_t)L.total > ctx->lds_max)
 			return fail(ctx, TWOPCF_E_LIMIT, "histogram does not fit in LDS (" + std::to_string(L.total) + " B)");
-		void (*fn)(const JobDev *, const WorkItem *, int, unsigned *, const int4 *, const int *, const float *, int,
-		           Lds) = nullptr;
+		KernelFn fn = nullptr;
+		if (fz && rg != RG_NONE) return fail(ctx, TWOPCF_E_LIMIT, "fused monopole with regions or a global histogram");
 #define PICK(K, F, W) fn = k_count<K, F, W>
-#define PICK_PI(K, W) fn = mid ? k_count<K, true, W, true> : k_count<K, true, W, false>
+#define PICK_FAST(K, W, M, FZ) fn = fast_kernel_of<K, W, M, FZ>(rg)
+#define PICK_PI(K, W, FZ) if (mid) PICK_FAST(K, W, true, FZ); else PICK_FAST(K, W, false, FZ)
 		if (fast) {
-			if (kind == K_MONO) { if (weighted) PICK(K_MONO, true, true); else PICK(K_MONO, true, false); }
+			if (kind == K_MONO) { if (weighted) PICK_FAST(K_MONO, true, false, false); else PICK_FAST(K_MONO, false, false, false); }
 			else if (kind == K_S_MU && fz) {
 				// s-mu jobs with a fused monopole
-				if (weighted) fn = mid ? k_count<K_S_MU, true, true, true, true> : k_count<K_S_MU, true, true, false, true>;
-				else fn = mid ? k_count<K_S_MU, true, false, true, true> : k_count<K_S_MU, true, false, false, true>;
+				if (weighted) { PICK_PI(K_S_MU, true, true); } else { PICK_PI(K_S_MU, false, true); }
 			}
-			else if (kind == K_S_MU) { if (weighted) PICK_PI(K_S_MU, true); else PICK_PI(K_S_MU, false); }
-			else if (kind == K_SIGMA_PI) { if (weighted) PICK_PI(K_SIGMA_PI, true); else PICK_PI(K_SIGMA_PI, false); }
-			else { if (weighted) PICK(K_ANGULAR, true, true); else PICK(K_ANGULAR, true, false); }
+			else if (kind == K_S_MU) { if (weighted) { PICK_PI(K_S_MU, true, false); } else { PICK_PI(K_S_MU, false, false); } }
+			else if (kind == K_SIGMA_PI) { if (weighted) { PICK_PI(K_SIGMA_PI, true, false); } else { PICK_PI(K_SIGMA_PI, false, false); } }
+			else { if (weighted) PICK_FAST(K_ANGULAR, true, false, false); else PICK_FAST(K_ANGULAR, false, false, false); }
 		} else {
 			if (kind == K_MONO) { if (weighted) PICK(K_MONO, false, true); else PICK(K_MONO, false, false); }
 			else if (kind == K_S_MU) { if (weighted) PICK(K_S_MU, false, true); else PICK(K_S_MU, false, false); }
@@ -2673,6 +2689,7 @@ int build_plan(twopcf_ctx *ctx, const twopcf_job *jobs, int n_jobs, int method, 
 			else { if (weighted) PICK(K_ANGULAR, false, true); else PICK(K_ANGULAR, false, false); }
 		}
 #undef PICK
+#undef PICK_FAST
 #undef PICK_PI
 		HIPCK(ctx, hipFuncSetAttribute((const void *)fn, hipFuncAttributeMaxDynamicSharedMemorySize, L.total));
 		if (n_it <= 0) return 0;
@@ -2687,7 +2704,7 @@ int build_plan(twopcf_ctx *ctx, const twopcf_job *jobs, int n_jobs, int method, 
 			while (k1 < items.size() && items[k1].job == q) ++k1;
 			int slots = 0;
 			for (size_t k = k0; k < k1; ++k) slots = std::max(slots, items[k].n_slots);
-			if ((rc = launch(hj[q].j->kind, prep[q].weighted, false, false, d_items + k0, (int)(k1 - k0),
+			if ((rc = launch(hj[q].j->kind, prep[q].weighted, false, false, RG_ANY, d_items + k0, (int)(k1 - k0),
 			                 prep[q].f_elems + prep[q].nb * slots, nullptr, (int)(k1 - k0), 0))) {
 				release();
 				return rc;
@@ -2696,19 +2713,29 @@ int build_plan(twopcf_ctx *ctx, const twopcf_job *jobs, int n_jobs, int method, 
 		}
 	} else {
 		// batched: one persistent launch per (kind, weighted, pi definition,
-		// fused monopole, LDS mask copy) class over the ordered queue of
-		// all jobs in that class; the classes run concurrently unless
-		// TWOPCF_SERIAL_CLASSES is set
+		// fused monopole, LDS mask copy, histogram layout) class over the
+		// ordered queue of all jobs in that class; the classes run
+		// concurrently unless TWOPCF_SERIAL_CLASSES is set
 		plan.concurrent = getenv("TWOPCF_SERIAL_CLASSES") == nullptr;
 		auto mid_of = [&](int q) {
 			const int k = hj[q].j->kind;
 			return (k == TWOPCF_SIGMA_PI || k == TWOPCF_S_MU) && hj[q].j->pi_def == TWOPCF_PI_MIDPOINT ? 1 : 0;
 		};
+		// histogram layout: regions or none, full histogram in LDS; jobs with
+		// the full histogram in the global pool take the generic kernel
+		auto rg_of = [&](int q) {
+			if (prep[q].jd.full_global) return RG_ANY;
+			return hj[q].j->n_jk > 1 ? RG_JK : RG_NONE;
+		};
 		std::map<std::tuple<int, int, int>, std::vector<WorkItem>> cls;
 		for (auto &W : items)
 			cls[{hj[W.job].j->kind, prep[W.job].weighted ? 1 : 0,
-			     mid_of(W.job) + (fused[W.job] >= 0 ? 2 : 0) + (prep[W.job].jd.mstage > 0 ? 4 : 0)}]
+			     mid_of(W.job) + (fused[W.job] >= 0 ? 2 : 0) + (prep[W.job].jd.mstage > 0 ? 4 : 0) + 8 * rg_of(W.job)}]
 			    .push_back(W);
+		if (cls.size() > 64) {
+			release();
+			return fail(ctx, TWOPCF_E_LIMIT, "more than 64 launch classes in one call");
+		}
 		size_t off = 0;
 		std::vector<WorkItem> flat;
 		std::vector<std::tuple<int, int, size_t, size_t, int>> spans;
@@ -2728,6 +2755,7 @@ int build_plan(twopcf_ctx *ctx, const twopcf_job *jobs, int n_jobs, int method, 
 			const int mid = std::get<4>(sp) & 1;
 			const bool fz = (std::get<4>(sp) & 2) != 0;   // s-mu jobs with a fused monopole
 			const bool st = (std::get<4>(sp) & 4) != 0;   // jobs with the LDS mask copy
+			const int rg = std::get<4>(sp) >> 3;            // histogram layout
 			// per job: its largest slot count over the launch's items
 			std::vector<int> slots_of(n_jobs, 0);
 			for (size_t k = std::get<2>(sp); k < std::get<2>(sp) + std::get<3>(sp); ++k)
@@ -2735,7 +2763,7 @@ int build_plan(twopcf_ctx *ctx, const twopcf_job *jobs, int n_jobs, int method, 
 			int h_elems = 1, mask_bytes = 0;
 			for (int q = 0; q < n_jobs; ++q)
 				if (!absorbed[q] && hj[q].j->kind == kind && prep[q].weighted == weighted && mid_of(q) == mid &&
-				    (fused[q] >= 0) == fz && (prep[q].jd.mstage > 0) == st) {
+				    (fused[q] >= 0) == fz && (prep[q].jd.mstage > 0) == st && rg_of(q) == rg) {
 					h_elems = std::max(h_elems, prep[q].f_elems + hj[q].nb * slots_of[q]);
 					mask_bytes = std::max(mask_bytes, prep[q].mask_bytes);
 				}
@@ -2745,7 +2773,7 @@ int build_plan(twopcf_ctx *ctx, const twopcf_job *jobs, int n_jobs, int method, 
 			Lds L = layout(true, weighted, h_elems, mask_bytes);
 			const int per_cu = std::max(1, std::min(8, (int)(ctx->lds_max / std::max(L.total, 1))));
 			const int grid = std::min<long long>((long long)ctx->n_cu * per_cu, (long long)std::get<3>(sp));
-			if ((rc = launch(kind, weighted, mid != 0, fz, d_items + std::get<2>(sp), (int)std::get<3>(sp), h_elems,
+			if ((rc = launch(kind, weighted, mid != 0, fz, rg, d_items + std::get<2>(sp), (int)std::get<3>(sp), h_elems,
 			                 d_queue + qi, std::max(grid, 1), mask_bytes))) {
 				release();
 				return rc;
