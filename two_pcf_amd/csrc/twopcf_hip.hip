@@ -60,6 +60,9 @@ constexpr int RQ = WAVE * (NP_MAX + 1);     // per-wave queue of pairs to re-eva
 constexpr int REGION_MAP = 512;             // region -> slot code table (fp32 path, n_jk up to this)
 constexpr int TILE_BLOCKS = TILE_FAST / 16; // 16-entry screening blocks per tile (one lane each)
 static_assert(TILE_BLOCKS <= 64, "one lane per tile block");
+// s_waitcnt immediate of gfx9-family targets: vmcnt 0 (bits 3:0 and 15:14),
+// expcnt 7 and lgkmcnt 15 (no wait on those)
+constexpr unsigned VMCNT0 = 0x0F70u;
 // tile + (fp32 path) first-side chunk with its two screening thresholds,
 // per-wave pair queues, the region map and the z bounds of the tile blocks
 __host__ __device__ constexpr int tile_bytes(bool fast) {
@@ -728,7 +731,11 @@ __device__ void process_item_fast(const JobDev &J, const WorkItem &W, const int4
 	// first-side groups so that no wave idles
 	const int ig_n = W.i_count > 2 * WAVE ? IG : (W.i_count > WAVE ? 2 : 1);
 	const int jg_n = (BLOCK / WAVE) / ig_n;
-	const int ig = wave % ig_n, jg = wave / ig_n;
+	// The waves of a block are placed on the CU's four SIMDs in turn (wave w
+	// on SIMD w mod 4).  Each group's waves are spread over the four SIMDs
+	// (groups take runs of four consecutive waves), so that a group with more
+	// pairs than the others loads every SIMD alike instead of one.
+	const int ig = (wave >> 2) % ig_n, jg = (wave & 3) | (((wave >> 2) / ig_n) << 2);
 	// The item's row blocks of 16 first-side objects are dealt to the groups
 	// in turn (row block b to group b mod ig_n, as its (b / ig_n)-th), so that
 	// the groups of a partial item differ by one row block at most.  The LDS
@@ -962,6 +969,13 @@ __device__ void process_item_fast(const JobDev &J, const WorkItem &W, const int4
 			}
 		}
 		rq_n = 0;
+		// Every load above has landed before the loop goes on (vmcnt 0):
+		// the exact pipeline loads some operands on paths that never use
+		// them, and without this wait the compiler keeps them pending into
+		// the screening and binning loops, where its waits for them would
+		// also wait for every global atomic issued since (regions without
+		// an LDS slot).
+		__builtin_amdgcn_s_waitcnt(VMCNT0);
 		__builtin_amdgcn_wave_barrier();
 	};
 
@@ -1278,6 +1292,13 @@ __device__ void process_item_fast(const JobDev &J, const WorkItem &W, const int4
 				const bool order = t0 < own_end;
 				const int nblk = (nt + 15) >> 4;
 				const unsigned long long valid = nblk >= 64 ? ~0ull : (1ull << nblk) - 1ull;
+				// The (row block, tile block) units of this wave group, row
+				// block by row block, are dealt to its jg_n waves in turn: unit
+				// u goes to wave u mod jg_n.  A row block's window of kb..ke
+				// then starts at this wave's next unit, and the waves differ by
+				// one unit at most over the whole tile (instead of by one per
+				// row block).
+				int dealt = 0;
 				for (int rb = 0; rb < rb_n; ++rb) {
 					const int o = ig * WAVE + rb * 16;   // first object of the row block
 					// The first-side objects are sorted along z too: the row
@@ -1295,6 +1316,8 @@ __device__ void process_item_fast(const JobDev &J, const WorkItem &W, const int4
 					if (need == 0ull || below == valid) continue;
 					const int kb = __builtin_ctzll(~below), ke = 64 - __builtin_clzll(need);
 					if (kb >= ke) continue;
+					const int first = (jg - dealt % jg_n + jg_n) % jg_n;   // this wave's first unit
+					dealt += ke - kb;
 					const float4 ca = cia[o + lrow];
 					const float av = lk == 0 ? -2.0f * ca.x : lk == 1 ? -2.0f * ca.y : lk == 2 ? -2.0f * ca.z : 1.0f;
 					const float4 th = *(const float4 *)&cth[o + 4 * lk];
@@ -1312,8 +1335,8 @@ __device__ void process_item_fast(const JobDev &J, const WorkItem &W, const int4
 					auto product = [&](float bv) {
 						return __builtin_amdgcn_mfma_f32_16x16x4f32(av, bv, f32x4{0.0f, 0.0f, 0.0f, 0.0f}, 0, 0, 0);
 					};
-					// this wave group's blocks: k = jg mod jg_n in [kb, ke)
-					const int q0 = 16 * (kb + (jg - kb % jg_n + jg_n) % jg_n), q1 = 16 * ke;
+					// this wave's blocks of the window: kb + first, then every jg_n-th
+					const int q0 = 16 * (kb + first), q1 = 16 * ke;
 					f32x4 d_next = product(b_load(q0));
 					float b_next = b_load(q0 + step);
 					for (int qb = q0; qb < q1; qb += step) {
