@@ -1079,8 +1079,13 @@ __device__ void process_item_fast(const JobDev &J, const WorkItem &W, const int4
 			// bin: closed-form estimate, checked against the exact thresholds,
 			// proved by the error margins; the axis variable is s, or
 			// 1 - cos = chord^2 / 2 for angular counts.  Pairs the screening
-			// let through from its error shell are certainly out of range.
-			in_range = s2 < s2_hi && s2 >= s2_lo_eff;
+			// let through from its error shell (s^2 within eps outside the
+			// range) are not sure in any bin: the first bin's lower threshold
+			// lies above s2_lo and the last bin's upper one below s2_hi, so
+			// the recheck drops them, which costs less than a range test on
+			// every pair.  With a fused monopole the screening range is the
+			// union of two: pairs out of the s-mu range are dropped here.
+			in_range = !FUSE || (s2 < s2_hi && s2 >= s2_lo_eff);
 			// s-mu takes s for its estimate as s2 rsq(s2).  At s2 = 0 that is
 			// 0 * inf = NaN: the estimate clamps to a valid bin, whose lower
 			// threshold T_k + E > 0 fails, and mu is NaN, so the pair is never
@@ -1105,16 +1110,17 @@ __device__ void process_item_fast(const JobDev &J, const WorkItem &W, const int4
 				}
 				// Bin km of mu is proved when xm lies at least the margin
 				// delta = c1 + c2 rsq(s2) (+ the midpoint term) inside it:
-				// |fract(xm) - 1/2| <= 1/2 - delta, and km < n2.  The host's
-				// h0 = 1/2 - c1 - 2^-21 leaves room for this test's own
-				// roundings.  At mu = 0 the lower edge is not a bin edge; the
-				// test still asks for the margin there, which only rechecks
-				// pairs with mu below about 1e-5.
+				// |fract(xm) - 1/2| <= 1/2 - delta.  The host's h0 = 1/2 - c1 -
+				// 2^-21 leaves room for this test's own roundings.  km < n2
+				// follows: mu <= 1, so xm exceeds n2 by at most its error, c1,
+				// and then fract(xm) <= c1 fails the test.  At mu = 0 the lower
+				// edge is not a bin edge; the test still asks for the margin
+				// there, which only rechecks pairs with mu below about 1e-5.
 				float hm = fmaf(-mu_c2, rs, mu_h0_v);
 				if constexpr (decltype(mid)::value) hm -= nmu * mid_c * inv_mid;
 				const int km = (int)xm;
 				const float fr = __builtin_amdgcn_fractf(xm);
-				sure = sure && (km < n2) && (fabsf(fr - 0.5f) <= hm);
+				sure = sure && (fabsf(fr - 0.5f) <= hm);
 				bin = __mul24(k, n2) + km;
 				if constexpr (FUSE) {
 					// the monopole bin from the same s: a pair goes to the
@@ -1128,7 +1134,7 @@ __device__ void process_item_fast(const JobDev &J, const WorkItem &W, const int4
 			} else {
 				bin = k;
 			}
-			sure = sure || !in_range;
+			if constexpr (FUSE) sure = sure || !in_range;
 		}
 		r.sure = sure;
 		r.in_range = in_range;
