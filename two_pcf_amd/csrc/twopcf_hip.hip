@@ -140,6 +140,7 @@ struct WorkItem {
 	int i_begin, i_count, i_cell;
 	unsigned long long row_mask;
 	int slot_base, n_slots;
+	int all_slotted;   // fp32 path: every region the item meets has an LDS slot
 };
 
 }  // namespace
@@ -748,6 +749,9 @@ __device__ void process_item_fast(const JobDev &J, const WorkItem &W, const int4
 	const bool is_auto = J.is_auto != 0;
 	const bool jk = RG == RG_ANY ? J.n_jk > 1 : RG == RG_JK;
 	const int n_slots = W.n_slots;
+	// every region the item meets has an LDS slot: no lane's pair needs the
+	// global pool (the region test per pair is skipped)
+	const bool all_slotted = W.all_slotted != 0;
 	// weights staged with the objects: the input weights for the rules built
 	// on them, else the weights
 	const int wmode = J.wmode;
@@ -897,7 +901,7 @@ __device__ void process_item_fast(const JobDev &J, const WorkItem &W, const int4
 	auto add = [&](int bin, float w, int si, int sj) {
 		if (!cross_only) add_full(bin, w);
 		if (jk) {
-			if (__builtin_amdgcn_ballot_w64(si < 0 || sj < 0) == 0ull) {
+			if (all_slotted || __builtin_amdgcn_ballot_w64(si < 0 || sj < 0) == 0ull) {
 				// every active lane's regions have LDS slots
 				if (WEIGHTED) atomicAdd((double *)&H[si + bin], (double)w);
 				else atomicAdd((unsigned int *)&H[si + bin], 1u);
@@ -2545,6 +2549,10 @@ int build_plan(twopcf_ctx *ctx, const twopcf_job *jobs, int n_jobs, int method, 
 					W.i_cell = c;
 					W.row_mask = mask;
 					W.slot_base = (int)slot_pool.size();
+					// regs holds every region of the chunk's cell and of the
+					// rows' cells: when they all fit, every pair's two regions
+					// have slots
+					W.all_slotted = group_regions && (int)regs.size() <= slot_cap ? 1 : 0;
 					if ((int)regs.size() > slot_cap) regs.resize(slot_cap);   // the rest: global atomics
 					W.n_slots = group_regions ? (int)regs.size() : 0;
 					if (group_regions) slot_pool.insert(slot_pool.end(), regs.begin(), regs.end());
