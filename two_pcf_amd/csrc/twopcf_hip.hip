@@ -1326,10 +1326,12 @@ __device__ void process_item_fast(const JobDev &J, const WorkItem &W, const int4
 					if (need == 0ull || below == valid) continue;
 					const int kb = __builtin_ctzll(~below), ke = 64 - __builtin_clzll(need);
 					if (kb >= ke) continue;
-					const int first = (jg - dealt % jg_n + jg_n) % jg_n;   // this wave's first unit
+					const int first = (jg - dealt) & (jg_n - 1);   // this wave's first unit (jg_n: 4, 8 or 16)
 					dealt += ke - kb;
-					const float4 ca = cia[o + lrow];
-					const float av = lk == 0 ? -2.0f * ca.x : lk == 1 ? -2.0f * ca.y : lk == 2 ? -2.0f * ca.z : 1.0f;
+					// A[row][lk]: -2 o of the row's object for lk < 3, else 1 (one
+					// load per lane, no branch on lk)
+					const float ca = ((const float *)cia)[4 * (o + lrow) + min(lk, 2)];
+					const float av = lk < 3 ? -2.0f * ca : 1.0f;
 					const float4 th = *(const float4 *)&cth[o + 4 * lk];
 					const float4 tl = *(const float4 *)&ctl[o + 4 * lk];
 					// gj > gi  <=>  lane_d > i_begin + o + r - t0 - qb
