@@ -63,17 +63,20 @@ static_assert(TILE_BLOCKS <= 64, "one lane per tile block");
 // s_waitcnt immediate of gfx9-family targets: vmcnt 0 (bits 3:0 and 15:14),
 // expcnt 7 and lgkmcnt 15 (no wait on those)
 constexpr unsigned VMCNT0 = 0x0F70u;
+constexpr int ROW_SPANS = 64;              // stencil rows of a work item with their second-side spans (fp32 path)
 // tile + (fp32 path) first-side chunk with its two screening thresholds,
-// per-wave pair queues, the region map and the z bounds of the tile blocks
+// per-wave pair queues, the region map, the z bounds of the tile blocks and
+// the item's row spans
 __host__ __device__ constexpr int tile_bytes(bool fast) {
 	return fast ? TILE_FAST * (2 * 16 + 4) + ICHUNK * (32 + 8) + (BLOCK / WAVE) * (RING * 2 + RQ * 5) + REGION_MAP * 4 +
-	                  TILE_BLOCKS * 8
+	                  TILE_BLOCKS * 8 + (ROW_SPANS + 1) * 8
 	            : TILE_EXACT * (6 * 8 + 4 + 4 + 4);
 }
 constexpr int MAX_SLOTS = 16;
 // histogram layouts of the fp32 kernel's jobs (its RG template parameter)
 constexpr int RG_ANY = 0, RG_NONE = 1, RG_JK = 2;
 constexpr int MAX_ROWS = 64;                // stencil rows per work item (a u64 row mask)
+static_assert(MAX_ROWS <= ROW_SPANS, "one LDS span per stencil row");
 constexpr int MAX_SUBDIV = 3;               // cells per rmax; 4 needs more than MAX_ROWS rows
 constexpr int MAX_AXIS_BINS = 4096;
 
@@ -777,6 +780,10 @@ __device__ void process_item_fast(const JobDev &J, const WorkItem &W, const int4
 	// z of the first and of the last entry of each 16-entry tile block
 	float *tbz = (float *)(rmap + REGION_MAP);
 	float *tbe = tbz + TILE_BLOCKS;
+	// per stencil row of the item: the second side's objects [x, y) (empty
+	// for rows outside the grid or the auto count's excluded half); entry
+	// ROW_SPANS: the end of the first-side cell (auto counts)
+	int2 *rspan = (int2 *)(tbe + TILE_BLOCKS);
 	float *tab = (float *)(smem + L.tables);
 	int *sreg = (int *)(smem + L.slots);
 	// the full histogram: 2^f_shift copies, bin-major, lane l adding to copy
@@ -821,19 +828,42 @@ __device__ void process_item_fast(const JobDev &J, const WorkItem &W, const int4
 	// s-mu with the angular pi stages its normals times n2 / 2, so that
 	// mu n2 = |d.(n1 + n2)| rsq(s2) takes no scale multiply per pair
 	const float nscale = (KIND == K_S_MU && !MID) ? 0.5f * (float)n2 : 1.0f;
+	// the item's row spans, one thread per stencil row, read from the cell
+	// starts once per item instead of once per row and wave
+	if (tid < J.n_rows || tid == ROW_SPANS) {
+		int2 sp = make_int2(0, 0);
+		if (tid == ROW_SPANS) {
+			if (is_auto) sp.x = gld(J.B.start)[W.i_cell + 1];
+		} else if ((W.row_mask >> tid) & 1ull) {
+			const int4 row = rows[J.row_off + tid];
+			const int qx = cx + row.x, qy = cy + row.y;
+			if (qx >= 0 && qy >= 0 && qx < J.dim[0] && qy < d1 && !(is_auto && (row.x < 0 || (row.x == 0 && row.y < 0)))) {
+				int z0 = max(cz + row.z, 0);
+				const int z1 = min(cz + row.w, d2 - 1);
+				if (is_auto && row.x == 0 && row.y == 0) z0 = max(z0, cz);
+				const int base = (qx * d1 + qy) * d2;
+				if (z0 <= z1) sp = make_int2(gld(J.B.start)[base + z0], gld(J.B.start)[base + z1 + 1]);
+			}
+		}
+		rspan[tid] = sp;
+	}
 	__syncthreads();
 	if (tid < n_rbk * 16) {
 		const int g = W.i_begin + min(tid, W.i_count - 1);
 		const int s = slot_of(tid);
-		const int code = jk ? code_of(gld(J.A.jk)[g]) : 0;
-		// angular counts work on the unit vectors (1 - n1.n2 = |n1 - n2|^2 / 2)
+		// every global load first, so that one latency covers them all
+		// (angular counts work on the unit vectors: 1 - n1.n2 = |n1 - n2|^2 / 2)
 		const double GLOBAL_AS *px = gld(KIND == K_ANGULAR ? J.A.nx : J.A.x);
 		const double GLOBAL_AS *py = gld(KIND == K_ANGULAR ? J.A.ny : J.A.y);
 		const double GLOBAL_AS *pz = gld(KIND == K_ANGULAR ? J.A.nz : J.A.z);
-		const float ox = (float)(px[g] - c0), oy = (float)(py[g] - c1), oz = (float)(pz[g] - c2);
-		cia[s] = make_float4(ox, oy, oz, stage_iw ? gld(J.A.iw)[g] : gld(J.A.w)[g]);
-		cib[s] = make_float4(nscale * (float)gld(J.A.nx)[g], nscale * (float)gld(J.A.ny)[g],
-		                       nscale * (float)gld(J.A.nz)[g], __int_as_float(code));
+		const double xg = px[g], yg = py[g], zg = pz[g];
+		const double nxg = gld(J.A.nx)[g], nyg = gld(J.A.ny)[g], nzg = gld(J.A.nz)[g];
+		const float wg = stage_iw ? gld(J.A.iw)[g] : gld(J.A.w)[g];
+		const int rg = jk ? gld(J.A.jk)[g] : 0;
+		const int code = jk ? code_of(rg) : 0;
+		const float ox = (float)(xg - c0), oy = (float)(yg - c1), oz = (float)(zg - c2);
+		cia[s] = make_float4(ox, oy, oz, wg);
+		cib[s] = make_float4(nscale * (float)nxg, nscale * (float)nyg, nscale * (float)nzg, __int_as_float(code));
 		// s^2 = |a|^2 + (|b|^2 - 2 a.b) may be in range iff the bracket lies
 		// within [s2_lo - eps - |a|^2, s2_hi + eps - |a|^2); objects past the
 		// end of the item never pass
@@ -1250,43 +1280,41 @@ __device__ void process_item_fast(const JobDev &J, const WorkItem &W, const int4
 	// gridding (h 2^-ZB_BITS) on both sides of a block's first and last entry
 	const float zpad = (float)(J.h * 0x1p-8);
 
+	const int own_cell_end = __builtin_amdgcn_readfirstlane(rspan[ROW_SPANS].x);
 	for (int r = 0; r < J.n_rows; ++r) {
 		if (!((W.row_mask >> r) & 1ull)) continue;
+		const int jb = __builtin_amdgcn_readfirstlane(rspan[r].x);
+		const int je = __builtin_amdgcn_readfirstlane(rspan[r].y);
+		if (jb >= je) continue;
 		const int4 row = rows[J.row_off + r];
-		const int qx = cx + row.x, qy = cy + row.y;
-		if (qx < 0 || qy < 0 || qx >= J.dim[0] || qy >= d1) continue;
 		// A pair can pass the screening only if its z separation is at most
 		// reach: the first-side cell and this row's cells are at least gx, gy
 		// apart across x and y, and the screening passes s^2 < s2_hi + eps.
 		const float gx = (float)(max(abs(row.x) - 1, 0) * J.h), gy = (float)(max(abs(row.y) - 1, 0) * J.h);
 		const float reach = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(
 		    __builtin_amdgcn_sqrtf(fmaxf(s2_hi + J.scr_eps - fmaf(gx, gx, gy * gy), 0.0f)) * (1.0f + 0x1p-10f) + zpad)));
-		if (is_auto && (row.x < 0 || (row.x == 0 && row.y < 0))) continue;
-		int z0 = max(cz + row.z, 0), z1 = min(cz + row.w, d2 - 1);
-		if (is_auto && row.x == 0 && row.y == 0) z0 = max(z0, cz);
-		if (z0 > z1) continue;
-		const int base = (qx * d1 + qy) * d2;
-		const int jb = __builtin_amdgcn_readfirstlane(gld(J.B.start)[base + z0]);
-		const int je = __builtin_amdgcn_readfirstlane(gld(J.B.start)[base + z1 + 1]);
 		// only the row through the first-side cell of an auto count holds
 		// pairs of the excluded half (cell-major order)
-		const int own_end = is_auto && row.x == 0 && row.y == 0
-		                        ? __builtin_amdgcn_readfirstlane(gld(J.B.start)[W.i_cell + 1]) : jb;
+		const int own_end = is_auto && row.x == 0 && row.y == 0 ? own_cell_end : jb;
 		for (int t0 = jb; t0 < je; t0 += TILE) {
 			const int nt = min(TILE, je - t0);
 			t_base = t0;
 			if (tid < nt) {
 				const int gj = t0 + tid;
-				const int sl = jk ? code_of(gld(J.B.jk)[gj]) : 0;
+				// every global load first, so that one latency covers them all
 				const double GLOBAL_AS *px = gld(KIND == K_ANGULAR ? J.B.nx : J.B.x);
 				const double GLOBAL_AS *py = gld(KIND == K_ANGULAR ? J.B.ny : J.B.y);
 				const double GLOBAL_AS *pz = gld(KIND == K_ANGULAR ? J.B.nz : J.B.z);
-				const float ox = (float)(px[gj] - c0), oy = (float)(py[gj] - c1), oz = (float)(pz[gj] - c2);
+				const double xg = px[gj], yg = py[gj], zg = pz[gj];
+				const double nxg = gld(J.B.nx)[gj], nyg = gld(J.B.ny)[gj], nzg = gld(J.B.nz)[gj];
+				const float wg = WEIGHTED ? (stage_iw ? gld(J.B.iw)[gj] : gld(J.B.w)[gj]) : 0.0f;
+				const int rj = jk ? gld(J.B.jk)[gj] : 0;
+				const int sl = jk ? code_of(rj) : 0;
+				const float ox = (float)(xg - c0), oy = (float)(yg - c1), oz = (float)(zg - c2);
 				tfa[tid] = make_float4(ox, oy, oz, fmaf(oz, oz, fmaf(oy, oy, ox * ox)));
 				// unweighted jobs keep the region code where the weight would be
-				tfb[tid] = make_float4(nscale * (float)gld(J.B.nx)[gj], nscale * (float)gld(J.B.ny)[gj],
-				                       nscale * (float)gld(J.B.nz)[gj],
-				                       WEIGHTED ? (stage_iw ? gld(J.B.iw)[gj] : gld(J.B.w)[gj]) : __int_as_float(sl));
+				tfb[tid] = make_float4(nscale * (float)nxg, nscale * (float)nyg, nscale * (float)nzg,
+				                       WEIGHTED ? wg : __int_as_float(sl));
 				tslot[tid] = sl;
 				// the row is sorted along z: each block's z range is its first
 				// and last entry (up to sort ties within the slack)
@@ -1309,6 +1337,9 @@ __device__ void process_item_fast(const JobDev &J, const WorkItem &W, const int4
 				// one unit at most over the whole tile (instead of by one per
 				// row block).
 				int dealt = 0;
+				// z span of tile block `lane` (lanes past the tile's blocks read
+				// stale bounds, which the valid mask drops)
+				const float tz0 = tbz[lane], tz1 = tbe[lane];
 				for (int rb = 0; rb < rb_n; ++rb) {
 					const int o = ig * WAVE + rb * 16;   // first object of the row block
 					// The first-side objects are sorted along z too: the row
@@ -1319,10 +1350,8 @@ __device__ void process_item_fast(const JobDev &J, const WorkItem &W, const int4
 					// the last block not above (tile block `lane` spans
 					// tbz[lane] .. tbe[lane]).
 					const float zlo = cia[o].z - reach, zhi = cia[o + 15].z + reach;
-					const unsigned long long below =
-					    __builtin_amdgcn_ballot_w64(lane < nblk && tbe[lane] < zlo) & valid;
-					const unsigned long long need =
-					    ~__builtin_amdgcn_ballot_w64(lane < nblk && tbz[lane] > zhi) & valid;
+					const unsigned long long below = __builtin_amdgcn_ballot_w64(tz1 < zlo) & valid;
+					const unsigned long long need = ~__builtin_amdgcn_ballot_w64(tz0 > zhi) & valid;
 					if (need == 0ull || below == valid) continue;
 					const int kb = __builtin_ctzll(~below), ke = 64 - __builtin_clzll(need);
 					if (kb >= ke) continue;
