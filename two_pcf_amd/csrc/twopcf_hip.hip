@@ -797,13 +797,10 @@ __device__ void process_item_fast(const JobDev &J, const WorkItem &W, const int4
 
 	for (int k = tid; k < n_slots; k += BLOCK) sreg[k] = slot_regions[W.slot_base + k];
 	const bool use_map = jk && J.n_jk <= REGION_MAP;
+	// the region map: every region without a slot here, then (after a
+	// barrier) each slot's region
 	if (use_map)
-		for (int r = tid; r < J.n_jk; r += BLOCK) {
-			int code = -(r + 1);
-			for (int k = 0; k < n_slots; ++k)
-				if (gld(slot_regions)[W.slot_base + k] == r) code = k * nb;
-			rmap[r] = code;
-		}
+		for (int r = tid; r < J.n_jk; r += BLOCK) rmap[r] = -(r + 1);
 	auto code_of = [&](int r) {
 		if (use_map) return rmap[r];
 		int code = -(r + 1);
@@ -848,6 +845,10 @@ __device__ void process_item_fast(const JobDev &J, const WorkItem &W, const int4
 		rspan[tid] = sp;
 	}
 	__syncthreads();
+	if (use_map) {
+		for (int k = tid; k < n_slots; k += BLOCK) rmap[sreg[k]] = k * nb;
+		__syncthreads();
+	}
 	if (tid < n_rbk * 16) {
 		const int g = W.i_begin + min(tid, W.i_count - 1);
 		const int s = slot_of(tid);
