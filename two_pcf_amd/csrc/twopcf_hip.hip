@@ -75,6 +75,10 @@ __host__ __device__ constexpr int tile_bytes(bool fast) {
 constexpr int MAX_SLOTS = 16;
 // histogram layouts of the fp32 kernel's jobs (its RG template parameter)
 constexpr int RG_ANY = 0, RG_NONE = 1, RG_JK = 2;
+// a second job binned in the pass of another (fp32 path, its FZ template
+// parameter): none; a monopole in an s-mu job's pass; a twin, an angular job
+// of the same catalogues and axis with another pair-weight rule
+constexpr int FZ_NONE = 0, FZ_MONO = 1, FZ_TWIN = 2;
 constexpr int MAX_ROWS = 64;                // stencil rows per work item (a u64 row mask)
 static_assert(MAX_ROWS <= ROW_SPANS, "one LDS span per stencil row");
 constexpr int MAX_SUBDIV = 3;               // cells per rmax; 4 needs more than MAX_ROWS rows
@@ -131,6 +135,12 @@ struct JobDev {
 	// with the unused ones zero (inv_step on a log axis, ils and ilb on a
 	// linear one)
 	int m_nb, m_f_shift, m_tq_off, m_t_s2_off, m_tA_off, m_tB_off;
+	// fp32 path, an angular job with a twin (same catalogues, axis and pair
+	// set, no regions; the twin's outputs in the m_* fields): the twin's
+	// weight is the product of the other staged weights (w when this job's
+	// rule stages iw, and the reverse; staged in the region-code places),
+	// or with t_w = 0 the same product of staged weights as this job's
+	int t_w;
 	float m_s_ilb_ln2, m_s_ils, m_s_inv_step;
 	AxisDev m_ax1;
 	unsigned long long *m_full_u;
@@ -719,10 +729,12 @@ __device__ void process_item(const JobDev &J, const WorkItem &W, const int4 *__r
 // has no branch on it either: RG_NONE no regions, RG_JK regions, both with
 // the full histogram in LDS; RG_ANY read from the job (any layout, the full
 // histogram possibly in the global pool).
-template <int KIND, bool WEIGHTED, bool MID, bool FUSE, int RG>
+template <int KIND, bool WEIGHTED, bool MID, int FZ, int RG>
 __device__ void process_item_fast(const JobDev &J, const WorkItem &W, const int4 *__restrict__ rows,
                                   const int *__restrict__ slot_regions, char *smem, const Lds &L) {
+	constexpr bool FUSE = FZ == FZ_MONO, TWIN = FZ == FZ_TWIN;
 	static_assert(!FUSE || KIND == K_S_MU, "a monopole is fused into s-mu jobs only");
+	static_assert(!TWIN || (KIND == K_ANGULAR && WEIGHTED && RG == RG_NONE), "twins: weighted angular jobs only");
 	using HT = typename HistT<WEIGHTED>::T;
 	constexpr int TILE = TILE_FAST;
 	constexpr int NP = np_of(KIND, WEIGHTED);   // pairs per lane in binning (MID: one)
@@ -860,8 +872,10 @@ __device__ void process_item_fast(const JobDev &J, const WorkItem &W, const int4
 		const double xg = px[g], yg = py[g], zg = pz[g];
 		const double nxg = gld(J.A.nx)[g], nyg = gld(J.A.ny)[g], nzg = gld(J.A.nz)[g];
 		const float wg = stage_iw ? gld(J.A.iw)[g] : gld(J.A.w)[g];
+		// a twin's other weight takes the place of the region code
+		const float wg2 = TWIN && J.t_w ? (stage_iw ? gld(J.A.w)[g] : gld(J.A.iw)[g]) : 0.0f;
 		const int rg = jk ? gld(J.A.jk)[g] : 0;
-		const int code = jk ? code_of(rg) : 0;
+		const int code = jk ? code_of(rg) : TWIN ? __float_as_int(wg2) : 0;
 		const float ox = (float)(xg - c0), oy = (float)(yg - c1), oz = (float)(zg - c2);
 		cia[s] = make_float4(ox, oy, oz, wg);
 		cib[s] = make_float4(nscale * (float)nxg, nscale * (float)nyg, nscale * (float)nzg, __int_as_float(code));
@@ -950,7 +964,8 @@ __device__ void process_item_fast(const JobDev &J, const WorkItem &W, const int4
 			}
 		}
 	};
-	// the fused monopole's full histogram follows the s-mu one (no regions)
+	// the fused monopole's (or the twin's) full histogram follows the job's
+	// own (no regions)
 	char *const F2_lane = (char *)(F + (nb << f_shift) + (lane & ((1 << J.m_f_shift) - 1)));
 	const int f2_bshift = J.m_f_shift + (WEIGHTED ? 3 : 2);
 	auto add2 = [&](int bin, float w) {
@@ -993,6 +1008,11 @@ __device__ void process_item_fast(const JobDev &J, const WorkItem &W, const int4
 					const int sj = jk ? code_of(gld(J.B.jk)[g2]) : 0;
 					const float w = weight(e.x, g2, cia[e.x].w * (stage_iw ? gld(J.B.iw)[g2] : gld(J.B.w)[g2]));
 					add(bin, w, __float_as_int(cib[e.x].w), sj);
+					if constexpr (TWIN) {
+						++n_acc2;
+						add2(bin, J.t_w ? cib[e.x].w * (stage_iw ? gld(J.B.w)[g2] : gld(J.B.iw)[g2])
+						                : cia[e.x].w * (stage_iw ? gld(J.B.iw)[g2] : gld(J.B.w)[g2]));
+					}
 				}
 				if constexpr (FUSE) {
 					const float *m_t = tab + J.m_t_s2_off;
@@ -1250,6 +1270,11 @@ __device__ void process_item_fast(const JobDev &J, const WorkItem &W, const int4
 						++n_acc2;
 						add2(p[k].bin2, w);
 					}
+					if constexpr (TWIN) {
+						// the same bin; the twin's weight from its staged pair
+						++n_acc2;
+						add2(p[k].bin, J.t_w ? __int_as_float(p[k].si) * __int_as_float(p[k].sj) : p[k].wp);
+					}
 				}
 #pragma unroll
 			for (int k = 0; k < NP; ++k) push(lane + k * WAVE < (int)n && !p[k].sure, p[k]);
@@ -1309,8 +1334,10 @@ __device__ void process_item_fast(const JobDev &J, const WorkItem &W, const int4
 				const double xg = px[gj], yg = py[gj], zg = pz[gj];
 				const double nxg = gld(J.B.nx)[gj], nyg = gld(J.B.ny)[gj], nzg = gld(J.B.nz)[gj];
 				const float wg = WEIGHTED ? (stage_iw ? gld(J.B.iw)[gj] : gld(J.B.w)[gj]) : 0.0f;
+				// a twin's other weight takes the place of the region code
+				const float wg2 = TWIN && J.t_w ? (stage_iw ? gld(J.B.w)[gj] : gld(J.B.iw)[gj]) : 0.0f;
 				const int rj = jk ? gld(J.B.jk)[gj] : 0;
-				const int sl = jk ? code_of(rj) : 0;
+				const int sl = jk ? code_of(rj) : TWIN ? __float_as_int(wg2) : 0;
 				const float ox = (float)(xg - c0), oy = (float)(yg - c1), oz = (float)(zg - c2);
 				tfa[tid] = make_float4(ox, oy, oz, fmaf(oz, oz, fmaf(oy, oy, ox * ox)));
 				// unweighted jobs keep the region code where the weight would be
@@ -1443,7 +1470,7 @@ __device__ void process_item_fast(const JobDev &J, const WorkItem &W, const int4
 			}
 		}
 	}
-	if constexpr (FUSE) {
+	if constexpr (FUSE || TWIN) {
 		HT *F2 = F + (nb << f_shift);
 		const int mc = 1 << J.m_f_shift;
 		for (int b = tid; b < J.m_nb; b += BLOCK) {
@@ -1461,13 +1488,13 @@ __device__ void process_item_fast(const JobDev &J, const WorkItem &W, const int4
 	for (int off = 32; off; off >>= 1) {
 		n_acc += __shfl_xor(n_acc, off);
 		n_rc += __shfl_xor(n_rc, off);
-		if (FUSE) n_acc2 += __shfl_xor(n_acc2, off);
+		if (FUSE || TWIN) n_acc2 += __shfl_xor(n_acc2, off);
 	}
 	if (lane == 0) {
 		if (n_acc) gadd(&J.counters[0], (unsigned long long)n_acc);
 		if (n_rc) gadd(&J.counters[1], (unsigned long long)n_rc);
-		if (FUSE && n_acc2) gadd(&J.m_counters[0], (unsigned long long)n_acc2);
-		if (FUSE && n_rc) gadd(&J.m_counters[1], (unsigned long long)n_rc);
+		if ((FUSE || TWIN) && n_acc2) gadd(&J.m_counters[0], (unsigned long long)n_acc2);
+		if ((FUSE || TWIN) && n_rc) gadd(&J.m_counters[1], (unsigned long long)n_rc);
 	}
 	__syncthreads();
 }
@@ -1475,7 +1502,7 @@ __device__ void process_item_fast(const JobDev &J, const WorkItem &W, const int4
 // One launch: item = blockIdx.x (plain) or pulled from a global queue
 // (persistent, queue != nullptr).  Every block leaves when the queue is
 // drained, so the grid always finishes.
-template <int KIND, bool FAST, bool WEIGHTED, bool MID = false, bool FUSE = false, int RG = 0>
+template <int KIND, bool FAST, bool WEIGHTED, bool MID = false, int FZ = FZ_NONE, int RG = 0>
 __global__ void __launch_bounds__(BLOCK) k_count(const JobDev *__restrict__ jobs, const WorkItem *__restrict__ items,
                                                  int n_items, unsigned int *queue, const int4 *__restrict__ rows,
                                                  const int *__restrict__ slot_regions, const float *__restrict__ tables,
@@ -1501,7 +1528,7 @@ __global__ void __launch_bounds__(BLOCK) k_count(const JobDev *__restrict__ jobs
 		const WorkItem W = items[it];
 		if constexpr (FAST) {
 			// MID: every job of this launch has the pair-midpoint pi
-			process_item_fast<KIND, WEIGHTED, MID, FUSE, RG>(jobs[W.job], W, rows, slot_regions, smem, L);
+			process_item_fast<KIND, WEIGHTED, MID, FZ, RG>(jobs[W.job], W, rows, slot_regions, smem, L);
 		} else {
 			process_item<KIND, WEIGHTED>(jobs[W.job], W, rows, slot_regions, smem, L);
 		}
@@ -1518,14 +1545,14 @@ double now_ms() {
 using KernelFn = void (*)(const JobDev *, const WorkItem *, int, unsigned *, const int4 *, const int *, const float *,
                           int, Lds);
 
-// the fp32 kernel of one histogram layout; a fused monopole comes only with
-// jobs without regions whose histograms fit in LDS
-template <int KIND, bool WEIGHTED, bool MID, bool FUSE>
+// the fp32 kernel of one histogram layout; a fused monopole or a twin comes
+// only with jobs without regions whose histograms fit in LDS
+template <int KIND, bool WEIGHTED, bool MID, int FZ>
 KernelFn fast_kernel_of(int rg) {
-	if constexpr (FUSE) return k_count<KIND, true, WEIGHTED, MID, true, RG_NONE>;
-	if (rg == RG_NONE) return k_count<KIND, true, WEIGHTED, MID, false, RG_NONE>;
-	if (rg == RG_JK) return k_count<KIND, true, WEIGHTED, MID, false, RG_JK>;
-	return k_count<KIND, true, WEIGHTED, MID, false, RG_ANY>;
+	if constexpr (FZ != FZ_NONE) return k_count<KIND, true, WEIGHTED, MID, FZ, RG_NONE>;
+	if (rg == RG_NONE) return k_count<KIND, true, WEIGHTED, MID, FZ_NONE, RG_NONE>;
+	if (rg == RG_JK) return k_count<KIND, true, WEIGHTED, MID, FZ_NONE, RG_JK>;
+	return k_count<KIND, true, WEIGHTED, MID, FZ_NONE, RG_ANY>;
 }
 
 struct PlanLaunch {
@@ -2246,6 +2273,35 @@ int build_plan(twopcf_ctx *ctx, const twopcf_job *jobs, int n_jobs, int method, 
 			break;
 		}
 	}
+	// Twins: an angular job with parent weights (the products of the weights
+	// or of the input weights) and another of the same catalogues, axis and
+	// pair set with the other product, neither with regions, are counted in
+	// one pass (fp32 path): each pair is screened, compacted and binned once
+	// and added to both histograms with its two weights.  twin[p] = t for the
+	// job p whose pass bins job t, absorbed[t] = true.
+	std::vector<int> twin(n_jobs, -1);
+	auto parent_rule = [](int m) { return m == TWOPCF_W_ANG_IW || m == TWOPCF_W_ANG_W; };
+	for (int p = 0; p < n_jobs && fast; ++p) {
+		const HostJob &P1 = hj[p];
+		if (absorbed[p] || fused[p] >= 0 || P1.j->kind != TWOPCF_ANGULAR || P1.j->n_jk > 1 || !P1.weighted ||
+		    !parent_rule(P1.j->weight_mode))
+			continue;
+		for (int t = 0; t < n_jobs; ++t) {
+			const HostJob &T = hj[t];
+			if (t == p || absorbed[t] || fused[t] >= 0 || twin[t] >= 0 || T.j->kind != TWOPCF_ANGULAR ||
+			    T.j->n_jk > 1 || !T.weighted || !parent_rule(T.j->weight_mode))
+				continue;
+			if (T.a != P1.a || T.b != P1.b || T.rmax != P1.rmax || std::memcmp(T.j->ax, P1.j->ax, sizeof T.j->ax) != 0)
+				continue;
+			const long long hp = (long long)P1.nb * 8 << copies_of(P1.nb, true);
+			const long long ht = (long long)T.nb * 8 << copies_of(T.nb, true);
+			if ((long long)lds_fixed + hp + ht + 16 > (long long)ctx->lds_max) continue;
+			twin[p] = t;
+			absorbed[t] = 1;
+			break;
+		}
+	}
+	auto fz_of = [&](int q) { return fused[q] >= 0 ? FZ_MONO : twin[q] >= 0 ? FZ_TWIN : FZ_NONE; };
 	for (int q = 0; q < n_jobs; ++q) {
 		HostJob &H = hj[q];
 		Prepared &P = prep[q];
@@ -2487,6 +2543,15 @@ int build_plan(twopcf_ctx *ctx, const twopcf_job *jobs, int n_jobs, int method, 
 			D.m_f_shift = copies_of(M.nb, M.weighted);
 			P.f_elems += M.nb << D.m_f_shift;
 		}
+		if (twin[q] >= 0) {
+			const HostJob &T = hj[twin[q]];
+			D.m_nb = T.nb;
+			D.m_f_shift = copies_of(T.nb, true);
+			P.f_elems += T.nb << D.m_f_shift;
+			// the twin's product is of the other staged weights unless its
+			// rule stages the same ones
+			D.t_w = T.j->weight_mode != H.j->weight_mode ? 1 : 0;
+		}
 		// LDS for the histograms: the full one (with its lane copies) and, with
 		// regions, at least one slot histogram.  When they do not fit, the
 		// lane copies go first, then the full histogram moves to the global
@@ -2497,13 +2562,13 @@ int build_plan(twopcf_ctx *ctx, const twopcf_job *jobs, int n_jobs, int method, 
 		const int hist_bytes = H.nb * (H.weighted ? 8 : 4);
 		const long long avail = (long long)lds_budget - lds_fixed - (P.mask_bytes ? P.mask_bytes + 16 : 0);
 		long long f_bytes = (long long)P.f_elems * (H.weighted ? 8 : 4);
-		if (fused[q] < 0 && f_bytes + (jk ? hist_bytes : 0) > avail && D.f_shift > 0) {
+		if (fz_of(q) == FZ_NONE && f_bytes + (jk ? hist_bytes : 0) > avail && D.f_shift > 0) {
 			D.f_shift = 0;
 			P.f_elems = H.nb;
 			f_bytes = hist_bytes;
 		}
 		if (f_bytes + (jk ? hist_bytes : 0) > avail) {
-			if (fused[q] >= 0) return fail(ctx, TWOPCF_E_LIMIT, "fused histograms do not fit in LDS");
+			if (fz_of(q) != FZ_NONE) return fail(ctx, TWOPCF_E_LIMIT, "fused histograms do not fit in LDS");
 			D.full_global = 1;
 			P.f_elems = 0;
 			f_bytes = 0;
@@ -2638,6 +2703,14 @@ int build_plan(twopcf_ctx *ctx, const twopcf_job *jobs, int n_jobs, int method, 
 		P.jd.counters = d_cnt + 2 * q;
 	}
 	for (int q = 0; q < n_jobs; ++q) {
+		if (twin[q] < 0) continue;
+		JobDev &D = prep[q].jd;
+		const JobDev &T = prep[twin[q]].jd;
+		D.m_full_u = T.full_u;
+		D.m_full_d = T.full_d;
+		D.m_counters = T.counters;
+	}
+	for (int q = 0; q < n_jobs; ++q) {
 		if (fused[q] < 0) continue;
 		JobDev &D = prep[q].jd;
 		const JobDev &M = prep[fused[q]].jd;
@@ -2734,26 +2807,32 @@ int build_plan(twopcf_ctx *ctx, const twopcf_job *jobs, int n_jobs, int method, 
 		}
 		return L;
 	};
-	auto launch = [&](int kind, bool weighted, bool mid, bool fz, int rg, const WorkItem *it, int n_it, int h_elems,
+	auto launch = [&](int kind, bool weighted, bool mid, int fz, int rg, const WorkItem *it, int n_it, int h_elems,
 	                  unsigned *queue, int grid, int mask_bytes) -> int {
 		const bool fast_kernel = fast;
 		Lds L = layout(fast_kernel, weighted, h_elems, mask_bytes);
 		if ((size_t)L.total > ctx->lds_max)
 			return fail(ctx, TWOPCF_E_LIMIT, "histogram does not fit in LDS (" + std::to_string(L.total) + " B)");
 		KernelFn fn = nullptr;
-		if (fz && rg != RG_NONE) return fail(ctx, TWOPCF_E_LIMIT, "fused monopole with regions or a global histogram");
+		if (fz != FZ_NONE && rg != RG_NONE)
+			return fail(ctx, TWOPCF_E_LIMIT, "fused monopole or twin with regions or a global histogram");
 #define PICK(K, F, W) fn = k_count<K, F, W>
 #define PICK_FAST(K, W, M, FZ) fn = fast_kernel_of<K, W, M, FZ>(rg)
 #define PICK_PI(K, W, FZ) if (mid) PICK_FAST(K, W, true, FZ); else PICK_FAST(K, W, false, FZ)
 		if (fast) {
-			if (kind == K_MONO) { if (weighted) PICK_FAST(K_MONO, true, false, false); else PICK_FAST(K_MONO, false, false, false); }
-			else if (kind == K_S_MU && fz) {
+			if (kind == K_MONO) { if (weighted) PICK_FAST(K_MONO, true, false, FZ_NONE); else PICK_FAST(K_MONO, false, false, FZ_NONE); }
+			else if (kind == K_S_MU && fz == FZ_MONO) {
 				// s-mu jobs with a fused monopole
-				if (weighted) { PICK_PI(K_S_MU, true, true); } else { PICK_PI(K_S_MU, false, true); }
+				if (weighted) { PICK_PI(K_S_MU, true, FZ_MONO); } else { PICK_PI(K_S_MU, false, FZ_MONO); }
 			}
-			else if (kind == K_S_MU) { if (weighted) { PICK_PI(K_S_MU, true, false); } else { PICK_PI(K_S_MU, false, false); } }
-			else if (kind == K_SIGMA_PI) { if (weighted) { PICK_PI(K_SIGMA_PI, true, false); } else { PICK_PI(K_SIGMA_PI, false, false); } }
-			else { if (weighted) PICK_FAST(K_ANGULAR, true, false, false); else PICK_FAST(K_ANGULAR, false, false, false); }
+			else if (kind == K_S_MU) { if (weighted) { PICK_PI(K_S_MU, true, FZ_NONE); } else { PICK_PI(K_S_MU, false, FZ_NONE); } }
+			else if (kind == K_SIGMA_PI) { if (weighted) { PICK_PI(K_SIGMA_PI, true, FZ_NONE); } else { PICK_PI(K_SIGMA_PI, false, FZ_NONE); } }
+			else if (fz == FZ_TWIN) {
+				// weighted angular jobs with a twin
+				if (!weighted) return fail(ctx, TWOPCF_E_ARG, "twin of an unweighted job");
+				PICK_FAST(K_ANGULAR, true, false, FZ_TWIN);
+			}
+			else { if (weighted) PICK_FAST(K_ANGULAR, true, false, FZ_NONE); else PICK_FAST(K_ANGULAR, false, false, FZ_NONE); }
 		} else {
 			if (kind == K_MONO) { if (weighted) PICK(K_MONO, false, true); else PICK(K_MONO, false, false); }
 			else if (kind == K_S_MU) { if (weighted) PICK(K_S_MU, false, true); else PICK(K_S_MU, false, false); }
@@ -2776,7 +2855,7 @@ int build_plan(twopcf_ctx *ctx, const twopcf_job *jobs, int n_jobs, int method, 
 			while (k1 < items.size() && items[k1].job == q) ++k1;
 			int slots = 0;
 			for (size_t k = k0; k < k1; ++k) slots = std::max(slots, items[k].n_slots);
-			if ((rc = launch(hj[q].j->kind, prep[q].weighted, false, false, RG_ANY, d_items + k0, (int)(k1 - k0),
+			if ((rc = launch(hj[q].j->kind, prep[q].weighted, false, FZ_NONE, RG_ANY, d_items + k0, (int)(k1 - k0),
 			                 prep[q].f_elems + prep[q].nb * slots, nullptr, (int)(k1 - k0), 0))) {
 				release();
 				return rc;
@@ -2802,7 +2881,7 @@ int build_plan(twopcf_ctx *ctx, const twopcf_job *jobs, int n_jobs, int method, 
 		std::map<std::tuple<int, int, int>, std::vector<WorkItem>> cls;
 		for (auto &W : items)
 			cls[{hj[W.job].j->kind, prep[W.job].weighted ? 1 : 0,
-			     mid_of(W.job) + (fused[W.job] >= 0 ? 2 : 0) + (prep[W.job].jd.mstage > 0 ? 4 : 0) + 8 * rg_of(W.job)}]
+			     mid_of(W.job) + 2 * fz_of(W.job) + (prep[W.job].jd.mstage > 0 ? 8 : 0) + 16 * rg_of(W.job)}]
 			    .push_back(W);
 		if (cls.size() > 64) {
 			release();
@@ -2825,9 +2904,9 @@ int build_plan(twopcf_ctx *ctx, const twopcf_job *jobs, int n_jobs, int method, 
 			const int kind = std::get<0>(sp);
 			const bool weighted = std::get<1>(sp) != 0;
 			const int mid = std::get<4>(sp) & 1;
-			const bool fz = (std::get<4>(sp) & 2) != 0;   // s-mu jobs with a fused monopole
-			const bool st = (std::get<4>(sp) & 4) != 0;   // jobs with the LDS mask copy
-			const int rg = std::get<4>(sp) >> 3;            // histogram layout
+			const int fz = (std::get<4>(sp) >> 1) & 3;   // s-mu jobs with a fused monopole, angular jobs with a twin
+			const bool st = (std::get<4>(sp) & 8) != 0;   // jobs with the LDS mask copy
+			const int rg = std::get<4>(sp) >> 4;            // histogram layout
 			// per job: its largest slot count over the launch's items
 			std::vector<int> slots_of(n_jobs, 0);
 			for (size_t k = std::get<2>(sp); k < std::get<2>(sp) + std::get<3>(sp); ++k)
@@ -2835,7 +2914,7 @@ int build_plan(twopcf_ctx *ctx, const twopcf_job *jobs, int n_jobs, int method, 
 			int h_elems = 1, mask_bytes = 0;
 			for (int q = 0; q < n_jobs; ++q)
 				if (!absorbed[q] && hj[q].j->kind == kind && prep[q].weighted == weighted && mid_of(q) == mid &&
-				    (fused[q] >= 0) == fz && (prep[q].jd.mstage > 0) == st && rg_of(q) == rg) {
+				    fz_of(q) == fz && (prep[q].jd.mstage > 0) == st && rg_of(q) == rg) {
 					h_elems = std::max(h_elems, prep[q].f_elems + hj[q].nb * slots_of[q]);
 					mask_bytes = std::max(mask_bytes, prep[q].mask_bytes);
 				}
