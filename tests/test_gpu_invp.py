@@ -122,6 +122,49 @@ def test_angular_counts_all_subdivisions(oracle_lib, axis):
                 assert r["n_pairs"] == acc
 
 
+def test_twin_angular_jobs(oracle_lib):
+    """Weighted angular jobs of the same catalogues and axis without regions,
+    one with the products of the input weights and one with the products of
+    the weights (correlators_invpweight.cpp:901, 968 and 853), are counted in
+    one pass of the batched method, each pair binned into both histograms
+    (twins); a job with the same rule twice is a twin too.  Every histogram
+    and accepted-pair count equals the oracle's and the same job counted
+    alone, in both methods (sums to 1e-12: the order of addition differs)."""
+    from two_pcf_amd import lib as L
+    o = oracle_lib
+    D = make(L, 2500, 41)
+    R = make(L, 5000, 42)   # random masks: w differs from iw
+    for cat in (D, R):
+        cat["jk"][:] = 0    # one region: none
+    oc = {k: o.Catalog(v["x"], v["y"], v["z"], v["nx"], v["ny"], v["nz"], v["w"], v["jk"], iw=v["iw"], masks=v["masks"])
+          for k, v in (("d", D), ("r", R))}
+    ang = (0.0, float(1.0 - np.cos(0.03)), 16, 1.6)
+    oang = o.axis(*ang)
+    # the DR pair: twins of two rules; the first two DD jobs: twins of one
+    # rule; the last DD job: no partner left
+    spec = [("d", "r", L.W_ANG_IW), ("d", "r", L.W_ANG_W), ("d", None, L.W_ANG_IW), ("d", None, L.W_ANG_IW),
+            ("d", None, L.W_ANG_W)]
+    want = []
+    for a, b, m in spec:
+        wo = o.WOpt(m, N_RUNS, None, o.Axis())
+        want.append(o.count(3, oc[a], oc[b] if b else None, oang, None, 1, wopt=wo))
+    with L.Context(0) as ctx:
+        dc = {k: ctx.upload(v["x"], v["y"], v["z"], v["nx"], v["ny"], v["nz"], v["w"], v["jk"], iw=v["iw"],
+                            masks=v["masks"]) for k, v in (("d", D), ("r", R))}
+        jobs = [dict(kind=L.ANGULAR, a=dc[a], b=dc[b] if b else None, axes=[ang], n_jk=1, weight_mode=m,
+                     n_runs=N_RUNS) for a, b, m in spec]
+        for method in (0, 1):
+            print(f"[gpu] twin angular jobs method={method}", flush=True)
+            together, _ = ctx.count(jobs, method=method)
+            for j, r, (full, _sub, acc) in zip(jobs, together, want):
+                alone, _ = ctx.count([j], method=method)
+                assert r["n_pairs"] == acc == alone[0]["n_pairs"]
+                np.testing.assert_allclose(r["full"], full, rtol=1e-12, atol=0)
+                np.testing.assert_allclose(r["full"], alone[0]["full"], rtol=1e-12, atol=0)
+        # the twin's weights differ from its partner's: the sums must too
+        assert not np.allclose(together[0]["full"], together[1]["full"], rtol=1e-6)
+
+
 def test_dither_weights(oracle_lib):
     """Dither weights (_USE_DITHER_WEIGHTS, correlators_invpweight.cpp:62-70,
     749-757): the inverse-p DD numerator is popc(dither1 & dither2), 0
