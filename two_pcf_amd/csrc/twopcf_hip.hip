@@ -2142,10 +2142,12 @@ int build_plan(twopcf_ctx *ctx, const twopcf_job *jobs, int n_jobs, int method, 
 		// below max_r_sq.
 		H.rmax = std::max(H.rmax, std::sqrt((double)H.max_r_sq)) * (1.0 + 1e-6);
 		H.weighted = !(a->unit_w && (!b || b->unit_w)) || J.weight_mode != TWOPCF_W_PLAIN;
-		// A cross count is symmetric: the smaller catalogue takes the first
-		// side (the lanes), so that the larger one fills the staged tiles
-		// (fewer, fuller tiles per pair than the other way round).
-		H.swapped = b && b->n < a->n;
+		// A cross count is symmetric: in the batched method the smaller
+		// catalogue takes the first side (the lanes), so that the larger one
+		// fills the staged tiles (fewer, fuller tiles per pair than the other
+		// way round); the plain method, one block per item, is faster with the
+		// larger one first.
+		H.swapped = b && (fast ? b->n < a->n : b->n > a->n);
 		H.a = H.swapped ? b : a;
 		H.b = H.swapped ? a : b;
 	}
