@@ -73,6 +73,10 @@ __host__ __device__ constexpr int tile_bytes(bool fast) {
 	            : TILE_EXACT * (6 * 8 + 4 + 4 + 4);
 }
 constexpr int MAX_SLOTS = 16;
+// item queues of a persistent launch: one per XCD (the dispatcher places
+// workgroup b on XCD b mod 8), each over a contiguous range of the items
+constexpr int NQ = 8;
+constexpr int MAX_CLASSES = 64;             // launch classes of one call
 // histogram layouts of the fp32 kernel's jobs (its RG template parameter)
 constexpr int RG_ANY = 0, RG_NONE = 1, RG_JK = 2;
 // a second job binned in the pass of another (fp32 path, its FZ template
@@ -1516,10 +1520,32 @@ __global__ void __launch_bounds__(BLOCK) k_count(const JobDev *__restrict__ jobs
 	HT *hist = (HT *)(smem + L.hist);
 	const int n_hist = ((L.masks ? L.masks : L.total) - L.hist) / (int)sizeof(HT);
 	for (int k = threadIdx.x; k < n_hist; k += BLOCK) hist[k] = (HT)0;
+	// Persistent launch: NQ queues over contiguous ranges of the items; a
+	// block takes items from its own XCD's range first, so that the items an
+	// XCD runs at one time are neighbours sharing their cells in its L2, and
+	// then from the other ranges in turn.  Every block leaves when all are
+	// drained.
+	// Range q holds (n_items - q + nq - 1) / nq items (item_ranges()).
+	const int nq = gridDim.x >= NQ && n_items >= NQ ? NQ : 1;
+	const int q_base = n_items / nq, q_extra = n_items % nq;
+	int tried = 0;   // (thread 0) ranges found drained
 	int it = blockIdx.x;
 	for (;;) {
 		if (queue) {
-			if (threadIdx.x == 0) s_item = (int)atomicAdd(queue, 1u);
+			if (threadIdx.x == 0) {
+				int got = n_items;
+				while (tried < nq) {
+					const int q = (int)((blockIdx.x + tried) % nq);
+					const int lo = q * q_base + min(q, q_extra), hi = lo + q_base + (q < q_extra ? 1 : 0);
+					const int k = lo + (int)atomicAdd(queue + q, 1u);
+					if (k < hi) {
+						got = k;
+						break;
+					}
+					++tried;
+				}
+				s_item = got;
+			}
 			__syncthreads();
 			it = __builtin_amdgcn_readfirstlane(s_item);
 			__syncthreads();
@@ -2689,7 +2715,7 @@ int build_plan(twopcf_ctx *ctx, const twopcf_job *jobs, int n_jobs, int method, 
 	    (rc = alloc((void **)&d_pool_u, sizeof(unsigned long long) * std::max<size_t>(pool_u, 1))) ||
 	    (rc = alloc((void **)&d_pool_d, sizeof(double) * std::max<size_t>(pool_d, 1))) ||
 	    (rc = alloc((void **)&d_cnt, sizeof(unsigned long long) * 2 * std::max(n_jobs, 1))) ||
-	    (rc = alloc((void **)&d_queue, sizeof(unsigned int) * 64))) {
+	    (rc = alloc((void **)&d_queue, sizeof(unsigned int) * NQ * MAX_CLASSES))) {
 		release();
 		return rc;
 	}
@@ -2885,7 +2911,7 @@ int build_plan(twopcf_ctx *ctx, const twopcf_job *jobs, int n_jobs, int method, 
 			cls[{hj[W.job].j->kind, prep[W.job].weighted ? 1 : 0,
 			     mid_of(W.job) + 2 * fz_of(W.job) + (prep[W.job].jd.mstage > 0 ? 8 : 0) + 16 * rg_of(W.job)}]
 			    .push_back(W);
-		if (cls.size() > 64) {
+		if (cls.size() > MAX_CLASSES) {
 			release();
 			return fail(ctx, TWOPCF_E_LIMIT, "more than 64 launch classes in one call");
 		}
@@ -2894,8 +2920,18 @@ int build_plan(twopcf_ctx *ctx, const twopcf_job *jobs, int n_jobs, int method, 
 		std::vector<std::tuple<int, int, size_t, size_t, int>> spans;
 		for (auto &kv : cls) {
 			spans.emplace_back(std::get<0>(kv.first), std::get<1>(kv.first), off, kv.second.size(), std::get<2>(kv.first));
-			flat.insert(flat.end(), kv.second.begin(), kv.second.end());
-			off += kv.second.size();
+			const size_t n_it = kv.second.size();
+			if (std::get<0>(kv.first) == K_ANGULAR && n_it >= (size_t)NQ) {
+				// angular items are ordered by cost: dealt to the launch's NQ
+				// queue ranges in turn, so that each range runs its items
+				// largest first and the ranges carry about the same cost
+				const size_t base = n_it / NQ, extra = n_it % NQ;
+				for (size_t q = 0; q < (size_t)NQ; ++q)
+					for (size_t j = 0; j < base + (q < extra ? 1 : 0); ++j) flat.push_back(kv.second[j * NQ + q]);
+			} else {
+				flat.insert(flat.end(), kv.second.begin(), kv.second.end());
+			}
+			off += n_it;
 		}
 		if ((rc = h2d(d_items, flat.data(), sizeof(WorkItem) * flat.size()))) {
 			release();
@@ -2927,7 +2963,7 @@ int build_plan(twopcf_ctx *ctx, const twopcf_job *jobs, int n_jobs, int method, 
 			const int per_cu = std::max(1, std::min(8, (int)(ctx->lds_max / std::max(L.total, 1))));
 			const int grid = std::min<long long>((long long)ctx->n_cu * per_cu, (long long)std::get<3>(sp));
 			if ((rc = launch(kind, weighted, mid != 0, fz, rg, d_items + std::get<2>(sp), (int)std::get<3>(sp), h_elems,
-			                 d_queue + qi, std::max(grid, 1), mask_bytes))) {
+			                 d_queue + NQ * qi, std::max(grid, 1), mask_bytes))) {
 				release();
 				return rc;
 			}
@@ -2980,7 +3016,7 @@ int enqueue_plan(twopcf_ctx *ctx, Plan &plan) {
 	fill(plan.d_pool_u, plan.pool_u);
 	fill((unsigned long long *)plan.d_pool_d, plan.pool_d);   // +0.0 has all bits clear
 	fill(plan.d_cnt, 2 * (size_t)plan.n_jobs);
-	fill((unsigned long long *)plan.d_queue, 32);
+	fill((unsigned long long *)plan.d_queue, NQ * MAX_CLASSES / 2);
 	HIPCK(ctx, hipGetLastError());
 	const int n = (int)plan.launches.size();
 	const int n_side = plan.concurrent ? std::min(N_SIDE, n - 1) : 0;
