@@ -43,8 +43,13 @@ FLOP_BY_KIND = {0: 15, 1: 29, 2: 26, 3: 15}
 # kernel source they were measured on: for any other source the line reports
 # no traffic.  Source: profiles/k_count_fp32_jackknife_batch_hbm.md
 KERNEL_SRC = os.path.join(ROOT, "two_pcf_amd", "csrc", "twopcf_hip.hip")
-TRAFFIC_BYTES = {("7154385d9f8d9581", 1.0, 100, 2, 9, 256): (3.454e7 * 2 + 2.532e6) * 1024}
-ALGO_BYTES_PER_PAIR = 0.5     # algorithmic HBM bytes per accepted pair (upper bound)
+TRAFFIC_BYTES = {("6831bd0ce39c6737", 1.0, 100, 2, 9, 256): (4.3131e7 * 2 + 1.9598e6) * 1024}
+# algorithmic HBM bytes of the counting kernel: every object a work item
+# reads (its first-side objects and its stencil rows' second-side objects,
+# the library's n_staged) once, at the bytes one object takes in the SoA:
+# x, y, z, nx, ny, nz (f64), plus the region (i32) when there are regions
+# and the weight (f32) when weighted
+OBJ_BYTES_POS = 48
 # region slots that fit in LDS on the default workload (2000-bin u32
 # histograms next to the 1024-entry tile, queues and tables)
 SLOTS_FIT = 9
@@ -240,6 +245,7 @@ def workload_jackknife(args, L, ctx):
               "n_data": int(cats["data"]["x"].size), "n_randoms": int(cats["randoms"]["x"].size),
               "scale": args.scale, "n_jk": n_jk}
     return dict(jobs=jobs, upload_ms=upload_ms, config=config, traffic_key=(args.scale, n_jk),
+                obj_bytes=OBJ_BYTES_POS + (4 if n_jk > 1 else 0),
                 cpu=lambda: cpu_baseline_jackknife(cats, n_jk, args.cpu_frac))
 
 
@@ -605,7 +611,10 @@ def main():
                                         f"{src_sha}), profiles/k_count_fp32_jackknife_batch_hbm.md" if traffic else
                                         f"not measured for this kernel source (sha256 {src_sha}) and workload"),
                      "traffic_bytes_per_pair": traffic / pairs if traffic else None,
-                     "algorithmic_bytes_per_pair": ALGO_BYTES_PER_PAIR,
+                     "algorithmic_bytes_per_pair": (t["n_staged"] * W["obj_bytes"] / pairs
+                                                    if W.get("obj_bytes") and t.get("n_staged") else None),
+                     "algorithmic_bytes": t["n_staged"] * W["obj_bytes"] if W.get("obj_bytes") and t.get("n_staged")
+                                          else None,
                      "note": "FP32 compute bound (pair screening on the matrix cores, binning on the vector "
                              "ALUs; both FP32 peaks are 157.3 TFLOP/s); algorithmic flop per accepted pair: "
                              "s-mu 26, sigma-pi 29, monopole and angular 15"},

@@ -32,7 +32,7 @@
 extern "C" {
 #endif
 
-#define TWOPCF_ABI_VERSION 4
+#define TWOPCF_ABI_VERSION 5
 
 enum {
 	TWOPCF_OK = 0,
@@ -144,6 +144,8 @@ typedef struct {
 	int32_t n_work_items;
 	uint64_t n_pairs;        /* accepted pairs over all jobs   */
 	uint64_t n_rechecked;    /* exact re-evaluations           */
+	uint64_t n_staged;       /* objects the work items read from the catalogues: each item's
+	                            first-side objects and its stencil rows' second-side objects */
 } twopcf_timing;
 
 /* Tuning knobs of TWOPCF_FP32_RECHECK.  A zero field takes the default

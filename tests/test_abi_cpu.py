@@ -33,7 +33,7 @@ def test_exports_every_declared_symbol(hip):
     for n in names:
         assert hasattr(L, n), n
     assert sorted(hip.EXPORTS) == names
-    assert L.twopcf_abi_version() == hip.ABI_VERSION == 4
+    assert L.twopcf_abi_version() == hip.ABI_VERSION == 5
 
 
 def test_null_arguments_are_rejected(hip):

@@ -1503,9 +1503,9 @@ __device__ void process_item_fast(const JobDev &J, const WorkItem &W, const int4
 	__syncthreads();
 }
 
-// One launch: item = blockIdx.x (plain) or pulled from a global queue
-// (persistent, queue != nullptr).  Every block leaves when the queue is
-// drained, so the grid always finishes.
+// One launch: item = blockIdx.x (plain) or pulled from the queues (persistent,
+// queue != nullptr: NQ counters over ranges of the items).  Every block
+// leaves when every range is drained, so the grid always finishes.
 template <int KIND, bool FAST, bool WEIGHTED, bool MID = false, int FZ = FZ_NONE, int RG = 0>
 __global__ void __launch_bounds__(BLOCK) k_count(const JobDev *__restrict__ jobs, const WorkItem *__restrict__ items,
                                                  int n_items, unsigned int *queue, const int4 *__restrict__ rows,
@@ -1611,6 +1611,7 @@ struct Plan {
 	unsigned int *d_queue = nullptr;
 	size_t pool_u = 0, pool_d = 0;
 	int n_jobs = 0, n_tables = 0, n_items = 0;
+	uint64_t n_staged = 0;   // objects read by the work items: first sides and their stencil rows' second sides
 	double prep_ms = 0;
 	std::vector<PlanLaunch> launches;
 	bool concurrent = false;   // launches forked over the context's streams (batched method)
@@ -2228,6 +2229,7 @@ int build_plan(twopcf_ctx *ctx, const twopcf_job *jobs, int n_jobs, int method, 
 	std::vector<int> slot_pool;
 	std::vector<WorkItem> items;
 	std::vector<double> item_cost;
+	uint64_t n_staged = 0;
 	size_t pool_u = 0, pool_d = 0;
 	int max_tab_n1 = 0;
 	// table sizes are known up front: every job pushes at most 3 * (n1 + 1)
@@ -2668,6 +2670,7 @@ int build_plan(twopcf_ctx *ctx, const twopcf_job *jobs, int n_jobs, int method, 
 						}
 						mask |= 1ull << ri[g1].r;
 						cost += (double)ni * (ri[g1].je - ri[g1].jb);
+						n_staged += (uint64_t)(ri[g1].je - ri[g1].jb);
 					}
 					if (cost * 1.0 >= 4.0e9) return fail(ctx, TWOPCF_E_LIMIT, "work item too large for 32-bit counters");
 					WorkItem W;
@@ -2686,6 +2689,7 @@ int build_plan(twopcf_ctx *ctx, const twopcf_job *jobs, int n_jobs, int method, 
 					if (group_regions) slot_pool.insert(slot_pool.end(), regs.begin(), regs.end());
 					items.push_back(W);
 					item_cost.push_back(cost);
+					n_staged += (uint64_t)ni;
 					g0 = g1;
 				}
 			}
@@ -2809,6 +2813,7 @@ int build_plan(twopcf_ctx *ctx, const twopcf_job *jobs, int n_jobs, int method, 
 	plan.n_jobs = n_jobs;
 	plan.n_tables = (int)tables.size();
 	plan.n_items = (int)items.size();
+	plan.n_staged = n_staged;
 	for (int q = 0; q < n_jobs; ++q)
 		plan.outs.push_back(PlanOut{prep[q].weighted, prep[q].full_off, prep[q].jk_off, prep[q].full_n, prep[q].jk_n,
 		                            fast && !prep[q].weighted && prep[q].jk_n > 0});
@@ -3085,6 +3090,7 @@ int read_results(twopcf_ctx *ctx, const Plan &plan_, const twopcf_job *jobs, int
 		timing->n_work_items = plan->n_items;
 		timing->n_pairs = tot_acc;
 		timing->n_rechecked = tot_rc;
+		timing->n_staged = plan->n_staged;
 	}
 	return 0;
 }

@@ -15,7 +15,7 @@ MONOPOLE, SIGMA_PI, S_MU, ANGULAR = 0, 1, 2, 3
 PLAIN_FP64, FP32_RECHECK = 0, 1
 W_PLAIN, W_INVP_3D, W_ANGUP_3D, W_ANG_IW, W_ANG_INVP, W_ANG_W = range(6)
 PI_ANGULAR, PI_MIDPOINT = 0, 1
-ABI_VERSION = 4   # TWOPCF_ABI_VERSION of the structures below
+ABI_VERSION = 5   # TWOPCF_ABI_VERSION of the structures below
 
 EXPORTS = [
     "twopcf_abi_version", "twopcf_ctx_create", "twopcf_ctx_destroy", "twopcf_last_error",
@@ -48,7 +48,7 @@ class JobDesc(C.Structure):
 class Timing(C.Structure):
     _fields_ = [("kernel_ms", C.c_double), ("prep_ms", C.c_double), ("total_ms", C.c_double),
                 ("n_launches", C.c_int32), ("n_work_items", C.c_int32),
-                ("n_pairs", C.c_uint64), ("n_rechecked", C.c_uint64)]
+                ("n_pairs", C.c_uint64), ("n_rechecked", C.c_uint64), ("n_staged", C.c_uint64)]
 
 
 class Tuning(C.Structure):
