@@ -790,7 +790,8 @@ __device__ void process_item_fast(const JobDev &J, const WorkItem &W, const int4
 	int *rq_g = (int *)((unsigned short *)(ctl + ICHUNK) + (BLOCK / WAVE) * RING) + wave * RQ;
 	unsigned char *rq_l = (unsigned char *)((int *)((unsigned short *)(ctl + ICHUNK) + (BLOCK / WAVE) * RING) +
 	                                        (BLOCK / WAVE) * RQ) + wave * RQ;
-	// region -> code (s * nb for LDS slot s, else -(region + 1)) for this item
+	// region -> code (the LDS byte address of slot s's histogram for LDS slot
+	// s, else -(region + 1)) for this item
 	int *rmap = (int *)((unsigned char *)((int *)((unsigned short *)(ctl + ICHUNK) + (BLOCK / WAVE) * RING) +
 	                                      (BLOCK / WAVE) * RQ) + (BLOCK / WAVE) * RQ);
 	// z of the first and of the last entry of each 16-entry tile block
@@ -810,6 +811,11 @@ __device__ void process_item_fast(const JobDev &J, const WorkItem &W, const int4
 	const bool full_g = RG == RG_ANY && J.full_global != 0;
 	HT *F = (HT *)(smem + L.hist);
 	HT *H = F + (full_g ? 0 : (nb << f_shift));
+	// region codes of slots are LDS byte addresses: slot s's bin b is at
+	// code + (b << hb), one address operation per atomic
+	constexpr int hb = WEIGHTED ? 3 : 2;
+	const int h_lds = (int)(uintptr_t)(HT LDS_AS *)H;
+	const int slot_bytes = nb << hb;
 
 	for (int k = tid; k < n_slots; k += BLOCK) sreg[k] = slot_regions[W.slot_base + k];
 	const bool use_map = jk && J.n_jk <= REGION_MAP;
@@ -821,7 +827,7 @@ __device__ void process_item_fast(const JobDev &J, const WorkItem &W, const int4
 		if (use_map) return rmap[r];
 		int code = -(r + 1);
 		for (int k = 0; k < n_slots; ++k)
-			if (sreg[k] == r) code = k * nb;
+			if (sreg[k] == r) code = h_lds + k * slot_bytes;
 		return code;
 	};
 	// below s2_lo: certainly out of range; at or above s2_hi: certainly out
@@ -862,7 +868,7 @@ __device__ void process_item_fast(const JobDev &J, const WorkItem &W, const int4
 	}
 	__syncthreads();
 	if (use_map) {
-		for (int k = tid; k < n_slots; k += BLOCK) rmap[sreg[k]] = k * nb;
+		for (int k = tid; k < n_slots; k += BLOCK) rmap[sreg[k]] = h_lds + k * slot_bytes;
 		__syncthreads();
 	}
 	if (tid < n_rbk * 16) {
@@ -914,14 +920,26 @@ __device__ void process_item_fast(const JobDev &J, const WorkItem &W, const int4
 
 	const float2 *tq = (const float2 *)(tab + J.tq_off);
 
+	// accepted and rechecked pairs: per lane (the recheck) and per wave
+	// (binning, counted from its lane masks on the scalar unit)
 	unsigned int n_acc = 0, n_rc = 0, n_acc2 = 0;
+	unsigned int n_acc_w = 0, n_acc2_w = 0;
 
-	// Region codes: s * nb for LDS slot s, -(region + 1) for a region without
-	// a slot in this item (its sums go straight to global memory).
+	// an LDS atomic at byte address a
+	auto lds_add = [](int a, float w) {
+		if (WEIGHTED)
+			__hip_atomic_fetch_add((double LDS_AS *)(uintptr_t)(unsigned)a, (double)w, __ATOMIC_RELAXED,
+			                       __HIP_MEMORY_SCOPE_WORKGROUP);
+		else
+			__hip_atomic_fetch_add((unsigned int LDS_AS *)(uintptr_t)(unsigned)a, 1u, __ATOMIC_RELAXED,
+			                       __HIP_MEMORY_SCOPE_WORKGROUP);
+	};
+	// Region codes: the histogram's LDS byte address for a region with an LDS
+	// slot, -(region + 1) for a region without a slot in this item (its sums
+	// go straight to global memory).
 	auto add_region = [&](int code, int bin, float w) {
 		if (code >= 0) {
-			if (WEIGHTED) atomicAdd((double *)&H[code + bin], (double)w);
-			else atomicAdd((unsigned int *)&H[code + bin], 1u);
+			lds_add(code + (bin << hb), w);
 		} else {
 			const size_t o = (size_t)(-code - 1) * nb + bin;
 			if (WEIGHTED) gadd(&J.jk_d[o], (double)w);
@@ -952,11 +970,9 @@ __device__ void process_item_fast(const JobDev &J, const WorkItem &W, const int4
 		if (jk) {
 			if (all_slotted || __builtin_amdgcn_ballot_w64(si < 0 || sj < 0) == 0ull) {
 				// every active lane's regions have LDS slots
-				if (WEIGHTED) atomicAdd((double *)&H[si + bin], (double)w);
-				else atomicAdd((unsigned int *)&H[si + bin], 1u);
+				lds_add(si + (bin << hb), w);
 				if (sj != si) {
-					if (WEIGHTED) atomicAdd((double *)&H[sj + bin], (double)w);
-					else atomicAdd((unsigned int *)&H[sj + bin], 1u);
+					lds_add(sj + (bin << hb), w);
 					if (cross_only) add_full(bin, w);
 				}
 			} else {
@@ -1083,10 +1099,12 @@ __device__ void process_item_fast(const JobDev &J, const WorkItem &W, const int4
 	// The FP32 decision for the pair of ring slot idx: bin, whether it is
 	// in range, and whether the margins prove both (sure); the ring holds
 	// pairs of the current tile, whose first entry is object t_base of the
-	// second side.
+	// second side.  The decisions are kept as lane masks (scalar registers):
+	// the lanes whose pair is not sure, and those whose pair is in range.
+	// Every lane of the wave evaluates a pair (uniform control flow).
 	int t_base = 0;
 	struct PairEval {
-		bool sure, in_range;
+		unsigned long long m_ns, m_in;
 		int bin, bin2, l2, q;   // bin2: the fused monopole's bin
 		float wp;     // product of the staged weights
 		int si, sj;   // region codes of the two objects
@@ -1100,9 +1118,12 @@ __device__ void process_item_fast(const JobDev &J, const WorkItem &W, const int4
 		r.sj = WEIGHTED ? tslot[q] : __float_as_int(b.w);
 		const float dx = ia.x - a.x, dy = ia.y - a.y, dz = ia.z - a.z;
 		const float s2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
-		bool sure, in_range = true;
+		// a ballot of each comparison is that comparison's own lane mask; the
+		// masks are combined on the scalar unit
+		unsigned long long m_ns, m_in = ~0ull;
 		int bin;
 		if constexpr (KIND == K_SIGMA_PI) {
+			bool sure, in_range = true;
 			// range test on s^2, sigma in the x = s^2 - pi^2 domain
 			// against sqrtf-domain thresholds, |pi| against its value
 			// thresholds; margins per pair from the bounds on s^2 and pi
@@ -1133,6 +1154,8 @@ __device__ void process_item_fast(const JobDev &J, const WorkItem &W, const int4
 				sure = true;   // from the screening's error shell: certainly out of range
 				in_range = false;
 			}
+			m_ns = __builtin_amdgcn_ballot_w64(!sure);
+			m_in = __builtin_amdgcn_ballot_w64(in_range);
 			bin = __mul24(k1, n2) + k2;
 		} else {
 			// bin: closed-form estimate, checked against the exact thresholds,
@@ -1144,7 +1167,8 @@ __device__ void process_item_fast(const JobDev &J, const WorkItem &W, const int4
 			// the recheck drops them, which costs less than a range test on
 			// every pair.  With a fused monopole the screening range is the
 			// union of two: pairs out of the s-mu range are dropped here.
-			in_range = !FUSE || (s2 < s2_hi && s2 >= s2_lo_eff);
+			if constexpr (FUSE)
+				m_in = __builtin_amdgcn_ballot_w64(s2 < s2_hi) & __builtin_amdgcn_ballot_w64(s2 >= s2_lo_eff);
 			// s-mu takes s for its estimate as s2 rsq(s2).  At s2 = 0 that is
 			// 0 * inf = NaN: the estimate clamps to a valid bin, whose lower
 			// threshold T_k + E > 0 fails, and mu is NaN, so the pair is never
@@ -1155,7 +1179,8 @@ __device__ void process_item_fast(const JobDev &J, const WorkItem &W, const int4
 			                                   : __builtin_amdgcn_sqrtf(s2) - s_min;
 			const int k = bin_estimate(sv, s_inv, s_ilb, s_ils, n1_top);
 			const float2 q = tq[k];
-			sure = (s2 >= q.x) && (s2 < q.y);   // T_k + E <= s2 < T_k+1 - E
+			// sure: T_k + E <= s2 < T_k+1 - E
+			m_ns = __builtin_amdgcn_ballot_w64(!(s2 >= q.x)) | __builtin_amdgcn_ballot_w64(!(s2 < q.y));
 			if (KIND == K_S_MU) {
 				const float nmu = (float)n2;
 				// mu n2 = |pi| rsq(s2) n2: the midpoint pi times n2, or for the
@@ -1179,7 +1204,7 @@ __device__ void process_item_fast(const JobDev &J, const WorkItem &W, const int4
 				if constexpr (decltype(mid)::value) hm -= nmu * mid_c * inv_mid;
 				const int km = (int)xm;
 				const float fr = __builtin_amdgcn_fractf(xm);
-				sure = sure && (fabsf(fr - 0.5f) <= hm);
+				m_ns |= __builtin_amdgcn_ballot_w64(!(fabsf(fr - 0.5f) <= hm));
 				bin = __mul24(k, n2) + km;
 				if constexpr (FUSE) {
 					// the monopole bin from the same s: a pair goes to the
@@ -1187,24 +1212,24 @@ __device__ void process_item_fast(const JobDev &J, const WorkItem &W, const int4
 					// decisions are rechecked
 					const int k2 = bin_estimate(sv, m_inv, m_ilb, m_ils, m_top);
 					const float2 qm = mtq[k2];
-					sure = sure && s2 >= qm.x && s2 < qm.y;
+					m_ns |= __builtin_amdgcn_ballot_w64(!(s2 >= qm.x)) | __builtin_amdgcn_ballot_w64(!(s2 < qm.y));
 					r.bin2 = k2;
 				}
 			} else {
 				bin = k;
 			}
-			if constexpr (FUSE) sure = sure || !in_range;
+			// out of the s-mu range: certainly not counted
+			if constexpr (FUSE) m_ns &= m_in;
 		}
-		r.sure = sure;
-		r.in_range = in_range;
+		r.m_ns = m_ns;
+		r.m_in = m_in;
 		r.bin = bin;
 		return r;
 	};
 	// bin n <= 2 * WAVE queued pairs, two per lane (independent chains for
 	// the scheduler to interleave); pairs the margins cannot decide go to the
 	// recheck queue
-	auto push = [&](bool c, const PairEval &p) {
-		const unsigned long long mr = __builtin_amdgcn_ballot_w64(c);
+	auto push = [&](unsigned long long mr, const PairEval &p) {
 		if (mr) {
 			if (__builtin_amdgcn_inverse_ballot_w64(mr)) {
 				const unsigned off = __builtin_amdgcn_mbcnt_hi((unsigned)(mr >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)mr, rq_n));
@@ -1228,16 +1253,16 @@ __device__ void process_item_fast(const JobDev &J, const WorkItem &W, const int4
 				const unsigned e = ring[(head + h + lane) & (RING - 1)];
 				const int l = ig * WAVE + (int)(e & 63u), q = min((int)(e >> 6), TILE - 1);
 				const PairEval p = eval(mid, l, q, cia[l], cib[l], tfa[q], tfb[q]);
-				if (v && p.sure && p.in_range) {
-					++n_acc;
+				const unsigned long long m_v = __builtin_amdgcn_ballot_w64(v);
+				const unsigned long long m_acc = m_v & p.m_in & ~p.m_ns, m_rc = m_v & p.m_ns;
+				if (__builtin_amdgcn_inverse_ballot_w64(m_acc)) {
 					const float w = weight(p.l2, t_base + p.q, p.wp);
 					add(p.bin, w, p.si, p.sj);
-					if constexpr (FUSE) {
-						++n_acc2;
-						add2(p.bin2, w);
-					}
+					if constexpr (FUSE) add2(p.bin2, w);
 				}
-				push(v && !p.sure, p);
+				n_acc_w += (unsigned)__popcll(m_acc);
+				if constexpr (FUSE) n_acc2_w += (unsigned)__popcll(m_acc);
+				push(m_rc, p);
 				__builtin_amdgcn_wave_barrier();
 			}
 		} else {
@@ -1264,24 +1289,32 @@ __device__ void process_item_fast(const JobDev &J, const WorkItem &W, const int4
 			PairEval p[NP];
 #pragma unroll
 			for (int k = 0; k < NP; ++k) p[k] = eval(mid, l[k], q[k], ia[k], ib[k], a[k], b[k]);
+			// the lanes whose pair is accepted and those whose pair goes to the
+			// recheck, taken where the decisions are made: the branches and the
+			// counts below use the masks (scalar registers)
+			unsigned long long m_acc[NP], m_rc[NP];
 #pragma unroll
-			for (int k = 0; k < NP; ++k)
-				if (lane + k * WAVE < (int)n && p[k].sure && p[k].in_range) {
-					++n_acc;
+			for (int k = 0; k < NP; ++k) {
+				// lanes with a queued pair (n is uniform)
+				const unsigned nk = n - min(n, (unsigned)(k * WAVE));
+				const unsigned long long m_v = nk >= (unsigned)WAVE ? ~0ull : (1ull << nk) - 1ull;
+				m_acc[k] = m_v & p[k].m_in & ~p[k].m_ns;
+				m_rc[k] = m_v & p[k].m_ns;
+			}
+#pragma unroll
+			for (int k = 0; k < NP; ++k) {
+				if (__builtin_amdgcn_inverse_ballot_w64(m_acc[k])) {
 					const float w = weight(p[k].l2, t_base + p[k].q, p[k].wp);
 					add(p[k].bin, w, p[k].si, p[k].sj);
-					if constexpr (FUSE) {
-						++n_acc2;
-						add2(p[k].bin2, w);
-					}
-					if constexpr (TWIN) {
-						// the same bin; the twin's weight from its staged pair
-						++n_acc2;
-						add2(p[k].bin, J.t_w ? __int_as_float(p[k].si) * __int_as_float(p[k].sj) : p[k].wp);
-					}
+					if constexpr (FUSE) add2(p[k].bin2, w);
+					// a twin: the same bin, its weight from its staged pair
+					if constexpr (TWIN) add2(p[k].bin, J.t_w ? __int_as_float(p[k].si) * __int_as_float(p[k].sj) : p[k].wp);
 				}
+				n_acc_w += (unsigned)__popcll(m_acc[k]);
+				if constexpr (FUSE || TWIN) n_acc2_w += (unsigned)__popcll(m_acc[k]);
+			}
 #pragma unroll
-			for (int k = 0; k < NP; ++k) push(lane + k * WAVE < (int)n && !p[k].sure, p[k]);
+			for (int k = 0; k < NP; ++k) push(m_rc[k], p[k]);
 		}
 		__builtin_amdgcn_wave_barrier();
 		__builtin_amdgcn_s_setprio(0);
@@ -1494,6 +1527,8 @@ __device__ void process_item_fast(const JobDev &J, const WorkItem &W, const int4
 		n_rc += __shfl_xor(n_rc, off);
 		if (FUSE || TWIN) n_acc2 += __shfl_xor(n_acc2, off);
 	}
+	n_acc += n_acc_w;
+	n_acc2 += n_acc2_w;
 	if (lane == 0) {
 		if (n_acc) gadd(&J.counters[0], (unsigned long long)n_acc);
 		if (n_rc) gadd(&J.counters[1], (unsigned long long)n_rc);
