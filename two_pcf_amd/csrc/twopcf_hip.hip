@@ -158,6 +158,9 @@ struct WorkItem {
 	unsigned long long row_mask;
 	int slot_base, n_slots;
 	int all_slotted;   // fp32 path: every region the item meets has an LDS slot
+	// fp32 path: the region of every first-side object of the item (its cell
+	// lies in one region), else -1
+	int uf_region;
 };
 
 }  // namespace
@@ -771,6 +774,12 @@ __device__ void process_item_fast(const JobDev &J, const WorkItem &W, const int4
 	// every region the item meets has an LDS slot: no lane's pair needs the
 	// global pool (the region test per pair is skipped)
 	const bool all_slotted = W.all_slotted != 0;
+	// Every first-side object of the item is in region R = W.uf_region: the
+	// full histogram (in LDS) then takes every pair, which is also region R's
+	// touching count of this item, and a pair adds to a slot histogram only
+	// when its second object lies in another region.  The item's R sums are
+	// the full histogram's, added at the end.
+	const bool uf = jk && W.uf_region >= 0;
 	// weights staged with the objects: the input weights for the rules built
 	// on them, else the weights
 	const int wmode = J.wmode;
@@ -966,6 +975,20 @@ __device__ void process_item_fast(const JobDev &J, const WorkItem &W, const int4
 		else atomicAdd((unsigned int *)fp, 1u);
 	};
 	auto add = [&](int bin, float w, int si, int sj) {
+		if (uf) {
+			add_full(bin, w);
+			if (sj != si) {
+				if (all_slotted || __builtin_amdgcn_ballot_w64(sj < 0) == 0ull) {
+					lds_add(sj + (bin << hb), w);
+				} else {
+					add_region(sj, bin, w);
+					// the cross count of a pair whose second region has no
+					// slot (slotted ones are summed from the slots at the end)
+					if (cross_only && sj < 0) gadd(&J.full_u[bin], 1ull);
+				}
+			}
+			return;
+		}
 		if (!cross_only) add_full(bin, w);
 		if (jk) {
 			if (all_slotted || __builtin_amdgcn_ballot_w64(si < 0 || sj < 0) == 0ull) {
@@ -1491,11 +1514,39 @@ __device__ void process_item_fast(const JobDev &J, const WorkItem &W, const int4
 			F[(b << f_shift) + r] = (HT)0;
 		}
 		if (v != (HT)0) {
-			if (WEIGHTED) gadd(&J.full_d[b], (double)v);
-			else gadd(&J.full_u[b], (unsigned long long)v);
+			if (uf) {
+				// every pair of the item: region R's touching count (and,
+				// weighted, the full sums)
+				const size_t o = (size_t)W.uf_region * nb + b;
+				if (WEIGHTED) {
+					gadd(&J.jk_d[o], (double)v);
+					gadd(&J.full_d[b], (double)v);
+				} else {
+					gadd(&J.jk_u[o], (unsigned long long)v);
+				}
+			} else if (WEIGHTED) {
+				gadd(&J.full_d[b], (double)v);
+			} else {
+				gadd(&J.full_u[b], (unsigned long long)v);
+			}
 		}
 	}
-	if (jk) {
+	if (uf && cross_only) {
+		// the slots hold the item's cross pairs, each once (its second
+		// region): their sum per bin is the item's cross count
+		for (int b = tid; b < nb; b += BLOCK) {
+			unsigned long long x = 0;
+			for (int s_ = 0; s_ < n_slots; ++s_) {
+				const HT v = H[s_ * nb + b];
+				if (v != (HT)0) {
+					gadd(&J.jk_u[(size_t)sreg[s_] * nb + b], (unsigned long long)v);
+					H[s_ * nb + b] = (HT)0;
+					x += (unsigned long long)v;
+				}
+			}
+			if (x) gadd(&J.full_u[b], x);
+		}
+	} else if (jk) {
 		for (int e = tid; e < n_slots * nb; e += BLOCK) {
 			HT v = H[e];
 			if (v != (HT)0) {
@@ -2672,11 +2723,16 @@ int build_plan(twopcf_ctx *ctx, const twopcf_job *jobs, int n_jobs, int method, 
 				ri.push_back({r, jb, je});
 			}
 			if (ri.empty()) continue;
+			// a cell in one region (fp32 path, full histogram in LDS): its
+			// items count every pair in the full histogram, which is also
+			// that region's touching count, and need no slot for it
+			const int uf_r = fast && group_regions && !D.full_global && A->reg_off[c + 1] - A->reg_off[c] == 1
+			                     ? A->reg_ids[A->reg_off[c]] : -1;
 			for (int i0 = ib; i0 < ie; i0 += i_chunk) {
 				const int ni = std::min(i_chunk, ie - i0);
 				// group rows by region budget
 				regs_i.clear();
-				if (group_regions) {
+				if (group_regions && uf_r < 0) {
 					// regions of this chunk: those present in its cell (a
 					// superset of the chunk's regions)
 					for (int k = A->reg_off[c]; k < A->reg_off[c + 1]; ++k) regs_i.push_back(A->reg_ids[k]);
@@ -2697,7 +2753,7 @@ int build_plan(twopcf_ctx *ctx, const twopcf_job *jobs, int n_jobs, int method, 
 							if (D.is_auto && row.x == 0 && row.y == 0) z0 = std::max(z0, cz);
 							for (int z = z0; z <= z1; ++z)
 								for (int k = B->reg_off[base + z]; k < B->reg_off[base + z + 1]; ++k)
-									if (std::find(regs.begin(), regs.end(), B->reg_ids[k]) == regs.end() &&
+									if (B->reg_ids[k] != uf_r && std::find(regs.begin(), regs.end(), B->reg_ids[k]) == regs.end() &&
 									    std::find(add.begin(), add.end(), B->reg_ids[k]) == add.end())
 										add.push_back(B->reg_ids[k]);
 							if ((int)(regs.size() + add.size()) > slot_cap && g1 > g0) break;
@@ -2716,9 +2772,10 @@ int build_plan(twopcf_ctx *ctx, const twopcf_job *jobs, int n_jobs, int method, 
 					W.row_mask = mask;
 					W.slot_base = (int)slot_pool.size();
 					// regs holds every region of the chunk's cell and of the
-					// rows' cells: when they all fit, every pair's two regions
-					// have slots
+					// rows' cells (but the cell's own with uf_r): when they
+					// all fit, every pair's two regions have slots
 					W.all_slotted = group_regions && (int)regs.size() <= slot_cap ? 1 : 0;
+					W.uf_region = uf_r;
 					if ((int)regs.size() > slot_cap) regs.resize(slot_cap);   // the rest: global atomics
 					W.n_slots = group_regions ? (int)regs.size() : 0;
 					if (group_regions) slot_pool.insert(slot_pool.end(), regs.begin(), regs.end());
