@@ -99,6 +99,47 @@ def test_jackknife_batch_vs_oracle(ctx, oracle_lib, method, slots):
         c.free()
 
 
+@pytest.mark.parametrize("weighted", [False, True])
+def test_regions_in_slabs(ctx, oracle_lib, weighted):
+    """Regions as slabs of x, 100 Mpc/h wide: most cells lie in one region
+    (the batched method then counts every pair of their items in the full
+    histogram, which is also that region's touching count) and the cells at
+    a boundary straddle two.  s-mu and monopole, DD and DR, unweighted and
+    weighted, both methods, with the default region slots and with one slot
+    (cross pairs of one-region items then go to regions without a slot),
+    against the oracle bit for bit."""
+    from two_pcf_amd import lib as L
+    o = oracle_lib
+    rng = np.random.default_rng(11)
+
+    def cat(n):
+        x, y, z = (rng.uniform(0.0, 400.0, size=(3, n)) + np.array([[1000.0], [-300.0], [200.0]]))
+        r = np.sqrt(x * x + y * y + z * z)
+        w = rng.uniform(0.5, 1.5, n).astype(np.float32) if weighted else np.ones(n, np.float32)
+        jk = np.floor((x - 1000.0) / 100.0).clip(0, 3).astype(np.int32)
+        return dict(x=x, y=y, z=z, nx=x / r, ny=y / r, nz=z / r, w=w, jk=jk)
+
+    cats = {"data": cat(15000), "randoms": cat(30000)}
+    dc = {t: ctx.upload(a["x"], a["y"], a["z"], a["nx"], a["ny"], a["nz"], a["w"] if weighted else None, a["jk"])
+          for t, a in cats.items()}
+    oc = {t: _oracle_cat(o, L, a) for t, a in cats.items()}
+    spec = (("data", None), ("data", "randoms"))
+    for kind, axes in ((2, [(0.0, 100.0, 40, 1.3), (0.0, 1.0, 50, 0.0)]), (0, [(0.0, 100.0, 30, 1.3)])):
+        ax1 = o.axis(*axes[0])
+        ax2 = o.axis(*axes[1]) if len(axes) > 1 else None
+        want = [o.count(kind, oc[a], oc[b] if b else None, ax1, ax2, 4) for a, b in spec]
+        for method, slots in ((0, None), (1, None), (1, 1)):
+            print(f"[gpu] slab regions weighted={weighted} kind={kind} method={method} slots={slots}", flush=True)
+            res, _ = ctx.count([dict(kind=kind, a=dc[a], b=dc[b] if b else None, axes=axes, n_jk=4) for a, b in spec],
+                               method=method, tuning={"max_slots": slots} if slots else None)
+            for r, w, (a, b) in zip(res, want, spec):
+                np.testing.assert_array_equal(r["full"], w[0], err_msg=f"kind {kind} {a}-{b or a} full")
+                np.testing.assert_array_equal(r["jk"], w[1], err_msg=f"kind {kind} {a}-{b or a} regions")
+                assert r["n_pairs"] == w[2]
+    for c_ in dc.values():
+        c_.free()
+
+
 def test_methods_agree_weighted_sigma_pi(ctx, oracle_lib):
     """Weighted sigma-pi, s-mu and monopole (weights U[0.5,1.5]), DD, DR and
     RR: both GPU methods equal the oracle bit for bit (partial sums are

@@ -1102,6 +1102,9 @@ __device__ void process_item_fast(const JobDev &J, const WorkItem &W, const int4
 	float mu_h0_v;
 	asm volatile("v_mov_b32 %0, %1" : "=v"(mu_h0_v) : "s"(mu_h0));
 	const float *x1t = tab + J.x1_off, *t2t = tab + J.t2_off;
+	// sigma-pi: the outer thresholds of both axes (uniform)
+	const float sp_x0 = KIND == K_SIGMA_PI ? x1t[0] : 0.0f, sp_xn = KIND == K_SIGMA_PI ? x1t[n1] : 0.0f;
+	const float sp_t0 = KIND == K_SIGMA_PI ? t2t[0] : 0.0f, sp_tn = KIND == K_SIGMA_PI ? t2t[n2] : 0.0f;
 	const float rA0 = tab[J.tA_off], rBn = tab[J.tB_off + n1];
 	// line of sight in FP32: the angular definition from the float unit
 	// vectors, or the pair midpoint from 2c + the two cell-local offsets;
@@ -1146,10 +1149,11 @@ __device__ void process_item_fast(const JobDev &J, const WorkItem &W, const int4
 		unsigned long long m_ns, m_in = ~0ull;
 		int bin;
 		if constexpr (KIND == K_SIGMA_PI) {
-			bool sure, in_range = true;
 			// range test on s^2, sigma in the x = s^2 - pi^2 domain
 			// against sqrtf-domain thresholds, |pi| against its value
-			// thresholds; margins per pair from the bounds on s^2 and pi
+			// thresholds; margins per pair from the bounds on s^2 and pi.
+			// Every threshold is loaded and every comparison made (no
+			// branches); the decisions are combined as lane masks.
 			float inv_mid;
 			const float p = fp32_pi(mid, dx, dy, dz, ia, ib, a, b, inv_mid);
 			const float ap = fabsf(p);
@@ -1158,27 +1162,29 @@ __device__ void process_item_fast(const JobDev &J, const WorkItem &W, const int4
 			if constexpr (decltype(mid)::value) dpi += mid_c * s * inv_mid;
 			const float x = fmaf(-p, p, s2);
 			const float ex = fmaf(J.e_c1, s2, J.e_c2 * s) + dpi * fmaf(2.0f, ap, dpi) + 3.6e-7f * fmaf(p, p, s2);
-			const bool r_in = s2 >= rA0 && s2 < rBn;   // min_r_sq < s2_ref < max_r_sq
 			const float sg = __builtin_amdgcn_sqrtf(fmaxf(x, 0.0f)) - s_min;
 			const int k1 = bin_estimate(sg, s_inv, s_ilb, s_ils, n1_top);
-			const bool sure1 = x >= x1t[k1] + ex && x < x1t[k1 + 1] - ex;
-			const bool out1 = x < x1t[0] - ex || x >= x1t[n1] + ex;
 			const float pv = ap - J.p_min;
 			const int k2 = bin_estimate(pv, p_inv, p_ilb, p_ils, n2_top);
-			const bool sure2 = ap >= t2t[k2] + dpi && ap < t2t[k2 + 1] - dpi;
-			const bool out2 = ap < t2t[0] - dpi || ap >= t2t[n2] + dpi;
-			if (r_in && (out1 || out2)) {
-				sure = true;   // in range but certainly outside the histogram
-				in_range = false;
-			} else {
-				sure = r_in && sure1 && sure2;
-			}
-			if (!(s2 < s2_hi && s2 >= s2_lo_eff)) {
-				sure = true;   // from the screening's error shell: certainly out of range
-				in_range = false;
-			}
-			m_ns = __builtin_amdgcn_ballot_w64(!sure);
-			m_in = __builtin_amdgcn_ballot_w64(in_range);
+			const float x_lo = x1t[k1], x_hi = x1t[k1 + 1], t_lo = t2t[k2], t_hi = t2t[k2 + 1];
+			// r_in: min_r_sq < s2_ref < max_r_sq
+			const unsigned long long m_rin = __builtin_amdgcn_ballot_w64(s2 >= rA0) & __builtin_amdgcn_ballot_w64(s2 < rBn);
+			// certainly outside the histogram's sigma or pi range
+			const unsigned long long m_out =
+			    __builtin_amdgcn_ballot_w64(x < sp_x0 - ex) | __builtin_amdgcn_ballot_w64(x >= sp_xn + ex) |
+			    __builtin_amdgcn_ballot_w64(ap < sp_t0 - dpi) | __builtin_amdgcn_ballot_w64(ap >= sp_tn + dpi);
+			// both bins proved
+			const unsigned long long m_sure12 =
+			    __builtin_amdgcn_ballot_w64(x >= x_lo + ex) & __builtin_amdgcn_ballot_w64(x < x_hi - ex) &
+			    __builtin_amdgcn_ballot_w64(ap >= t_lo + dpi) & __builtin_amdgcn_ballot_w64(ap < t_hi - dpi);
+			// within the screening's range (not from its error shell)
+			const unsigned long long m_shell =
+			    __builtin_amdgcn_ballot_w64(s2 < s2_hi) & __builtin_amdgcn_ballot_w64(s2 >= s2_lo_eff);
+			// in range: not from the shell and not in range but certainly
+			// outside the histogram; sure: out of range, or in range with
+			// both bins proved
+			m_in = m_shell & ~(m_rin & m_out);
+			m_ns = m_shell & ~(m_rin & (m_out | m_sure12));
 			bin = __mul24(k1, n2) + k2;
 		} else {
 			// bin: closed-form estimate, checked against the exact thresholds,
