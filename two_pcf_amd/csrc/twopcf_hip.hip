@@ -2689,6 +2689,21 @@ int build_plan(twopcf_ctx *ctx, const twopcf_job *jobs, int n_jobs, int method, 
 			P.f_elems = H.nb;
 			f_bytes = hist_bytes;
 		}
+		// Without regions the LDS beside the fixed part holds only the full
+		// histograms: a large one gets as many lane copies (a power of two, up
+		// to one per lane) as fit, so that lanes adding to one bin in the
+		// same atomic hit different addresses.
+		if (fast && !jk && P.mask_bytes == 0 && f_bytes <= avail) {
+			const long long es = H.weighted ? 8 : 4;
+			const long long other = f_bytes - ((long long)H.nb << D.f_shift) * es;   // a fused or twin job's
+			int fs = D.f_shift;
+			while (fs < 6 && (((long long)H.nb << (fs + 1)) * es) + other <= avail) ++fs;
+			if (fs > D.f_shift) {
+				P.f_elems += (H.nb << fs) - (H.nb << D.f_shift);
+				D.f_shift = fs;
+				f_bytes = (long long)P.f_elems * es;
+			}
+		}
 		if (f_bytes + (jk ? hist_bytes : 0) > avail) {
 			if (fz_of(q) != FZ_NONE) return fail(ctx, TWOPCF_E_LIMIT, "fused histograms do not fit in LDS");
 			D.full_global = 1;
