@@ -1279,7 +1279,7 @@ __device__ void process_item_fast(const JobDev &J, const WorkItem &W, const int4
 			// the midpoint's longer per-pair code: one pair per lane at a time
 			for (unsigned h = 0; h < n; h += WAVE) {
 				const bool v = h + lane < n;
-				const unsigned e = ring[(head + h + lane) & (RING - 1)];
+				const unsigned e = ring[(head & (RING - 1)) + h + lane];
 				const int l = ig * WAVE + (int)(e & 63u), q = min((int)(e >> 6), TILE - 1);
 				const PairEval p = eval(mid, l, q, cia[l], cib[l], tfa[q], tfb[q]);
 				const unsigned long long m_v = __builtin_amdgcn_ballot_w64(v);
@@ -1301,7 +1301,7 @@ __device__ void process_item_fast(const JobDev &J, const WorkItem &W, const int4
 			float4 ia[NP], ib[NP], a[NP], b[NP];
 #pragma unroll
 			for (int k = 0; k < NP; ++k) {
-				const unsigned e = ring[(head + k * WAVE + lane) & (RING - 1)];
+				const unsigned e = ring[(head & (RING - 1)) + k * WAVE + lane];
 				l[k] = ig * WAVE + (int)(e & 63u);
 				q[k] = min((int)(e >> 6), TILE - 1);
 			}
@@ -1350,7 +1350,11 @@ __device__ void process_item_fast(const JobDev &J, const WorkItem &W, const int4
 		if (rq_n >= WAVE) flush_rq();
 	};
 
-	// append the lanes of mask m with their ring entries
+	// append the lanes of mask m with their ring entries.  head and tail
+	// start at 0 with every tile, and head moves by NP * 64 except at the
+	// tile's end: every binning pass reads a window that starts at a multiple
+	// of NP * 64 and does not wrap around the ring.
+	static_assert(RING % (NP_MAX * WAVE) == 0, "binning windows do not wrap");
 	unsigned head = 0, tail = 0;
 	auto append = [&](unsigned long long m, unsigned entry) {
 		if (m == 0ull) return;
@@ -1459,12 +1463,14 @@ __device__ void process_item_fast(const JobDev &J, const WorkItem &W, const int4
 					const float4 tl = *(const float4 *)&ctl[o + 4 * lk];
 					// gj > gi  <=>  lane_d > i_begin + o + r - t0 - qb
 					const int ord0 = W.i_begin + ((rb * ig_n + ig) << 4) - t0;
-					// Software pipeline: the next block's operand is loaded and
-					// its product started while this block's masks are
-					// appended.  The tile's last block is padded with entries
-					// never in range; the loads past it feed only the unused
-					// last product and stay inside the tile arrays (the tile
-					// is followed by its normals, TILE entries of 16 B).
+					// The next block's operand is loaded while this block's
+					// product is formed and its masks appended (a product
+					// started a block ahead as well, which needs its result
+					// copied between registers every block, measured 1.7 %
+					// slower).  The tile's last block is padded with entries
+					// never in range; the load past it stays inside the tile
+					// arrays (the tile is followed by its normals, TILE
+					// entries of 16 B) and is not used.
 					auto b_load = [&](int qb) { return tff[4 * (qb + lrow) + lk]; };
 					const int step = jg_n * 16;
 					auto product = [&](float bv) {
@@ -1472,12 +1478,10 @@ __device__ void process_item_fast(const JobDev &J, const WorkItem &W, const int4
 					};
 					// this wave's blocks of the window: kb + first, then every jg_n-th
 					const int q0 = 16 * (kb + first), q1 = 16 * ke;
-					f32x4 d_next = product(b_load(q0));
-					float b_next = b_load(q0 + step);
+					float b_next = b_load(q0);
 					for (int qb = q0; qb < q1; qb += step) {
-						const f32x4 d = d_next;
-						d_next = product(b_next);
-						b_next = b_load(qb + 2 * step);
+						const f32x4 d = product(b_next);
+						b_next = b_load(qb + step);
 						const unsigned base = ((unsigned)qb << 6) + (unsigned)(rb * 16) + lane_code;
 						unsigned long long m[4];
 						m[0] = __builtin_amdgcn_ballot_w64(d[0] < th.x);
@@ -1502,10 +1506,8 @@ __device__ void process_item_fast(const JobDev &J, const WorkItem &W, const int4
 						}
 					}
 				}
-				if (tail != head) {
-					consume(head, tail - head);
-					head = tail;
-				}
+				if (tail != head) consume(head, tail - head);
+				head = tail = 0;
 			}
 			__syncthreads();
 		}
