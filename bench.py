@@ -37,7 +37,7 @@ FLOP_PER_PAIR = 26            # FP32 flop per accepted s-mu pair
 # log index 3, weight 1)
 FLOP_BY_KIND = {0: 15, 1: 29, 2: 26, 3: 15}
 # HBM bytes per launch of the counting kernel on the default workload (scale
-# 1, 100 regions, tuning (2, 12, 256) with 9 region slots fitting), from two
+# 1, 100 regions, tuning (2, 12, 256) with 8 region slots fitting), from two
 # rocprofv3 --pmc runs: FETCH_SIZE x 2 + WRITE_SIZE x 1 (factors from a 1 GiB
 # copy), KiB -> B.  Keyed by the first 16 hex digits of the SHA-256 of the
 # kernel source they were measured on: for any other source the line reports
@@ -51,8 +51,9 @@ TRAFFIC_BYTES = {("fa24611b030e0005", 1.0, 100, 2, 9, 256): (4.3626e7 * 2 + 1.93
 # and the weight (f32) when weighted
 OBJ_BYTES_POS = 48
 # region slots that fit in LDS on the default workload (2000-bin u32
-# histograms next to the 1024-entry tile, queues and tables)
-SLOTS_FIT = 9
+# histograms next to the 1024-entry tile, queues, tables and two copies of
+# the full histogram)
+SLOTS_FIT = 8
 PEAK_FP32_TFLOPS = 157.3      # MI355X dense FP32 peak (matrix cores; the vector peak is the same figure)
 AXES = [(0.0, 100.0, 40, 1.3), (0.0, 1.0, 50, 0.0)]
 
