@@ -538,17 +538,23 @@ __device__ float pair_weight(const JobDev &J, int gi, int gj, const double *ni, 
 // their float product wp (w1 * w2, or iw1 * iw2 for the input-weight rules):
 // the same operations in the same order.
 template <int CH>
-__device__ float pair_weight_staged(const JobDev &J, int gi, int gj, const double *ni, const double *nj, float wp,
-                                    const uint64_t LDS_AS *m1l, bool use_l) {
+__device__ float pair_weight_staged(const JobDev &J, int gi, int gj, float wp, const uint64_t LDS_AS *m1l,
+                                    bool use_l) {
+	// 1 - cos of the pair's unit vectors, read only by the rules that need it
+	auto one_minus_cos = [&]() {
+		const double ni[3] = {gld(J.A.nx)[gi], gld(J.A.ny)[gi], gld(J.A.nz)[gi]};
+		const double nj[3] = {gld(J.B.nx)[gj], gld(J.B.ny)[gj], gld(J.B.nz)[gj]};
+		return ref_one_minus_cos(ni, nj);
+	};
 	switch (J.wmode) {
 	case TWOPCF_W_INVP_3D: {
 		const float pc = popc_weight<CH>(J, gi, gj, m1l, use_l);
-		double t = interp_up(J, ref_one_minus_cos(ni, nj)) * (double)invp_numerator<CH>(J, gi, gj, m1l, use_l);
+		double t = interp_up(J, one_minus_cos()) * (double)invp_numerator<CH>(J, gi, gj, m1l, use_l);
 		t = t / (double)pc;
 		return wp * (float)t;
 	}
 	case TWOPCF_W_ANGUP_3D:
-		return (float)((double)wp * interp_up(J, ref_one_minus_cos(ni, nj)));
+		return (float)((double)wp * interp_up(J, one_minus_cos()));
 	case TWOPCF_W_ANG_INVP: {
 		const float pc = popc_weight<CH>(J, gi, gj, m1l, use_l);
 		return wp * (invp_numerator<CH>(J, gi, gj, m1l, use_l) / pc);
@@ -1021,9 +1027,7 @@ __device__ void process_item_fast(const JobDev &J, const WorkItem &W, const int4
 	auto weight = [&](int l2, int g2, float wp) {
 		if (WEIGHTED && needs_more) {
 			const int g1 = W.i_begin + item_of(l2);
-			const double ni[3] = {gld(J.A.nx)[g1], gld(J.A.ny)[g1], gld(J.A.nz)[g1]};
-			const double nj[3] = {gld(J.B.nx)[g2], gld(J.B.ny)[g2], gld(J.B.nz)[g2]};
-			wp = pair_weight_staged<KIND == K_S_MU ? 2 : 4>(J, g1, g2, ni, nj, wp, m_lds + (size_t)l2 * J.mstage,
+			wp = pair_weight_staged<KIND == K_S_MU ? 2 : 4>(J, g1, g2, wp, m_lds + (size_t)l2 * J.mstage,
 			                                                 masks_in_lds);
 		}
 		return wp;
