@@ -48,8 +48,11 @@ constexpr int TILE_FAST = 1024;             // staged second-side objects (fp32 
 constexpr int TILE_EXACT = 512;             // staged second-side objects (fp64 path)
 constexpr int RING = 512;                   // per-wave queue of in-range pairs, u16 entries (fp32 path)
 // pairs per lane in one binning pass of the fp32 path (three per lane
-// measured slower than two on the s-mu jackknife batch)
-__host__ __device__ constexpr int np_of(int, bool) { return 2; }
+// measured slower than two on the s-mu jackknife batch).  Sigma-pi bins one:
+// its decision (two axes, per-pair margins) holds more registers, and with two
+// pairs per lane the weighted kernel spilled (equatorial 155.3 -> 151.4 ms
+// with one).
+__host__ __device__ constexpr int np_of(int kind, bool) { return kind == K_SIGMA_PI ? 1 : 2; }
 constexpr int NP_MAX = 2;
 static_assert(RING >= NP_MAX * 64 + 4 * 64, "ring: pending pairs plus one screening step");
 static_assert(TILE_FAST <= BLOCK && TILE_FAST <= 1024, "tile: one entry per thread, 10-bit ring index");
@@ -1521,9 +1524,13 @@ __device__ void process_item_fast(const JobDev &J, const WorkItem &W, const int4
 
 	for (int b = tid; b < (full_g ? 0 : nb); b += BLOCK) {
 		HT v = (HT)0;
+		// (only the copies that hold something are cleared)
 		for (int r = 0; r < (1 << f_shift); ++r) {
-			v += F[(b << f_shift) + r];
-			F[(b << f_shift) + r] = (HT)0;
+			const HT x = F[(b << f_shift) + r];
+			if (x != (HT)0) {
+				v += x;
+				F[(b << f_shift) + r] = (HT)0;
+			}
 		}
 		if (v != (HT)0) {
 			if (uf) {
