@@ -422,13 +422,14 @@ def test_long_column_z_window(ctx, oracle_lib, kind):
         c.free()
 
 
-@pytest.mark.parametrize("mono", [(40, 1.3), (25, 1.6), (12, 1.0)], ids=["log40", "log25", "linear12"])
+@pytest.mark.parametrize("mono", [(40, 1.3), (30, 1.3), (25, 1.6), (12, 1.0)], ids=["log40", "log30", "log25", "linear12"])
 @pytest.mark.parametrize("name", ["cart_uniform", "eq_clustered", "edge", "cart_midpoint", "eq_midpoint"])
 def test_fused_monopole_s_mu(ctx, oracle_lib, name, mono):
     """A monopole over the same catalogues, weights and s range as an s-mu job
     is binned in the s-mu job's pass (one launch for DD, DR and RR of both).
-    Its bins differ from the s-mu's s bins (other count, base, or linear), so
-    each pair is proved in both histograms or rechecked in both; the golden
+    Its bins are the s-mu's s bins (40 bins, base 1.3), the upper 30 of them
+    (30 bins to the same s maximum), or differ from them (other base, or
+    linear); each pair is proved in both histograms or rechecked in both; the golden
     cases bring clustered and equatorial catalogues, edge separations,
     weights and the pair-midpoint line of sight.  Every histogram equals the
     oracle's count of that job alone; the plain method takes no fusion."""
