@@ -27,6 +27,7 @@
 #include <memory>
 #include <mutex>
 #include <string>
+#include <thread>
 #include <tuple>
 #include <type_traits>
 #include <vector>
@@ -2205,6 +2206,10 @@ int build_plan(twopcf_ctx *ctx, const twopcf_job *jobs, int n_jobs, int method, 
 		GridKey g;
 	};
 	std::vector<HostJob> hj(n_jobs);
+	// The float thresholds depend only on the kind and the axes, and each
+	// search checks a window of floats around every threshold (milliseconds
+	// per axis): jobs with the same kind and axes take the first one's.
+	std::map<std::string, int> axes_first;
 	for (int q = 0; q < n_jobs; ++q) {
 		const twopcf_job &J = jobs[q];
 		HostJob &H = hj[q];
@@ -2254,20 +2259,36 @@ int build_plan(twopcf_ctx *ctx, const twopcf_job *jobs, int n_jobs, int method, 
 		else H.ax2 = AxisF(0.0f, 1.0f, 1, 0.0f);
 		H.nb = H.ax1.n * (two_axes ? H.ax2.n : 1);
 		range_sq(J.kind, H.ax1, H.ax2, &H.min_r_sq, &H.max_r_sq);
+		std::string akey((const char *)&J.kind, sizeof J.kind);
+		for (const twopcf_axis &x : J.ax) {
+			akey.append((const char *)&x.min, sizeof x.min);
+			akey.append((const char *)&x.max, sizeof x.max);
+			akey.append((const char *)&x.n_bins, sizeof x.n_bins);
+			akey.append((const char *)&x.log_base, sizeof x.log_base);
+		}
+		const auto seen = axes_first.find(akey);
+		const HostJob *same = seen != axes_first.end() ? &hj[seen->second] : nullptr;
+		if (!same) axes_first.emplace(akey, q);
 		if (J.kind == TWOPCF_ANGULAR) {
 			// cells over the unit vectors, side = the largest chord whose
 			// 1 - cos can still land in the histogram
-			if (!value_thresholds(H.ax1, H.t1)) return fail(ctx, TWOPCF_E_AXIS, "axis index not monotone");
+			if (same) H.t1 = same->t1;
+			else if (!value_thresholds(H.ax1, H.t1)) return fail(ctx, TWOPCF_E_AXIS, "axis index not monotone");
 			const double vmax = std::max(0.0, (double)H.t1[H.ax1.n]);
 			H.rmax = std::sqrt(2.0 * std::min(vmax, 2.0) * (1.0 + 1e-6) + 1e-14);
 			H.max_r_sq = (float)(H.rmax * H.rmax);
 		} else if (J.kind == TWOPCF_SIGMA_PI) {
 			H.rmax = std::sqrt(J.ax[0].max * J.ax[0].max + J.ax[1].max * J.ax[1].max);
-			if (!value_thresholds(H.ax1, H.t1) || !value_thresholds(H.ax2, H.t2))
+			if (same) {
+				H.t1 = same->t1;
+				H.t2 = same->t2;
+			} else if (!value_thresholds(H.ax1, H.t1) || !value_thresholds(H.ax2, H.t2)) {
 				return fail(ctx, TWOPCF_E_AXIS, "axis index not monotone");
+			}
 		} else {
 			H.rmax = J.ax[0].max;
-			if (!s2_thresholds(J.kind, H.ax1, H.ax2, H.t_s2)) return fail(ctx, TWOPCF_E_AXIS, "axis index not monotone");
+			if (same) H.t_s2 = same->t_s2;
+			else if (!s2_thresholds(J.kind, H.ax1, H.ax2, H.t_s2)) return fail(ctx, TWOPCF_E_AXIS, "axis index not monotone");
 		}
 		if (!(H.rmax > 0) || !std::isfinite(H.rmax)) return fail(ctx, TWOPCF_E_ARG, "rmax must be positive");
 		// The range test is on float s^2; the grid must cover every float s^2
@@ -2744,94 +2765,129 @@ int build_plan(twopcf_ctx *ctx, const twopcf_job *jobs, int n_jobs, int method, 
 		const int d1 = H.g.dim[1], d2 = H.g.dim[2];
 		std::vector<int> rws;
 		for (int r = 0; r < D.n_rows; ++r) rws.push_back(r);
-		// scratch reused across cells and chunks (no allocation per row)
-		std::vector<int> regs_i, regs, add;
-		struct RowInfo {
-			int r, jb, je;
+		// The cells are split into contiguous ranges built on host threads;
+		// each range's items, slot lists and counts are then appended in cell
+		// order, so the item list is the one a single thread would build.
+		struct RangeOut {
+			std::vector<WorkItem> items;
+			std::vector<double> cost;
+			std::vector<int> slots;
+			uint64_t n_staged = 0;
+			bool too_large = false;
 		};
-		std::vector<RowInfo> ri;
-		for (int c = 0; c < n_cell; ++c) {
-			const int ib = A->start_h[c], ie = A->start_h[c + 1];
-			if (ib == ie) continue;
-			const int cx = c / (d1 * d2), cy = (c / d2) % d1, cz = c % d2;
-			// per row: j range, cost
-			ri.clear();
-			for (int r : rws) {
-				const int4 row = row_pool[D.row_off + r];
-				const int qx = cx + row.x, qy = cy + row.y;
-				if (qx < 0 || qy < 0 || qx >= H.g.dim[0] || qy >= d1) continue;
-				if (D.is_auto && (row.x < 0 || (row.x == 0 && row.y < 0))) continue;
-				int z0 = std::max(cz + row.z, 0), z1 = std::min(cz + row.w, d2 - 1);
-				if (D.is_auto && row.x == 0 && row.y == 0) z0 = std::max(z0, cz);
-				if (z0 > z1) continue;
-				const int base = (qx * d1 + qy) * d2;
-				const int jb = B->start_h[base + z0], je = B->start_h[base + z1 + 1];
-				if (jb == je) continue;
-				ri.push_back({r, jb, je});
-			}
-			if (ri.empty()) continue;
-			// a cell in one region (fp32 path, full histogram in LDS): its
-			// items count every pair in the full histogram, which is also
-			// that region's touching count, and need no slot for it
-			const int uf_r = fast && group_regions && !D.full_global && A->reg_off[c + 1] - A->reg_off[c] == 1
-			                     ? A->reg_ids[A->reg_off[c]] : -1;
-			for (int i0 = ib; i0 < ie; i0 += i_chunk) {
-				const int ni = std::min(i_chunk, ie - i0);
-				// group rows by region budget
-				regs_i.clear();
-				if (group_regions && uf_r < 0) {
-					// regions of this chunk: those present in its cell (a
-					// superset of the chunk's regions)
-					for (int k = A->reg_off[c]; k < A->reg_off[c + 1]; ++k) regs_i.push_back(A->reg_ids[k]);
+		auto build_range = [&](int c_lo, int c_hi, RangeOut &out) {
+			// scratch reused across cells and chunks (no allocation per row)
+			std::vector<int> regs_i, regs, add;
+			struct RowInfo {
+				int r, jb, je;
+			};
+			std::vector<RowInfo> ri;
+			for (int c = c_lo; c < c_hi; ++c) {
+				const int ib = A->start_h[c], ie = A->start_h[c + 1];
+				if (ib == ie) continue;
+				const int cx = c / (d1 * d2), cy = (c / d2) % d1, cz = c % d2;
+				// per row: j range, cost
+				ri.clear();
+				for (int r : rws) {
+					const int4 row = row_pool[D.row_off + r];
+					const int qx = cx + row.x, qy = cy + row.y;
+					if (qx < 0 || qy < 0 || qx >= H.g.dim[0] || qy >= d1) continue;
+					if (D.is_auto && (row.x < 0 || (row.x == 0 && row.y < 0))) continue;
+					int z0 = std::max(cz + row.z, 0), z1 = std::min(cz + row.w, d2 - 1);
+					if (D.is_auto && row.x == 0 && row.y == 0) z0 = std::max(z0, cz);
+					if (z0 > z1) continue;
+					const int base = (qx * d1 + qy) * d2;
+					const int jb = B->start_h[base + z0], je = B->start_h[base + z1 + 1];
+					if (jb == je) continue;
+					ri.push_back({r, jb, je});
 				}
-				size_t g0 = 0;
-				while (g0 < ri.size()) {
-					regs.assign(regs_i.begin(), regs_i.end());
-					unsigned long long mask = 0;
-					double cost = 0;
-					size_t g1 = g0;
-					for (; g1 < ri.size(); ++g1) {
-						if (group_regions) {
-							add.clear();
-							const int4 row = row_pool[D.row_off + ri[g1].r];
-							const int qx = cx + row.x, qy = cy + row.y;
-							const int base = (qx * d1 + qy) * d2;
-							int z0 = std::max(cz + row.z, 0), z1 = std::min(cz + row.w, d2 - 1);
-							if (D.is_auto && row.x == 0 && row.y == 0) z0 = std::max(z0, cz);
-							for (int z = z0; z <= z1; ++z)
-								for (int k = B->reg_off[base + z]; k < B->reg_off[base + z + 1]; ++k)
-									if (B->reg_ids[k] != uf_r && std::find(regs.begin(), regs.end(), B->reg_ids[k]) == regs.end() &&
-									    std::find(add.begin(), add.end(), B->reg_ids[k]) == add.end())
-										add.push_back(B->reg_ids[k]);
-							if ((int)(regs.size() + add.size()) > slot_cap && g1 > g0) break;
-							regs.insert(regs.end(), add.begin(), add.end());
-						}
-						mask |= 1ull << ri[g1].r;
-						cost += (double)ni * (ri[g1].je - ri[g1].jb);
-						n_staged += (uint64_t)(ri[g1].je - ri[g1].jb);
+				if (ri.empty()) continue;
+				// a cell in one region (fp32 path, full histogram in LDS): its
+				// items count every pair in the full histogram, which is also
+				// that region's touching count, and need no slot for it
+				const int uf_r = fast && group_regions && !D.full_global && A->reg_off[c + 1] - A->reg_off[c] == 1
+				                     ? A->reg_ids[A->reg_off[c]] : -1;
+				for (int i0 = ib; i0 < ie; i0 += i_chunk) {
+					const int ni = std::min(i_chunk, ie - i0);
+					// group rows by region budget
+					regs_i.clear();
+					if (group_regions && uf_r < 0) {
+						// regions of this chunk: those present in its cell (a
+						// superset of the chunk's regions)
+						for (int k = A->reg_off[c]; k < A->reg_off[c + 1]; ++k) regs_i.push_back(A->reg_ids[k]);
 					}
-					if (cost * 1.0 >= 4.0e9) return fail(ctx, TWOPCF_E_LIMIT, "work item too large for 32-bit counters");
-					WorkItem W;
-					W.job = q;
-					W.i_begin = i0;
-					W.i_count = ni;
-					W.i_cell = c;
-					W.row_mask = mask;
-					W.slot_base = (int)slot_pool.size();
-					// regs holds every region of the chunk's cell and of the
-					// rows' cells (but the cell's own with uf_r): when they
-					// all fit, every pair's two regions have slots
-					W.all_slotted = group_regions && (int)regs.size() <= slot_cap ? 1 : 0;
-					W.uf_region = uf_r;
-					if ((int)regs.size() > slot_cap) regs.resize(slot_cap);   // the rest: global atomics
-					W.n_slots = group_regions ? (int)regs.size() : 0;
-					if (group_regions) slot_pool.insert(slot_pool.end(), regs.begin(), regs.end());
-					items.push_back(W);
-					item_cost.push_back(cost);
-					n_staged += (uint64_t)ni;
-					g0 = g1;
+					size_t g0 = 0;
+					while (g0 < ri.size()) {
+						regs.assign(regs_i.begin(), regs_i.end());
+						unsigned long long mask = 0;
+						double cost = 0;
+						size_t g1 = g0;
+						for (; g1 < ri.size(); ++g1) {
+							if (group_regions) {
+								add.clear();
+								const int4 row = row_pool[D.row_off + ri[g1].r];
+								const int qx = cx + row.x, qy = cy + row.y;
+								const int base = (qx * d1 + qy) * d2;
+								int z0 = std::max(cz + row.z, 0), z1 = std::min(cz + row.w, d2 - 1);
+								if (D.is_auto && row.x == 0 && row.y == 0) z0 = std::max(z0, cz);
+								for (int z = z0; z <= z1; ++z)
+									for (int k = B->reg_off[base + z]; k < B->reg_off[base + z + 1]; ++k)
+										if (B->reg_ids[k] != uf_r && std::find(regs.begin(), regs.end(), B->reg_ids[k]) == regs.end() &&
+										    std::find(add.begin(), add.end(), B->reg_ids[k]) == add.end())
+											add.push_back(B->reg_ids[k]);
+								if ((int)(regs.size() + add.size()) > slot_cap && g1 > g0) break;
+								regs.insert(regs.end(), add.begin(), add.end());
+							}
+							mask |= 1ull << ri[g1].r;
+							cost += (double)ni * (ri[g1].je - ri[g1].jb);
+							out.n_staged += (uint64_t)(ri[g1].je - ri[g1].jb);
+						}
+						if (cost * 1.0 >= 4.0e9) {
+							out.too_large = true;
+							return;
+						}
+						WorkItem W;
+						W.job = q;
+						W.i_begin = i0;
+						W.i_count = ni;
+						W.i_cell = c;
+						W.row_mask = mask;
+						W.slot_base = (int)out.slots.size();   // within this range's list
+						// regs holds every region of the chunk's cell and of the
+						// rows' cells (but the cell's own with uf_r): when they
+						// all fit, every pair's two regions have slots
+						W.all_slotted = group_regions && (int)regs.size() <= slot_cap ? 1 : 0;
+						W.uf_region = uf_r;
+						if ((int)regs.size() > slot_cap) regs.resize(slot_cap);   // the rest: global atomics
+						W.n_slots = group_regions ? (int)regs.size() : 0;
+						if (group_regions) out.slots.insert(out.slots.end(), regs.begin(), regs.end());
+						out.items.push_back(W);
+						out.cost.push_back(cost);
+						out.n_staged += (uint64_t)ni;
+						g0 = g1;
+					}
 				}
 			}
+		};
+		const int n_thr = std::max(1, std::min<int>({16, (int)std::thread::hardware_concurrency(), n_cell / 256 + 1}));
+		std::vector<RangeOut> outs(n_thr);
+		if (n_thr == 1) {
+			build_range(0, n_cell, outs[0]);
+		} else {
+			std::vector<std::thread> thr;
+			for (int t = 0; t < n_thr; ++t)
+				thr.emplace_back(build_range, (int)((long long)n_cell * t / n_thr), (int)((long long)n_cell * (t + 1) / n_thr),
+				                 std::ref(outs[t]));
+			for (auto &th : thr) th.join();
+		}
+		for (RangeOut &o : outs) {
+			if (o.too_large) return fail(ctx, TWOPCF_E_LIMIT, "work item too large for 32-bit counters");
+			const int sb = (int)slot_pool.size();
+			for (WorkItem &W : o.items) W.slot_base += sb;
+			items.insert(items.end(), o.items.begin(), o.items.end());
+			item_cost.insert(item_cost.end(), o.cost.begin(), o.cost.end());
+			slot_pool.insert(slot_pool.end(), o.slots.begin(), o.slots.end());
+			n_staged += o.n_staged;
 		}
 	}
 	if (slot_pool.empty()) slot_pool.push_back(0);
