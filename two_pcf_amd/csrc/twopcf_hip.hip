@@ -18,10 +18,12 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <functional>
 #include <initializer_list>
 #include <map>
 #include <memory>
@@ -1757,6 +1759,10 @@ struct twopcf_ctx {
 	std::map<std::string, std::unique_ptr<Plan>> plans;
 	std::map<std::string, twopcf_tuning> tuned;   // autotuned choice per job set
 	uint64_t next_cat_id = 1;
+	// temporaries of the gridding sort, kept between catalogues and grown
+	// as needed (every use is on the context stream, in order)
+	void *grid_tmp = nullptr;
+	size_t grid_tmp_bytes = 0;
 };
 
 struct SortedCat {
@@ -1766,13 +1772,12 @@ struct SortedCat {
 	uint64_t *masks = nullptr;
 	int m_stride = 0;
 	int *start = nullptr;
+	void *block = nullptr;   // one device allocation holding every array above
 	std::vector<int> start_h;
 	// regions present in each cell, CSR over cells (n_region > 1 only)
 	std::vector<int> reg_off, reg_ids;
 	void release() {
-		for (void *p : {(void *)x, (void *)y, (void *)z, (void *)nx, (void *)ny, (void *)nz, (void *)w, (void *)iw,
-		                (void *)jk, (void *)masks, (void *)start})
-			if (p) (void)hipFree(p);
+		if (block) (void)hipFree(block);
 		*this = SortedCat();
 	}
 };
@@ -1825,6 +1830,21 @@ int dalloc(twopcf_ctx *ctx, T **p, size_t n) {
 	return 0;
 }
 
+// Arrays laid out one after another in one device allocation, each at a
+// 256-byte boundary: add() them, then place() them at the base address.
+struct Carve {
+	size_t at = 0;
+	std::vector<std::pair<void **, size_t>> parts;
+	template <class T>
+	void add(T **p, size_t count) {
+		parts.emplace_back((void **)p, at);
+		at += (std::max<size_t>(count, 1) * sizeof(T) + 255) & ~size_t(255);
+	}
+	void place(void *base) {
+		for (auto &pt : parts) *pt.first = (char *)base + pt.second;
+	}
+};
+
 // Sort one catalogue into the cell geometry g (cached per geometry).  With
 // may_alloc false only a cached grid is taken (twopcf_count_noalloc).
 int grid_catalog(twopcf_ctx *ctx, twopcf_cat *c, const GridKey &g, SortedCat **out, bool may_alloc = true) {
@@ -1840,35 +1860,53 @@ int grid_catalog(twopcf_ctx *ctx, twopcf_cat *c, const GridKey &g, SortedCat **o
 	SortedCat s;
 	const int64_t n = c->n;
 	const int n_cell = g.dim[0] * g.dim[1] * g.dim[2];
-	// temporaries of the sort, freed on every return
-	struct Temps {
-		std::vector<void *> p;
-		~Temps() {
-			for (void *q : p) (void)hipFree(q);
-		}
-	} tmp;
-	auto talloc = [&](auto **q, size_t count) -> int {
-		int rc_ = dalloc(ctx, q, count);
-		if (!rc_) tmp.p.push_back((void *)*q);
-		return rc_;
-	};
+	const int64_t n_scan = (int64_t)n_cell + 1;
+	const int n_tiles = (int)((n_scan + SORT_BLOCK - 1) / SORT_BLOCK);
+	// Arrays are carved out of one allocation each (the sorted catalogue,
+	// and the sort's temporaries, kept in the context): one hipMalloc per
+	// catalogue instead of one per array.
 	int *cell = nullptr, *tperm = nullptr, *perm = nullptr;
 	unsigned short *zb = nullptr, *tzb = nullptr;
 	unsigned *count = nullptr, *sums = nullptr;
-	const int64_t n_scan = (int64_t)n_cell + 1;
-	const int n_tiles = (int)((n_scan + SORT_BLOCK - 1) / SORT_BLOCK);
-	int rc = 0;
-	if ((rc = talloc(&cell, n)) || (rc = talloc(&zb, n)) || (rc = talloc(&tperm, n)) || (rc = talloc(&tzb, n)) ||
-	    (rc = talloc(&perm, n)) || (rc = talloc(&count, n_scan)) || (rc = talloc(&sums, n_tiles)))
-		return rc;
-	if ((rc = dalloc(ctx, &s.x, n)) || (rc = dalloc(ctx, &s.y, n)) || (rc = dalloc(ctx, &s.z, n)) ||
-	    (rc = dalloc(ctx, &s.nx, n)) || (rc = dalloc(ctx, &s.ny, n)) || (rc = dalloc(ctx, &s.nz, n)) ||
-	    (rc = dalloc(ctx, &s.w, n)) || (rc = dalloc(ctx, &s.iw, n)) || (rc = dalloc(ctx, &s.jk, n)) ||
-	    (rc = dalloc(ctx, &s.masks, (size_t)n * c->m_stride)) ||
-	    (rc = dalloc(ctx, &s.start, (size_t)n_cell + 1))) {
-		s.release();
-		return rc;
+	Carve tc;
+	tc.add(&cell, n);
+	tc.add(&zb, n);
+	tc.add(&tperm, n);
+	tc.add(&tzb, n);
+	tc.add(&perm, n);
+	tc.add(&count, n_scan);
+	tc.add(&sums, n_tiles);
+	if (tc.at > ctx->grid_tmp_bytes) {
+		if (ctx->grid_tmp) (void)hipFree(ctx->grid_tmp);
+		ctx->grid_tmp = nullptr;
+		ctx->grid_tmp_bytes = 0;
+		HIPCK(ctx, hipMalloc(&ctx->grid_tmp, tc.at));
+		ctx->grid_tmp_bytes = tc.at;
 	}
+	tc.place(ctx->grid_tmp);
+	Carve sc;
+	sc.add(&s.x, n);
+	sc.add(&s.y, n);
+	sc.add(&s.z, n);
+	sc.add(&s.nx, n);
+	sc.add(&s.ny, n);
+	sc.add(&s.nz, n);
+	sc.add(&s.w, n);
+	sc.add(&s.iw, n);
+	sc.add(&s.jk, n);
+	sc.add(&s.masks, (size_t)n * c->m_stride);
+	sc.add(&s.start, (size_t)n_cell + 1);
+	HIPCK(ctx, hipMalloc(&s.block, sc.at));
+	sc.place(s.block);
+	// the block is freed on every error return
+	struct Own {
+		SortedCat *s;
+		bool keep = false;
+		~Own() {
+			if (!keep) s->release();
+		}
+	} own{&s};
+	int rc = 0;
 	s.m_stride = c->m_stride;
 	// counting sort by cell (count, scan into the cell starts, scatter), then
 	// by z bucket and catalogue index within each cell
@@ -1876,10 +1914,7 @@ int grid_catalog(twopcf_ctx *ctx, twopcf_cat *c, const GridKey &g, SortedCat **o
 	const int nblk = (int)((n + 255) / 256);
 	auto check = [&]() -> int {
 		hipError_t e = hipGetLastError();
-		if (e != hipSuccess) {
-			s.release();
-			return fail(ctx, TWOPCF_E_HIP, std::string("gridding: ") + hipGetErrorString(e));
-		}
+		if (e != hipSuccess) return fail(ctx, TWOPCF_E_HIP, std::string("gridding: ") + hipGetErrorString(e));
 		return 0;
 	};
 	k_fill_u32<<<(unsigned)((n_scan + 255) / 256), 256, 0, ctx->stream>>>(count, n_scan, 0u);
@@ -1924,6 +1959,7 @@ int grid_catalog(twopcf_ctx *ctx, twopcf_cat *c, const GridKey &g, SortedCat **o
 		s.reg_off[n_cell] = (int)s.reg_ids.size();
 	}
 	auto res = c->grids.emplace(g, s);
+	own.keep = true;
 	*out = &res.first->second;
 	return 0;
 }
@@ -2029,6 +2065,7 @@ int twopcf_ctx_destroy(twopcf_ctx *ctx) {
 		if (ctx->join[k]) (void)hipEventDestroy(ctx->join[k]);
 		if (ctx->side[k]) (void)hipStreamDestroy(ctx->side[k]);
 	}
+	if (ctx->grid_tmp) (void)hipFree(ctx->grid_tmp);
 	if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
 	delete ctx;
 	return 0;
@@ -2458,6 +2495,19 @@ int build_plan(twopcf_ctx *ctx, const twopcf_job *jobs, int n_jobs, int method, 
 		}
 	}
 	auto fz_of = [&](int q) { return fused[q] >= 0 ? FZ_MONO : twin[q] >= 0 ? FZ_TWIN : FZ_NONE; };
+	// Work items are built after this loop, on host threads: each job leaves
+	// a builder of the items of a range of its cells, and the ranges' items,
+	// slot lists and counts are appended in job and cell order, so that the
+	// item list is the one a single thread would build.
+	struct RangeOut {
+		std::vector<WorkItem> items;
+		std::vector<double> cost;
+		std::vector<int> slots;
+		uint64_t n_staged = 0;
+		bool too_large = false;
+	};
+	std::vector<std::function<void(int, int, RangeOut &)>> builders;
+	std::vector<int> builder_cells;
 	for (int q = 0; q < n_jobs; ++q) {
 		HostJob &H = hj[q];
 		Prepared &P = prep[q];
@@ -2765,17 +2815,8 @@ int build_plan(twopcf_ctx *ctx, const twopcf_job *jobs, int n_jobs, int method, 
 		const int d1 = H.g.dim[1], d2 = H.g.dim[2];
 		std::vector<int> rws;
 		for (int r = 0; r < D.n_rows; ++r) rws.push_back(r);
-		// The cells are split into contiguous ranges built on host threads;
-		// each range's items, slot lists and counts are then appended in cell
-		// order, so the item list is the one a single thread would build.
-		struct RangeOut {
-			std::vector<WorkItem> items;
-			std::vector<double> cost;
-			std::vector<int> slots;
-			uint64_t n_staged = 0;
-			bool too_large = false;
-		};
-		auto build_range = [&](int c_lo, int c_hi, RangeOut &out) {
+		builder_cells.push_back(n_cell);
+		builders.emplace_back([=, &H, &D, &row_pool](int c_lo, int c_hi, RangeOut &out) {
 			// scratch reused across cells and chunks (no allocation per row)
 			std::vector<int> regs_i, regs, add;
 			struct RowInfo {
@@ -2868,18 +2909,30 @@ int build_plan(twopcf_ctx *ctx, const twopcf_job *jobs, int n_jobs, int method, 
 					}
 				}
 			}
+		});
+	}
+	{
+		// each job's cells in up to 16 ranges, the ranges pulled by the threads
+		struct Task {
+			int b, lo, hi;
 		};
-		const int n_thr = std::max(1, std::min<int>({16, (int)std::thread::hardware_concurrency(), n_cell / 256 + 1}));
-		std::vector<RangeOut> outs(n_thr);
-		if (n_thr == 1) {
-			build_range(0, n_cell, outs[0]);
-		} else {
-			std::vector<std::thread> thr;
-			for (int t = 0; t < n_thr; ++t)
-				thr.emplace_back(build_range, (int)((long long)n_cell * t / n_thr), (int)((long long)n_cell * (t + 1) / n_thr),
-				                 std::ref(outs[t]));
-			for (auto &th : thr) th.join();
+		std::vector<Task> tasks;
+		for (size_t b = 0; b < builders.size(); ++b) {
+			const int nc = builder_cells[b];
+			const int nr = std::max(1, std::min(16, nc / 256 + 1));
+			for (int t = 0; t < nr; ++t)
+				tasks.push_back({(int)b, (int)((long long)nc * t / nr), (int)((long long)nc * (t + 1) / nr)});
 		}
+		std::vector<RangeOut> outs(tasks.size());
+		std::atomic<size_t> next{0};
+		auto work = [&]() {
+			for (size_t k; (k = next.fetch_add(1)) < tasks.size();) builders[tasks[k].b](tasks[k].lo, tasks[k].hi, outs[k]);
+		};
+		const int n_thr = std::max(1, std::min<int>({16, (int)std::thread::hardware_concurrency(), (int)tasks.size()}));
+		std::vector<std::thread> thr;
+		for (int t = 1; t < n_thr; ++t) thr.emplace_back(work);
+		work();
+		for (auto &th : thr) th.join();
 		for (RangeOut &o : outs) {
 			if (o.too_large) return fail(ctx, TWOPCF_E_LIMIT, "work item too large for 32-bit counters");
 			const int sb = (int)slot_pool.size();
@@ -2964,13 +3017,12 @@ int build_plan(twopcf_ctx *ctx, const twopcf_job *jobs, int n_jobs, int method, 
 	// shortens it 1.5 %.  Angular items are ordered by cost, largest first,
 	// which measured faster for them.
 	if (fast) {
-		std::vector<size_t> ord(items.size());
-		for (size_t k = 0; k < ord.size(); ++k) ord[k] = k;
-		std::stable_sort(ord.begin(), ord.end(), [&](size_t x, size_t y) {
-			const bool ax = hj[items[x].job].j->kind == TWOPCF_ANGULAR, ay = hj[items[y].job].j->kind == TWOPCF_ANGULAR;
-			if (ax != ay) return ax;
-			return ax && item_cost[x] > item_cost[y];
-		});
+		std::vector<size_t> ord, rest;
+		ord.reserve(items.size());
+		for (size_t k = 0; k < items.size(); ++k)
+			(hj[items[k].job].j->kind == TWOPCF_ANGULAR ? ord : rest).push_back(k);
+		std::stable_sort(ord.begin(), ord.end(), [&](size_t x, size_t y) { return item_cost[x] > item_cost[y]; });
+		ord.insert(ord.end(), rest.begin(), rest.end());
 		std::vector<WorkItem> s(items.size());
 		for (size_t k = 0; k < ord.size(); ++k) s[k] = items[ord[k]];
 		if (sample_stride > 1) {
@@ -2986,7 +3038,7 @@ int build_plan(twopcf_ctx *ctx, const twopcf_job *jobs, int n_jobs, int method, 
 		return 0;
 	};
 	if ((rc = h2d(d_jobs, jd_h.data(), sizeof(JobDev) * n_jobs)) ||
-	    (rc = h2d(d_items, items.data(), sizeof(WorkItem) * items.size())) ||
+	    (rc = fast ? 0 : h2d(d_items, items.data(), sizeof(WorkItem) * items.size())) ||   // batched: by class below
 	    (rc = h2d(d_rows, row_pool.data(), sizeof(int4) * row_pool.size())) ||
 	    (rc = h2d(d_slots, slot_pool.data(), sizeof(int) * slot_pool.size())) ||
 	    (rc = h2d(d_tab, tables.data(), sizeof(float) * tables.size()))) {
@@ -3106,30 +3158,45 @@ int build_plan(twopcf_ctx *ctx, const twopcf_job *jobs, int n_jobs, int method, 
 			if (prep[q].jd.full_global) return RG_ANY;
 			return hj[q].j->n_jk > 1 ? RG_JK : RG_NONE;
 		};
-		std::map<std::tuple<int, int, int>, std::vector<WorkItem>> cls;
-		for (auto &W : items)
-			cls[{hj[W.job].j->kind, prep[W.job].weighted ? 1 : 0,
-			     mid_of(W.job) + 2 * fz_of(W.job) + (prep[W.job].jd.mstage > 0 ? 8 : 0) + 16 * rg_of(W.job)}]
-			    .push_back(W);
+		// launch class of each job with items, numbered in key order
+		auto key_of = [&](int q) {
+			return std::make_tuple(hj[q].j->kind, prep[q].weighted ? 1 : 0,
+			                       mid_of(q) + 2 * fz_of(q) + (prep[q].jd.mstage > 0 ? 8 : 0) + 16 * rg_of(q));
+		};
+		std::vector<char> has_items(n_jobs, 0);
+		for (const WorkItem &W : items) has_items[W.job] = 1;
+		std::map<std::tuple<int, int, int>, int> cls;
+		for (int q = 0; q < n_jobs; ++q)
+			if (has_items[q]) cls.emplace(key_of(q), 0);
 		if (cls.size() > MAX_CLASSES) {
 			release();
 			return fail(ctx, TWOPCF_E_LIMIT, "more than 64 launch classes in one call");
 		}
+		int n_cls = 0;
+		for (auto &kv : cls) kv.second = n_cls++;
+		std::vector<int> job_cls(n_jobs, -1);
+		for (int q = 0; q < n_jobs; ++q)
+			if (has_items[q]) job_cls[q] = cls[key_of(q)];
+		// each class's items in queue order
+		std::vector<std::vector<uint32_t>> members(n_cls);
+		for (size_t k = 0; k < items.size(); ++k) members[job_cls[items[k].job]].push_back((uint32_t)k);
 		size_t off = 0;
 		std::vector<WorkItem> flat;
+		flat.reserve(items.size());
 		std::vector<std::tuple<int, int, size_t, size_t, int>> spans;
 		for (auto &kv : cls) {
-			spans.emplace_back(std::get<0>(kv.first), std::get<1>(kv.first), off, kv.second.size(), std::get<2>(kv.first));
-			const size_t n_it = kv.second.size();
+			const std::vector<uint32_t> &mem = members[kv.second];
+			const size_t n_it = mem.size();
+			spans.emplace_back(std::get<0>(kv.first), std::get<1>(kv.first), off, n_it, std::get<2>(kv.first));
 			if (std::get<0>(kv.first) == K_ANGULAR && n_it >= (size_t)NQ) {
 				// angular items are ordered by cost: dealt to the launch's NQ
 				// queue ranges in turn, so that each range runs its items
 				// largest first and the ranges carry about the same cost
 				const size_t base = n_it / NQ, extra = n_it % NQ;
 				for (size_t q = 0; q < (size_t)NQ; ++q)
-					for (size_t j = 0; j < base + (q < extra ? 1 : 0); ++j) flat.push_back(kv.second[j * NQ + q]);
+					for (size_t j = 0; j < base + (q < extra ? 1 : 0); ++j) flat.push_back(items[mem[j * NQ + q]]);
 			} else {
-				flat.insert(flat.end(), kv.second.begin(), kv.second.end());
+				for (uint32_t k : mem) flat.push_back(items[k]);
 			}
 			off += n_it;
 		}
