@@ -43,7 +43,7 @@ FLOP_BY_KIND = {0: 15, 1: 29, 2: 26, 3: 15}
 # kernel source they were measured on: for any other source the line reports
 # no traffic.  Source: profiles/k_count_fp32_jackknife_batch_hbm.md
 KERNEL_SRC = os.path.join(ROOT, "two_pcf_amd", "csrc", "twopcf_hip.hip")
-TRAFFIC_BYTES = {("0ff6f03dde16d334", 1.0, 100, 2, 8, 256): (4.1786e7 * 2 + 2.5050e6) * 1024}
+TRAFFIC_BYTES = {("3786eef705ed7a8e", 1.0, 100, 2, 8, 256): (4.1820e7 * 2 + 2.5050e6) * 1024}
 # algorithmic HBM bytes of the counting kernel: every object a work item
 # reads (its first-side objects and its stencil rows' second-side objects,
 # the library's n_staged) once, at the bytes one object takes in the SoA:

@@ -29,6 +29,7 @@
 #include <memory>
 #include <mutex>
 #include <string>
+#include <system_error>
 #include <thread>
 #include <tuple>
 #include <type_traits>
@@ -2017,40 +2018,62 @@ float round_dn(double v) {
 
 // ================================================================== ABI ==
 
+namespace {
+
+// Runs one entry point's body: a C++ exception (host memory exhausted, a
+// thread that could not start) becomes an error code and message, so that
+// none crosses the C boundary.
+template <class F>
+int guarded(twopcf_ctx *ctx, F &&body) {
+	try {
+		return body();
+	} catch (const std::bad_alloc &) {
+		return fail(ctx, TWOPCF_E_LIMIT, "host memory exhausted");
+	} catch (const std::exception &e) {
+		return fail(ctx, TWOPCF_E_LIMIT, std::string("host error: ") + e.what());
+	} catch (...) {
+		return fail(ctx, TWOPCF_E_LIMIT, "host error");
+	}
+}
+
+}  // namespace
+
 extern "C" {
 
 int twopcf_abi_version(void) { return TWOPCF_ABI_VERSION; }
 
 int twopcf_ctx_create(int device, twopcf_ctx **out) {
-	if (!out) return TWOPCF_E_ARG;
-	*out = nullptr;
-	int n = 0;
-	if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) return TWOPCF_E_NODEV;
-	if (device < 0 || device >= n) return TWOPCF_E_ARG;
-	auto *ctx = new twopcf_ctx();
-	ctx->device = device;
-	hipDeviceProp_t prop;
-	bool ok = hipSetDevice(device) == hipSuccess && hipGetDeviceProperties(&prop, device) == hipSuccess &&
-	          hipStreamCreate(&ctx->stream) == hipSuccess && hipEventCreate(&ctx->ev0) == hipSuccess &&
-	          hipEventCreate(&ctx->ev1) == hipSuccess &&
-	          hipEventCreateWithFlags(&ctx->fork, hipEventDisableTiming) == hipSuccess;
-	for (int k = 0; k < N_SIDE && ok; ++k)
-		ok = hipStreamCreate(&ctx->side[k]) == hipSuccess &&
-		     hipEventCreateWithFlags(&ctx->join[k], hipEventDisableTiming) == hipSuccess;
-	if (!ok) {
-		twopcf_ctx_destroy(ctx);
-		return TWOPCF_E_HIP;
-	}
-	ctx->n_cu = prop.multiProcessorCount;
-	ctx->lds_max = std::max(prop.sharedMemPerBlock, prop.sharedMemPerBlockOptin);
-	ctx->name = prop.gcnArchName;
-	// the no-allocation entry point's scratch on this device (device current)
-	if (resolve_scratch(device) != hipSuccess) {
-		twopcf_ctx_destroy(ctx);
-		return TWOPCF_E_HIP;
-	}
-	*out = ctx;
-	return 0;
+	return guarded(nullptr, [&]() -> int {
+		if (!out) return TWOPCF_E_ARG;
+		*out = nullptr;
+		int n = 0;
+		if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) return TWOPCF_E_NODEV;
+		if (device < 0 || device >= n) return TWOPCF_E_ARG;
+		auto *ctx = new twopcf_ctx();
+		ctx->device = device;
+		hipDeviceProp_t prop;
+		bool ok = hipSetDevice(device) == hipSuccess && hipGetDeviceProperties(&prop, device) == hipSuccess &&
+		          hipStreamCreate(&ctx->stream) == hipSuccess && hipEventCreate(&ctx->ev0) == hipSuccess &&
+		          hipEventCreate(&ctx->ev1) == hipSuccess &&
+		          hipEventCreateWithFlags(&ctx->fork, hipEventDisableTiming) == hipSuccess;
+		for (int k = 0; k < N_SIDE && ok; ++k)
+			ok = hipStreamCreate(&ctx->side[k]) == hipSuccess &&
+			     hipEventCreateWithFlags(&ctx->join[k], hipEventDisableTiming) == hipSuccess;
+		if (!ok) {
+			twopcf_ctx_destroy(ctx);
+			return TWOPCF_E_HIP;
+		}
+		ctx->n_cu = prop.multiProcessorCount;
+		ctx->lds_max = std::max(prop.sharedMemPerBlock, prop.sharedMemPerBlockOptin);
+		ctx->name = prop.gcnArchName;
+		// the no-allocation entry point's scratch on this device (device current)
+		if (resolve_scratch(device) != hipSuccess) {
+			twopcf_ctx_destroy(ctx);
+			return TWOPCF_E_HIP;
+		}
+		*out = ctx;
+		return 0;
+	});
 }
 
 int twopcf_ctx_destroy(twopcf_ctx *ctx) {
@@ -2085,84 +2108,86 @@ int twopcf_device_info(twopcf_ctx *ctx, char *name, int name_len, int *n_cu, int
 }
 
 int twopcf_catalog_upload(twopcf_ctx *ctx, const twopcf_cat_desc *d, twopcf_cat **out) {
-	if (!ctx || !d || !out) return fail(ctx, TWOPCF_E_ARG, "null argument");
-	*out = nullptr;
-	if (d->n < 0 || d->n > INT32_MAX - 1) return fail(ctx, TWOPCF_E_LIMIT, "catalogue size out of range");
-	if (d->n > 0 && (!d->x || !d->y || !d->z || !d->nx || !d->ny || !d->nz))
-		return fail(ctx, TWOPCF_E_ARG, "positions and unit vectors are required");
-	if (d->n_words < 0 || d->n_words > 64 || (d->n_words > 0 && !d->masks) || (d->dither && d->n_words < 1))
-		return fail(ctx, TWOPCF_E_ARG, "bad mask description");
-	HIPCK(ctx, hipSetDevice(ctx->device));
-	auto *c = new twopcf_cat();
-	c->ctx = ctx;
-	c->id = ctx->next_cat_id++;
-	c->n = d->n;
-	c->n_words = d->n_words;
-	c->has_dither = d->dither != nullptr;
-	c->m_stride = d->n_words * (c->has_dither ? 2 : 1);
-	const int64_t n = d->n;
-	// masks on the device: per object the bit mask words, then the dither mask words
-	std::vector<uint64_t> mk;
-	if (c->has_dither) {
-		mk.resize((size_t)n * c->m_stride);
-		for (int64_t i = 0; i < n; ++i) {
-			std::copy(d->masks + i * d->n_words, d->masks + (i + 1) * d->n_words, mk.begin() + i * c->m_stride);
-			std::copy(d->dither + i * d->n_words, d->dither + (i + 1) * d->n_words,
-			          mk.begin() + i * c->m_stride + d->n_words);
-		}
-	}
-	std::vector<float> w(n, 1.0f), iw;
-	std::vector<int> jk(n, 0);
-	for (int k = 0; k < 3; ++k) {
-		c->lo[k] = c->nlo[k] = INFINITY;
-		c->hi[k] = c->nhi[k] = -INFINITY;
-	}
-	const double *P[3] = {d->x, d->y, d->z}, *N[3] = {d->nx, d->ny, d->nz};
-	for (int64_t i = 0; i < n; ++i)
-		for (int k = 0; k < 3; ++k) {
-			if (!std::isfinite(P[k][i])) {
-				delete c;
-				return fail(ctx, TWOPCF_E_ARG, "non-finite position");
+	return guarded(ctx, [&]() -> int {
+		if (!ctx || !d || !out) return fail(ctx, TWOPCF_E_ARG, "null argument");
+		*out = nullptr;
+		if (d->n < 0 || d->n > INT32_MAX - 1) return fail(ctx, TWOPCF_E_LIMIT, "catalogue size out of range");
+		if (d->n > 0 && (!d->x || !d->y || !d->z || !d->nx || !d->ny || !d->nz))
+			return fail(ctx, TWOPCF_E_ARG, "positions and unit vectors are required");
+		if (d->n_words < 0 || d->n_words > 64 || (d->n_words > 0 && !d->masks) || (d->dither && d->n_words < 1))
+			return fail(ctx, TWOPCF_E_ARG, "bad mask description");
+		HIPCK(ctx, hipSetDevice(ctx->device));
+		auto *c = new twopcf_cat();
+		c->ctx = ctx;
+		c->id = ctx->next_cat_id++;
+		c->n = d->n;
+		c->n_words = d->n_words;
+		c->has_dither = d->dither != nullptr;
+		c->m_stride = d->n_words * (c->has_dither ? 2 : 1);
+		const int64_t n = d->n;
+		// masks on the device: per object the bit mask words, then the dither mask words
+		std::vector<uint64_t> mk;
+		if (c->has_dither) {
+			mk.resize((size_t)n * c->m_stride);
+			for (int64_t i = 0; i < n; ++i) {
+				std::copy(d->masks + i * d->n_words, d->masks + (i + 1) * d->n_words, mk.begin() + i * c->m_stride);
+				std::copy(d->dither + i * d->n_words, d->dither + (i + 1) * d->n_words,
+				          mk.begin() + i * c->m_stride + d->n_words);
 			}
-			c->lo[k] = std::min(c->lo[k], P[k][i]);
-			c->hi[k] = std::max(c->hi[k], P[k][i]);
-			c->nlo[k] = std::min(c->nlo[k], N[k][i]);
-			c->nhi[k] = std::max(c->nhi[k], N[k][i]);
 		}
-	if (d->w) std::copy(d->w, d->w + n, w.begin());
-	for (int64_t i = 0; i < n; ++i) c->unit_w &= (w[i] == 1.0f);
-	if (d->jk) std::copy(d->jk, d->jk + n, jk.begin());
-	if (n > 0) {
-		c->jk_min = *std::min_element(jk.begin(), jk.end());
-		c->jk_max = *std::max_element(jk.begin(), jk.end());
-	}
-	c->has_iw = d->iw != nullptr;
-	iw.assign(d->iw ? d->iw : w.data(), (d->iw ? d->iw : w.data()) + n);
-	int rc = 0;
-	if ((rc = dalloc(ctx, &c->x, n)) || (rc = dalloc(ctx, &c->y, n)) || (rc = dalloc(ctx, &c->z, n)) ||
-	    (rc = dalloc(ctx, &c->nx, n)) || (rc = dalloc(ctx, &c->ny, n)) || (rc = dalloc(ctx, &c->nz, n)) ||
-	    (rc = dalloc(ctx, &c->w, n)) || (rc = dalloc(ctx, &c->iw, n)) || (rc = dalloc(ctx, &c->jk, n)) ||
-	    (rc = dalloc(ctx, &c->masks, (size_t)n * c->m_stride))) {
+		std::vector<float> w(n, 1.0f), iw;
+		std::vector<int> jk(n, 0);
+		for (int k = 0; k < 3; ++k) {
+			c->lo[k] = c->nlo[k] = INFINITY;
+			c->hi[k] = c->nhi[k] = -INFINITY;
+		}
+		const double *P[3] = {d->x, d->y, d->z}, *N[3] = {d->nx, d->ny, d->nz};
+		for (int64_t i = 0; i < n; ++i)
+			for (int k = 0; k < 3; ++k) {
+				if (!std::isfinite(P[k][i])) {
+					delete c;
+					return fail(ctx, TWOPCF_E_ARG, "non-finite position");
+				}
+				c->lo[k] = std::min(c->lo[k], P[k][i]);
+				c->hi[k] = std::max(c->hi[k], P[k][i]);
+				c->nlo[k] = std::min(c->nlo[k], N[k][i]);
+				c->nhi[k] = std::max(c->nhi[k], N[k][i]);
+			}
+		if (d->w) std::copy(d->w, d->w + n, w.begin());
+		for (int64_t i = 0; i < n; ++i) c->unit_w &= (w[i] == 1.0f);
+		if (d->jk) std::copy(d->jk, d->jk + n, jk.begin());
+		if (n > 0) {
+			c->jk_min = *std::min_element(jk.begin(), jk.end());
+			c->jk_max = *std::max_element(jk.begin(), jk.end());
+		}
+		c->has_iw = d->iw != nullptr;
+		iw.assign(d->iw ? d->iw : w.data(), (d->iw ? d->iw : w.data()) + n);
+		int rc = 0;
+		if ((rc = dalloc(ctx, &c->x, n)) || (rc = dalloc(ctx, &c->y, n)) || (rc = dalloc(ctx, &c->z, n)) ||
+		    (rc = dalloc(ctx, &c->nx, n)) || (rc = dalloc(ctx, &c->ny, n)) || (rc = dalloc(ctx, &c->nz, n)) ||
+		    (rc = dalloc(ctx, &c->w, n)) || (rc = dalloc(ctx, &c->iw, n)) || (rc = dalloc(ctx, &c->jk, n)) ||
+		    (rc = dalloc(ctx, &c->masks, (size_t)n * c->m_stride))) {
+			ctx->cats.push_back(c);
+			twopcf_catalog_free(c);
+			return rc;
+		}
 		ctx->cats.push_back(c);
-		twopcf_catalog_free(c);
-		return rc;
-	}
-	ctx->cats.push_back(c);
-	auto up = [&](void *dst, const void *src, size_t bytes) -> int {
-		if (bytes) HIPCK(ctx, hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, ctx->stream));
+		auto up = [&](void *dst, const void *src, size_t bytes) -> int {
+			if (bytes) HIPCK(ctx, hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, ctx->stream));
+			return 0;
+		};
+		if ((rc = up(c->x, d->x, n * 8)) || (rc = up(c->y, d->y, n * 8)) || (rc = up(c->z, d->z, n * 8)) ||
+		    (rc = up(c->nx, d->nx, n * 8)) || (rc = up(c->ny, d->ny, n * 8)) || (rc = up(c->nz, d->nz, n * 8)) ||
+		    (rc = up(c->w, w.data(), n * 4)) || (rc = up(c->iw, iw.data(), n * 4)) || (rc = up(c->jk, jk.data(), n * 4)) ||
+		    (rc = up(c->masks, c->has_dither ? (const void *)mk.data() : (const void *)d->masks,
+		             (size_t)n * c->m_stride * 8))) {
+			twopcf_catalog_free(c);
+			return rc;
+		}
+		HIPCK(ctx, hipStreamSynchronize(ctx->stream));
+		*out = c;
 		return 0;
-	};
-	if ((rc = up(c->x, d->x, n * 8)) || (rc = up(c->y, d->y, n * 8)) || (rc = up(c->z, d->z, n * 8)) ||
-	    (rc = up(c->nx, d->nx, n * 8)) || (rc = up(c->ny, d->ny, n * 8)) || (rc = up(c->nz, d->nz, n * 8)) ||
-	    (rc = up(c->w, w.data(), n * 4)) || (rc = up(c->iw, iw.data(), n * 4)) || (rc = up(c->jk, jk.data(), n * 4)) ||
-	    (rc = up(c->masks, c->has_dither ? (const void *)mk.data() : (const void *)d->masks,
-	             (size_t)n * c->m_stride * 8))) {
-		twopcf_catalog_free(c);
-		return rc;
-	}
-	HIPCK(ctx, hipStreamSynchronize(ctx->stream));
-	*out = c;
-	return 0;
+	});
 }
 
 int twopcf_catalog_free(twopcf_cat *c) {
@@ -2925,14 +2950,27 @@ int build_plan(twopcf_ctx *ctx, const twopcf_job *jobs, int n_jobs, int method, 
 		}
 		std::vector<RangeOut> outs(tasks.size());
 		std::atomic<size_t> next{0};
+		std::atomic<bool> failed{false};
 		auto work = [&]() {
-			for (size_t k; (k = next.fetch_add(1)) < tasks.size();) builders[tasks[k].b](tasks[k].lo, tasks[k].hi, outs[k]);
+			try {
+				for (size_t k; (k = next.fetch_add(1)) < tasks.size();) builders[tasks[k].b](tasks[k].lo, tasks[k].hi, outs[k]);
+			} catch (...) {
+				failed = true;   // host memory: the call fails once every thread has joined
+				next = tasks.size();
+			}
 		};
 		const int n_thr = std::max(1, std::min<int>({16, (int)std::thread::hardware_concurrency(), (int)tasks.size()}));
 		std::vector<std::thread> thr;
-		for (int t = 1; t < n_thr; ++t) thr.emplace_back(work);
+		for (int t = 1; t < n_thr; ++t) {
+			try {
+				thr.emplace_back(work);
+			} catch (const std::system_error &) {
+				break;   // fewer threads pull the same tasks
+			}
+		}
 		work();
 		for (auto &th : thr) th.join();
+		if (failed) throw std::bad_alloc();
 		for (RangeOut &o : outs) {
 			if (o.too_large) return fail(ctx, TWOPCF_E_LIMIT, "work item too large for 32-bit counters");
 			const int sb = (int)slot_pool.size();
@@ -3363,60 +3401,62 @@ extern "C" {
 
 int twopcf_count(twopcf_ctx *ctx, const twopcf_job *jobs, int n_jobs, int method, const twopcf_tuning *tuning,
                  twopcf_timing *timing) {
-	const double t_start = now_ms();
-	if (!ctx || (!jobs && n_jobs > 0) || n_jobs < 0) return fail(ctx, TWOPCF_E_ARG, "null argument");
-	if (method != TWOPCF_PLAIN_FP64 && method != TWOPCF_FP32_RECHECK) return fail(ctx, TWOPCF_E_ARG, "bad method");
-	for (int q = 0; q < n_jobs; ++q)
-		if (!jobs[q].a || !jobs[q].full) return fail(ctx, TWOPCF_E_ARG, "job needs a catalogue and a full[] output");
-	HIPCK(ctx, hipSetDevice(ctx->device));
-	// no tuning given: the autotuned choice for this job set, if any
-	std::vector<uint64_t> ids;
-	const std::string jkey = job_set_key(jobs, n_jobs, method, &ids);
-	twopcf_tuning tu{};
-	if (tuning) {
-		tu = *tuning;
-	} else if (method == TWOPCF_FP32_RECHECK) {
-		auto t = ctx->tuned.find(jkey);
-		if (t != ctx->tuned.end()) tu = t->second;
-	}
-	tuning = &tu;
-	// plan cache key: the job set and the tuning
-	std::string key = jkey;
-	key.append((const char *)&tu, sizeof tu);
-	Plan *plan = nullptr;
-	bool built = false;
-	auto it = ctx->plans.find(key);
-	if (it != ctx->plans.end()) {
-		plan = it->second.get();
-	} else {
-		auto np = std::make_unique<Plan>();
-		int rc = build_plan(ctx, jobs, n_jobs, method, tuning, *np);
-		if (rc) return rc;
-		np->cat_ids = ids;
-		// capture the launch sequence once; replays run the whole batch as one graph
-		if (!getenv("TWOPCF_NO_GRAPH")) {
-			hipGraph_t g = nullptr;
-			HIPCK(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
-			rc = enqueue_plan(ctx, *np);
-			hipError_t e = hipStreamEndCapture(ctx->stream, &g);
-			if (rc) return rc;
-			if (e != hipSuccess) return fail(ctx, TWOPCF_E_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
-			np->graph = g;
-			HIPCK(ctx, hipGraphInstantiate(&np->exec, g, nullptr, nullptr, 0));
+	return guarded(ctx, [&]() -> int {
+		const double t_start = now_ms();
+		if (!ctx || (!jobs && n_jobs > 0) || n_jobs < 0) return fail(ctx, TWOPCF_E_ARG, "null argument");
+		if (method != TWOPCF_PLAIN_FP64 && method != TWOPCF_FP32_RECHECK) return fail(ctx, TWOPCF_E_ARG, "bad method");
+		for (int q = 0; q < n_jobs; ++q)
+			if (!jobs[q].a || !jobs[q].full) return fail(ctx, TWOPCF_E_ARG, "job needs a catalogue and a full[] output");
+		HIPCK(ctx, hipSetDevice(ctx->device));
+		// no tuning given: the autotuned choice for this job set, if any
+		std::vector<uint64_t> ids;
+		const std::string jkey = job_set_key(jobs, n_jobs, method, &ids);
+		twopcf_tuning tu{};
+		if (tuning) {
+			tu = *tuning;
+		} else if (method == TWOPCF_FP32_RECHECK) {
+			auto t = ctx->tuned.find(jkey);
+			if (t != ctx->tuned.end()) tu = t->second;
 		}
-		plan = np.get();
-		ctx->plans[key] = std::move(np);
-		built = true;
-	}
-	HIPCK(ctx, hipEventRecord(ctx->ev0, ctx->stream));
-	if (plan->exec) {
-		HIPCK(ctx, hipGraphLaunch(plan->exec, ctx->stream));
-	} else {
-		int rc = enqueue_plan(ctx, *plan);
-		if (rc) return rc;
-	}
-	HIPCK(ctx, hipEventRecord(ctx->ev1, ctx->stream));
-	return read_results(ctx, *plan, jobs, n_jobs, built, t_start, timing);
+		tuning = &tu;
+		// plan cache key: the job set and the tuning
+		std::string key = jkey;
+		key.append((const char *)&tu, sizeof tu);
+		Plan *plan = nullptr;
+		bool built = false;
+		auto it = ctx->plans.find(key);
+		if (it != ctx->plans.end()) {
+			plan = it->second.get();
+		} else {
+			auto np = std::make_unique<Plan>();
+			int rc = build_plan(ctx, jobs, n_jobs, method, tuning, *np);
+			if (rc) return rc;
+			np->cat_ids = ids;
+			// capture the launch sequence once; replays run the whole batch as one graph
+			if (!getenv("TWOPCF_NO_GRAPH")) {
+				hipGraph_t g = nullptr;
+				HIPCK(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+				rc = enqueue_plan(ctx, *np);
+				hipError_t e = hipStreamEndCapture(ctx->stream, &g);
+				if (rc) return rc;
+				if (e != hipSuccess) return fail(ctx, TWOPCF_E_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+				np->graph = g;
+				HIPCK(ctx, hipGraphInstantiate(&np->exec, g, nullptr, nullptr, 0));
+			}
+			plan = np.get();
+			ctx->plans[key] = std::move(np);
+			built = true;
+		}
+		HIPCK(ctx, hipEventRecord(ctx->ev0, ctx->stream));
+		if (plan->exec) {
+			HIPCK(ctx, hipGraphLaunch(plan->exec, ctx->stream));
+		} else {
+			int rc = enqueue_plan(ctx, *plan);
+			if (rc) return rc;
+		}
+		HIPCK(ctx, hipEventRecord(ctx->ev1, ctx->stream));
+		return read_results(ctx, *plan, jobs, n_jobs, built, t_start, timing);
+	});
 }
 
 // twopcf_count without device allocation: the plan is built for this call
@@ -3429,29 +3469,31 @@ int twopcf_count(twopcf_ctx *ctx, const twopcf_job *jobs, int n_jobs, int method
 // share the array and race.
 int twopcf_count_noalloc(twopcf_ctx *ctx, const twopcf_job *jobs, int n_jobs, int method,
                          const twopcf_tuning *tuning, twopcf_timing *timing) {
-	const double t_start = now_ms();
-	if (!ctx || (!jobs && n_jobs > 0) || n_jobs < 0) return fail(ctx, TWOPCF_E_ARG, "null argument");
-	if (method != TWOPCF_PLAIN_FP64 && method != TWOPCF_FP32_RECHECK) return fail(ctx, TWOPCF_E_ARG, "bad method");
-	for (int q = 0; q < n_jobs; ++q)
-		if (!jobs[q].a || !jobs[q].full) return fail(ctx, TWOPCF_E_ARG, "job needs a catalogue and a full[] output");
-	HIPCK(ctx, hipSetDevice(ctx->device));
-	void *scr = noalloc_scratch_ptr(ctx->device);
-	if (!scr) return fail(ctx, TWOPCF_E_ARG, "no scratch array on this device");
-	twopcf_tuning tu{};
-	if (tuning) {
-		tu = *tuning;
-	} else if (method == TWOPCF_FP32_RECHECK) {
-		auto t = ctx->tuned.find(job_set_key(jobs, n_jobs, method, nullptr));
-		if (t != ctx->tuned.end()) tu = t->second;
-	}
-	Arena arena{(char *)scr, NOALLOC_SCRATCH_BYTES, 0};
-	Plan plan;   // owns no device memory and no graph
-	int rc = build_plan(ctx, jobs, n_jobs, method, &tu, plan, 1, &arena);
-	if (rc) return rc;
-	HIPCK(ctx, hipEventRecord(ctx->ev0, ctx->stream));
-	if ((rc = enqueue_plan(ctx, plan))) return rc;
-	HIPCK(ctx, hipEventRecord(ctx->ev1, ctx->stream));
-	return read_results(ctx, plan, jobs, n_jobs, true, t_start, timing);
+	return guarded(ctx, [&]() -> int {
+		const double t_start = now_ms();
+		if (!ctx || (!jobs && n_jobs > 0) || n_jobs < 0) return fail(ctx, TWOPCF_E_ARG, "null argument");
+		if (method != TWOPCF_PLAIN_FP64 && method != TWOPCF_FP32_RECHECK) return fail(ctx, TWOPCF_E_ARG, "bad method");
+		for (int q = 0; q < n_jobs; ++q)
+			if (!jobs[q].a || !jobs[q].full) return fail(ctx, TWOPCF_E_ARG, "job needs a catalogue and a full[] output");
+		HIPCK(ctx, hipSetDevice(ctx->device));
+		void *scr = noalloc_scratch_ptr(ctx->device);
+		if (!scr) return fail(ctx, TWOPCF_E_ARG, "no scratch array on this device");
+		twopcf_tuning tu{};
+		if (tuning) {
+			tu = *tuning;
+		} else if (method == TWOPCF_FP32_RECHECK) {
+			auto t = ctx->tuned.find(job_set_key(jobs, n_jobs, method, nullptr));
+			if (t != ctx->tuned.end()) tu = t->second;
+		}
+		Arena arena{(char *)scr, NOALLOC_SCRATCH_BYTES, 0};
+		Plan plan;   // owns no device memory and no graph
+		int rc = build_plan(ctx, jobs, n_jobs, method, &tu, plan, 1, &arena);
+		if (rc) return rc;
+		HIPCK(ctx, hipEventRecord(ctx->ev0, ctx->stream));
+		if ((rc = enqueue_plan(ctx, plan))) return rc;
+		HIPCK(ctx, hipEventRecord(ctx->ev1, ctx->stream));
+		return read_results(ctx, plan, jobs, n_jobs, true, t_start, timing);
+	});
 }
 
 // Autotuning of the batched method: trials on every 8th item of the
@@ -3460,97 +3502,105 @@ int twopcf_count_noalloc(twopcf_ctx *ctx, const twopcf_job *jobs, int n_jobs, in
 // size, one knob at a time from the defaults.  The choice is kept in the
 // context for this job set and used by twopcf_count when it gets no tuning.
 int twopcf_autotune(twopcf_ctx *ctx, const twopcf_job *jobs, int n_jobs, twopcf_tuning *best, double *best_ms) {
-	if (!ctx || !jobs || n_jobs <= 0) return fail(ctx, TWOPCF_E_ARG, "null argument");
-	for (int q = 0; q < n_jobs; ++q)
-		if (!jobs[q].a || !jobs[q].full) return fail(ctx, TWOPCF_E_ARG, "job needs a catalogue and a full[] output");
-	HIPCK(ctx, hipSetDevice(ctx->device));
-	const int method = TWOPCF_FP32_RECHECK;
-	const int stride = 8;
-	bool any_jk = false;
-	for (int q = 0; q < n_jobs; ++q) any_jk = any_jk || jobs[q].n_jk > 1;
-	auto trial = [&](const twopcf_tuning &t, double *ms) -> int {
-		Plan p;
-		int rc = build_plan(ctx, jobs, n_jobs, method, &t, p, stride);
-		if (rc) return rc;
-		double m = INFINITY;
-		for (int rep = 0; rep < 2; ++rep) {
-			HIPCK(ctx, hipEventRecord(ctx->ev0, ctx->stream));
-			if ((rc = enqueue_plan(ctx, p))) return rc;
-			HIPCK(ctx, hipEventRecord(ctx->ev1, ctx->stream));
-			HIPCK(ctx, hipStreamSynchronize(ctx->stream));
-			float k = 0.0f;
-			HIPCK(ctx, hipEventElapsedTime(&k, ctx->ev0, ctx->ev1));
-			m = std::min(m, (double)k);
-		}
-		*ms = m;
-		return 0;
-	};
-	twopcf_tuning cur{2, 12, ICHUNK};
-	double cur_ms = INFINITY;
-	int rc = trial(cur, &cur_ms);
-	if (rc) return rc;
-	auto try_knob = [&](int32_t twopcf_tuning::*f, std::initializer_list<int> vals) -> int {
-		const twopcf_tuning base = cur;
-		for (int v : vals) {
-			if (v == base.*f) continue;
-			twopcf_tuning t = base;
-			t.*f = v;
-			double ms;
-			int r = trial(t, &ms);
-			if (r == TWOPCF_E_LIMIT) continue;   // not feasible for these jobs (stencil, LDS)
-			if (r) return r;
-			if (ms < 0.99 * cur_ms) {   // a change must gain more than 1 % (timing noise)
-				cur = t;
-				cur_ms = ms;
+	return guarded(ctx, [&]() -> int {
+		if (!ctx || !jobs || n_jobs <= 0) return fail(ctx, TWOPCF_E_ARG, "null argument");
+		for (int q = 0; q < n_jobs; ++q)
+			if (!jobs[q].a || !jobs[q].full) return fail(ctx, TWOPCF_E_ARG, "job needs a catalogue and a full[] output");
+		HIPCK(ctx, hipSetDevice(ctx->device));
+		const int method = TWOPCF_FP32_RECHECK;
+		const int stride = 8;
+		bool any_jk = false;
+		for (int q = 0; q < n_jobs; ++q) any_jk = any_jk || jobs[q].n_jk > 1;
+		auto trial = [&](const twopcf_tuning &t, double *ms) -> int {
+			Plan p;
+			int rc = build_plan(ctx, jobs, n_jobs, method, &t, p, stride);
+			if (rc) return rc;
+			double m = INFINITY;
+			for (int rep = 0; rep < 2; ++rep) {
+				HIPCK(ctx, hipEventRecord(ctx->ev0, ctx->stream));
+				if ((rc = enqueue_plan(ctx, p))) return rc;
+				HIPCK(ctx, hipEventRecord(ctx->ev1, ctx->stream));
+				HIPCK(ctx, hipStreamSynchronize(ctx->stream));
+				float k = 0.0f;
+				HIPCK(ctx, hipEventElapsedTime(&k, ctx->ev0, ctx->ev1));
+				m = std::min(m, (double)k);
 			}
-		}
+			*ms = m;
+			return 0;
+		};
+		twopcf_tuning cur{2, 12, ICHUNK};
+		double cur_ms = INFINITY;
+		int rc = trial(cur, &cur_ms);
+		if (rc) return rc;
+		auto try_knob = [&](int32_t twopcf_tuning::*f, std::initializer_list<int> vals) -> int {
+			const twopcf_tuning base = cur;
+			for (int v : vals) {
+				if (v == base.*f) continue;
+				twopcf_tuning t = base;
+				t.*f = v;
+				double ms;
+				int r = trial(t, &ms);
+				if (r == TWOPCF_E_LIMIT) continue;   // not feasible for these jobs (stencil, LDS)
+				if (r) return r;
+				if (ms < 0.99 * cur_ms) {   // a change must gain more than 1 % (timing noise)
+					cur = t;
+					cur_ms = ms;
+				}
+			}
+			return 0;
+		};
+		ctx->err.clear();
+		if ((rc = try_knob(&twopcf_tuning::cell_subdiv, {1, 3}))) return rc;
+		if (any_jk && (rc = try_knob(&twopcf_tuning::max_slots, {4, 8, 16}))) return rc;
+		if ((rc = try_knob(&twopcf_tuning::i_chunk, {128}))) return rc;
+		ctx->tuned[job_set_key(jobs, n_jobs, method, nullptr)] = cur;
+		if (best) *best = cur;
+		if (best_ms) *best_ms = cur_ms * stride;
 		return 0;
-	};
-	ctx->err.clear();
-	if ((rc = try_knob(&twopcf_tuning::cell_subdiv, {1, 3}))) return rc;
-	if (any_jk && (rc = try_knob(&twopcf_tuning::max_slots, {4, 8, 16}))) return rc;
-	if ((rc = try_knob(&twopcf_tuning::i_chunk, {128}))) return rc;
-	ctx->tuned[job_set_key(jobs, n_jobs, method, nullptr)] = cur;
-	if (best) *best = cur;
-	if (best_ms) *best_ms = cur_ms * stride;
-	return 0;
+	});
 }
 
 // ------------------------------------------------------ host-only helpers --
 
 int twopcf_s2_thresholds(int kind, const twopcf_axis *ax1, const twopcf_axis *ax2, float *t) {
-	if (!ax1 || !t) return TWOPCF_E_ARG;
-	AxisF a1((float)ax1->min, (float)ax1->max, ax1->n_bins, (float)ax1->log_base);
-	AxisF a2 = ax2 ? AxisF((float)ax2->min, (float)ax2->max, ax2->n_bins, (float)ax2->log_base) : a1;
-	if (kind == TWOPCF_SIGMA_PI) {
-		float mn, mx;
-		range_sq(kind, a1, a2, &mn, &mx);
-		t[0] = std::nextafter(mn, INFINITY);
-		t[1] = mx;
+	return guarded(nullptr, [&]() -> int {
+		if (!ax1 || !t) return TWOPCF_E_ARG;
+		AxisF a1((float)ax1->min, (float)ax1->max, ax1->n_bins, (float)ax1->log_base);
+		AxisF a2 = ax2 ? AxisF((float)ax2->min, (float)ax2->max, ax2->n_bins, (float)ax2->log_base) : a1;
+		if (kind == TWOPCF_SIGMA_PI) {
+			float mn, mx;
+			range_sq(kind, a1, a2, &mn, &mx);
+			t[0] = std::nextafter(mn, INFINITY);
+			t[1] = mx;
+			return 0;
+		}
+		if (kind != TWOPCF_MONOPOLE && kind != TWOPCF_S_MU) return TWOPCF_E_ARG;
+		std::vector<float> v;
+		if (!s2_thresholds(kind, a1, a2, v)) return TWOPCF_E_AXIS;
+		std::copy(v.begin(), v.end(), t);
 		return 0;
-	}
-	if (kind != TWOPCF_MONOPOLE && kind != TWOPCF_S_MU) return TWOPCF_E_ARG;
-	std::vector<float> v;
-	if (!s2_thresholds(kind, a1, a2, v)) return TWOPCF_E_AXIS;
-	std::copy(v.begin(), v.end(), t);
-	return 0;
+	});
 }
 
 int twopcf_axis_thresholds(const twopcf_axis *ax, float *t) {
-	if (!ax || !t) return TWOPCF_E_ARG;
-	AxisF a((float)ax->min, (float)ax->max, ax->n_bins, (float)ax->log_base);
-	std::vector<float> v;
-	if (!value_thresholds(a, v)) return TWOPCF_E_AXIS;
-	std::copy(v.begin(), v.end(), t);
-	return 0;
+	return guarded(nullptr, [&]() -> int {
+		if (!ax || !t) return TWOPCF_E_ARG;
+		AxisF a((float)ax->min, (float)ax->max, ax->n_bins, (float)ax->log_base);
+		std::vector<float> v;
+		if (!value_thresholds(a, v)) return TWOPCF_E_AXIS;
+		std::copy(v.begin(), v.end(), t);
+		return 0;
+	});
 }
 
 int twopcf_comoving_distance(double omega_m, double h, double z_min, double z_max, int64_t n, const double *z,
                              double *d) {
-	if ((n > 0 && (!z || !d)) || n < 0) return TWOPCF_E_ARG;
-	Cosmology c(omega_m, h, z_min, z_max);
-	for (int64_t i = 0; i < n; ++i) d[i] = c.comov_dist(z[i]);
-	return 0;
+	return guarded(nullptr, [&]() -> int {
+		if ((n > 0 && (!z || !d)) || n < 0) return TWOPCF_E_ARG;
+		Cosmology c(omega_m, h, z_min, z_max);
+		for (int64_t i = 0; i < n; ++i) d[i] = c.comov_dist(z[i]);
+		return 0;
+	});
 }
 
 int twopcf_eq_to_cart(int64_t n, const double *ra, const double *dec, const double *d, double *x, double *y,
